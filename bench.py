@@ -1,0 +1,165 @@
+"""Benchmark: grid-point-steps/s of the tethered flapping-insect workload
+(512^3, RK4 with explicit diffusion, insect mask and us regenerated in every
+stage) on one MI355X, FP32-mixed, with the FP64 path and the FP64 CPU oracle
+beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512] [--no-cpu] [--no-fp64]
+
+Prints one JSON line (rank 0).  Data are synthetic: the flow starts from rest
+and is driven by the flapping wings (quiescent initial condition), with the
+fixed time step dt = 4e-5 so both precisions take identical steps.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt·steps/s at 512^3, FP32-mixed vs FP64 (force rel err≤1e-4); % HBM roofline"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+B_PT_RK4 = {4: 895.2, 8: 1790.5}   # algorithmic bytes per grid-point-step, RK4 (penalised)
+
+
+def make_ini(tmpdir, n, nt):
+    src = os.path.join(ROOT, "flusi_amd", "configs", "cfg3_insect_512.ini")
+    txt = open(src).read()
+    for k in ("nx", "ny", "nz"):
+        txt = txt.replace(f"{k}=512;", f"{k}={n};")
+    txt = txt.replace("nt=999;", f"nt={nt};")
+    path = os.path.join(tmpdir, "bench_insect.ini")
+    with open(path, "w") as fh:
+        fh.write(txt)
+    import shutil
+    shutil.copy(os.path.join(ROOT, "flusi_amd", "configs", "kin_cfg3.ini"), tmpdir)
+    return path
+
+
+def pass_bytes(n, s):
+    """Algorithmic HBM bytes per launch of each pass at an n^3 grid (complex
+    field C = 2*(n/2+1)*n*n*s bytes, real field R = n^3*s), RK4 stages."""
+    C = 2 * (n // 2 + 1) * n * n * s
+    R = n ** 3 * s
+    return {
+        "ZINV": 8 * C,            # 3 fields in, 5 out
+        "YINV": 11 * C,           # 5 in, 6 out
+        "XPASS": 9 * C + R,       # 6 in, 3 out, mask (us only inside the mask)
+        "YFWD": 6 * C,            # 3 in, 3 out
+        "ZSTEP": 15 * C,          # RK4 average of 12/18/18/12 C
+    }
+
+
+def run_gpu(ini, prec, warmup, steps, timers):
+    """Warm up, then time `steps` steps; flusi_timers synchronises the solver's
+    stream on both sides of the timed region and collects the per-pass HIP
+    event times recorded on that stream."""
+    import flusi_amd
+    s = flusi_amd.Flusi(ini, prec)
+    s.init_fields()
+    s.advance(warmup)
+    s.timers(1 if timers else 0)
+    t0 = time.perf_counter()
+    s.advance(steps)
+    ms = s.timers(-1)
+    t1 = time.perf_counter()
+    integ = s.write_integrals(s.get_time()[0])
+    s.close()
+    return t1 - t0, ms, integ
+
+
+def cpu_baseline(n, workers):
+    """One right-hand side (cal_nlk incl. the insect mask) of the same workload
+    with the FP64 oracle in a child process; steps/s = 1 / (4 t_RHS)."""
+    code = f"""
+import os, sys, time, json
+sys.path.insert(0, {ROOT!r})
+from oracle import flusi_oracle as fo
+import bench
+d = bench.tempfile.mkdtemp()
+ini = bench.make_ini(d, {n}, 0)
+sim = fo.Simulation(ini, workers={workers})
+sim.init_fields()
+t0 = time.perf_counter()
+sim.cal_nlk(0.0, sim.uk)
+t1 = time.perf_counter()
+print(json.dumps({{"t_rhs": t1 - t0}}))
+"""
+    env = dict(os.environ)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=900)
+    if r.returncode != 0:
+        return None, r.stderr[-400:]
+    return json.loads(r.stdout.strip().splitlines()[-1])["t_rhs"], None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+    import flusi_amd
+    n = args.grid
+    N = n ** 3
+    tmp = tempfile.mkdtemp()
+    ini = make_ini(tmp, n, args.warmup + args.steps + 10)
+
+    dt32, ms32, integ32 = run_gpu(ini, flusi_amd.FP32_MIXED, args.warmup, args.steps, True)
+    gps32 = N * args.steps / dt32
+    out = {
+        "metric": METRIC, "value": gps32, "unit": "grid-pt·steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt32 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"tethered flapping insect {n}^3, RK4, mask+us per stage, FP32-mixed",
+                   "grid": [n, n, n], "stepper": "RK4", "dt": 4e-5, "eta": 1e-4, "nu": 1.4e-3},
+    }
+    # roofline of the dominant pass (per launch: 4 launches of each pass per RK4 step)
+    pb = pass_bytes(n, 4)
+    launches = 4 * args.steps
+    best = max((k for k in pb), key=lambda k: ms32.get(k, 0.0))
+    avg_s = ms32[best] / launches / 1e3
+    ach = pb[best] / avg_s
+    out["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK, "traffic": None, "kernel": best,
+                       "algorithmic_bytes_per_launch": pb[best], "avg_launch_ms": avg_s * 1e3}
+    out["pass_ms_per_step"] = {k: v / args.steps for k, v in ms32.items() if v > 0}
+    out["step_roofline_frac"] = B_PT_RK4[4] * gps32 / HBM_PEAK
+
+    if not args.no_fp64:
+        dt64, ms64, integ64 = run_gpu(ini, flusi_amd.FP64, args.warmup, args.steps, False)
+        out["fp64_value"] = N * args.steps / dt64
+        out["fp32_over_fp64"] = gps32 / out["fp64_value"]
+        f32, f64 = integ32[0:3], integ64[0:3]
+        rel = max(abs(a - b) for a, b in zip(f32, f64)) / max(1e-300, max(abs(b) for b in f64))
+        out["force_rel_err_vs_fp64"] = rel
+        out["forces_fp64"] = list(f64)
+        out["forces_fp32"] = list(f32)
+
+    if not args.no_cpu:
+        workers = min(16, os.cpu_count() or 1)
+        t_rhs, err = cpu_baseline(n, workers)
+        if t_rhs:
+            out["cpu_baseline"] = {"value": N / (4.0 * t_rhs), "unit": "grid-pt·steps/s", "cores": workers,
+                                   "kind": "port",
+                                   "sample": f"one FP64 right-hand side (cal_nlk with insect mask) of the {n}^3 "
+                                             f"workload by the numpy/scipy oracle, t_RHS={t_rhs:.2f}s; "
+                                             f"steps/s = 1/(4 t_RHS)"}
+        else:
+            out["cpu_baseline"] = {"value": None, "error": err}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,248 @@
+"""flusi_amd -- MI355X implementation of FLUSI's fluid hot path.
+
+Python binding over the C ABI in ``include/flusi_hip.h`` (libflusi_hip.so,
+built in-tree by ``flusi_amd.build``).  Method names follow the reference's
+routines (``cal_nlk``, ``fluid_timestep``, ``create_mask``, ``cal_drag`` via
+``write_integrals``, ``cal_vis``, ``fft``/``ifft``, ``compute_energies_k`` as
+``energy_k``).
+
+Array conventions are the reference's single-rank layouts written as C-order
+numpy shapes: physical fields ``(3, nz, ny, nx)`` float64, spectral fields
+``(3, nx//2+1, ny, nz)`` complex128.  There is no CPU fallback: importing
+works without a GPU, but constructing a ``Flusi`` needs the built library and
+a visible MI355X.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+__all__ = ["Flusi", "FlusiError", "FP64", "FP32_MIXED", "load_library", "INTEGRALS_NV", "EXPORTED"]
+
+FP64 = 0
+FP32_MIXED = 1
+INTEGRALS_NV = 61
+TIMER_SLOTS = 8
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libflusi_hip.so")
+
+# name -> (restype, argtypes)
+_D = ctypes.POINTER(ctypes.c_double)
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+EXPORTED: Dict[str, tuple] = {
+    "flusi_init": (_I, [ctypes.c_char_p, _I, ctypes.POINTER(_P)]),
+    "flusi_free": (None, [_P]),
+    "flusi_last_error": (ctypes.c_char_p, [_P]),
+    "flusi_get_dims": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "flusi_set_output_dir": (_I, [_P, ctypes.c_char_p]),
+    "flusi_init_fields": (_I, [_P]),
+    "flusi_set_uk": (_I, [_P, _D]),
+    "flusi_get_uk": (_I, [_P, _D]),
+    "flusi_get_u": (_I, [_P, _D]),
+    "flusi_get_vorticity": (_I, [_P, _D]),
+    "flusi_cal_nlk": (_I, [_P, ctypes.c_double, _I, _D, _D]),
+    "flusi_fluid_timestep": (_I, [_P, _D, ctypes.POINTER(_I), _D, _D]),
+    "flusi_step": (_I, [_P]),
+    "flusi_run": (_I, [_P, _I]),
+    "flusi_advance": (_I, [_P, _I]),
+    "flusi_get_time": (_I, [_P, _D, ctypes.POINTER(_I), _D, _D]),
+    "flusi_create_mask": (_I, [_P, ctypes.c_double]),
+    "flusi_get_mask": (_I, [_P, _D, _D, ctypes.POINTER(ctypes.c_int16)]),
+    "flusi_write_integrals": (_I, [_P, ctypes.c_double, _D]),
+    "flusi_energy_k": (_I, [_P, _D]),
+    "flusi_cal_vis": (_I, [_P, ctypes.c_double, _D]),
+    "flusi_fft": (_I, [_P, _D, _D]),
+    "flusi_ifft": (_I, [_P, _D, _D]),
+    "flusi_timers": (_I, [_P, _I, _D, ctypes.POINTER(ctypes.c_char_p)]),
+}
+
+_lib = None
+
+
+def load_library(path: Optional[str] = None):
+    """Load libflusi_hip.so and declare every exported entry point.  Raises
+    if the library has not been built (there is no fallback path)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -m flusi_amd.build` "
+                           "(the HIP path has no CPU fallback)")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in EXPORTED.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class FlusiError(RuntimeError):
+    """A non-zero return of the C ABI: the reference's abort code + message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"flusi error {code}: {msg}")
+        self.code = code
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+class Flusi:
+    """One solver context (one GPU, one stream) for a FLUSI .ini file."""
+
+    def __init__(self, ini_path: str, precision: int = FP64, output_dir: Optional[str] = None):
+        self._lib = load_library()
+        self._ctx = _P()
+        rc = self._lib.flusi_init(os.fsencode(ini_path), int(precision), ctypes.byref(self._ctx))
+        if rc != 0:
+            msg = self._lib.flusi_last_error(self._ctx).decode() if self._ctx else ""
+            if self._ctx:
+                self._lib.flusi_free(self._ctx)
+                self._ctx = _P()
+            raise FlusiError(rc, msg)
+        nx, ny, nz = _I(), _I(), _I()
+        self._check(self._lib.flusi_get_dims(self._ctx, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
+        self.nx, self.ny, self.nz = nx.value, ny.value, nz.value
+        self.nxh = self.nx // 2 + 1
+        self.precision = precision
+        if output_dir:
+            self.set_output_dir(output_dir)
+
+    # -- plumbing -----------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != 0:
+            raise FlusiError(rc, self._lib.flusi_last_error(self._ctx).decode())
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.flusi_free(self._ctx)
+            self._ctx = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def spectral_shape(self):
+        return (3, self.nxh, self.ny, self.nz)
+
+    @property
+    def physical_shape(self):
+        return (3, self.nz, self.ny, self.nx)
+
+    # -- API ------------------------------------------------------------------
+    def set_output_dir(self, d: Optional[str]):
+        self._check(self._lib.flusi_set_output_dir(self._ctx, os.fsencode(d) if d else None))
+
+    def init_fields(self):
+        self._check(self._lib.flusi_init_fields(self._ctx))
+
+    def set_uk(self, uk: np.ndarray):
+        a = np.ascontiguousarray(uk, dtype=np.complex128).reshape(self.spectral_shape)
+        self._check(self._lib.flusi_set_uk(self._ctx, _dp(a.view(np.float64))))
+
+    def get_uk(self) -> np.ndarray:
+        a = np.empty(self.spectral_shape, dtype=np.complex128)
+        self._check(self._lib.flusi_get_uk(self._ctx, _dp(a.view(np.float64))))
+        return a
+
+    def get_u(self) -> np.ndarray:
+        a = np.empty(self.physical_shape)
+        self._check(self._lib.flusi_get_u(self._ctx, _dp(a)))
+        return a
+
+    def get_vorticity(self) -> np.ndarray:
+        a = np.empty(self.physical_shape)
+        self._check(self._lib.flusi_get_vorticity(self._ctx, _dp(a)))
+        return a
+
+    def cal_nlk(self, time: float, it: int = 0, want_u: bool = False):
+        nlk = np.empty(self.spectral_shape, dtype=np.complex128)
+        u = np.empty(self.physical_shape) if want_u else None
+        self._check(self._lib.flusi_cal_nlk(self._ctx, float(time), int(it), _dp(nlk.view(np.float64)),
+                                            _dp(u) if u is not None else None))
+        return (nlk, u) if want_u else nlk
+
+    def fluid_timestep(self, time: float, it: int, dt0: float) -> float:
+        t, i, d0, d1 = ctypes.c_double(time), _I(it), ctypes.c_double(dt0), ctypes.c_double(0.0)
+        self._check(self._lib.flusi_fluid_timestep(self._ctx, ctypes.byref(t), ctypes.byref(i),
+                                                   ctypes.byref(d0), ctypes.byref(d1)))
+        return d1.value
+
+    def step(self):
+        self._check(self._lib.flusi_step(self._ctx))
+
+    def run(self, nsteps: int = 0):
+        self._check(self._lib.flusi_run(self._ctx, int(nsteps)))
+
+    def advance(self, nsteps: int):
+        self._check(self._lib.flusi_advance(self._ctx, int(nsteps)))
+
+    def get_time(self):
+        t, i, d0, d1 = ctypes.c_double(), _I(), ctypes.c_double(), ctypes.c_double()
+        self._check(self._lib.flusi_get_time(self._ctx, ctypes.byref(t), ctypes.byref(i),
+                                             ctypes.byref(d0), ctypes.byref(d1)))
+        return t.value, i.value, d0.value, d1.value
+
+    def create_mask(self, time: float):
+        self._check(self._lib.flusi_create_mask(self._ctx, float(time)))
+
+    def get_mask(self):
+        n = self.nx * self.ny * self.nz
+        mask = np.empty((self.nz, self.ny, self.nx))
+        us = np.empty((3, self.nz, self.ny, self.nx))
+        col = np.empty((self.nz, self.ny, self.nx), dtype=np.int16)
+        self._check(self._lib.flusi_get_mask(self._ctx, _dp(mask), _dp(us),
+                                             col.ctypes.data_as(ctypes.POINTER(ctypes.c_int16))))
+        assert mask.size == n
+        return mask, us, col
+
+    def write_integrals(self, time: float) -> np.ndarray:
+        out = np.zeros(INTEGRALS_NV)
+        self._check(self._lib.flusi_write_integrals(self._ctx, float(time), _dp(out)))
+        return out
+
+    def energy_k(self) -> float:
+        e = ctypes.c_double()
+        self._check(self._lib.flusi_energy_k(self._ctx, ctypes.byref(e)))
+        return e.value
+
+    def cal_vis(self, dt: float) -> np.ndarray:
+        v = np.empty((self.nxh, self.ny, self.nz))
+        self._check(self._lib.flusi_cal_vis(self._ctx, float(dt), _dp(v)))
+        return v
+
+    def fft(self, f: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(self.nz, self.ny, self.nx)
+        out = np.empty((self.nxh, self.ny, self.nz), dtype=np.complex128)
+        self._check(self._lib.flusi_fft(self._ctx, _dp(a), _dp(out.view(np.float64))))
+        return out
+
+    def ifft(self, fk: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(fk, dtype=np.complex128).reshape(self.nxh, self.ny, self.nz)
+        out = np.empty((self.nz, self.ny, self.nx))
+        self._check(self._lib.flusi_ifft(self._ctx, _dp(a.view(np.float64)), _dp(out)))
+        return out
+
+    def timers(self, enable: int = -1):
+        ms = np.zeros(TIMER_SLOTS)
+        names = (ctypes.c_char_p * TIMER_SLOTS)()
+        self._check(self._lib.flusi_timers(self._ctx, int(enable), _dp(ms), names))
+        return {names[i].decode(): float(ms[i]) for i in range(TIMER_SLOTS)}

@@ -1,0 +1,87 @@
+// Point-wise field kernels (masks, initial conditions, spectral helpers,
+// diagnostics) and their argument blocks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flusi {
+
+struct GridGeom {
+  int nx, ny, nz, nxh;
+  long npts, ncplx;
+  double dx, dy, dz, xl, yl, zl;
+  int periodic;
+};
+
+// geometry/hyd/draw_sphere.f90:2-43
+struct SphereArgs { double x0, y0, z0, radius, smoothing, safety, color_thr; };
+
+// geometry/add_cavity.f90:8-111 (iCavity = yes | freestream)
+struct CavityArgs { int size; double ux, uy, uz; };
+
+// One insect mask update: module_insects.f90:338-399 Draw_Insect, executed
+// point by point (every stage of the reference is local to a grid point).
+struct InsectArgs {
+  int full_reset;           // delete_old_mask: reset everything (else wing colours only)
+  int draw_body;            // body drawn in this call (once for a tethered insect)
+  int body_type;            // 1 = jerry ellipsoid + head/eyes, 2 = ellipsoid only
+  double xc[3];             // body centre (global)
+  double Mb[9];             // body rotation (row-major, global -> body)
+  double b_body, L_body, smooth, safety;
+  int nspheres;             // head and eyes (jerry)
+  double sph_c[3][3], sph_r[3];
+  int sph_lo[3][3], sph_hi[3][3];  // index bounding boxes [sphere][axis]
+  int wing_shape;           // 1 = TwoEllipses, 2 = rectangular
+  double L_span, b_top, b_bot, thickness;
+  int draw_wing[2];         // right, left
+  double Mw[2][9];          // wing rotation (body -> wing)
+  double piv[2][3];         // pivots in the body frame
+  double rot[2][3];         // relative wing angular velocity in the wing frame
+  int color_w[2];           // 3 (right), 2 (left)
+  double vb[3];             // M_body * v_body
+  double rot_body_b[3];
+};
+
+// drag.f90:17-454 + integrals.f90:48-238 reductions over physical u, w, div
+struct DiagArgs {
+  const void* u;  const void* w;  const void* div;   // physical fields (pass element type)
+  const void* mask; const void* us; const uint8_t* color;
+  double eps_inv;
+  double x0[3];                    // torque reference point
+  double piv_l[3], piv_r[3];       // insect pivots (global frame)
+  int insect;                      // colours 2/3 use the pivots
+  double* partial;                 // per block DIAG_NV doubles
+};
+enum {
+  DG_FORCE = 0,          // [4 colours][3]
+  DG_TORQUE0 = 12,       // [4][3] about (x0,y0,z0)
+  DG_TORQUE = 24,        // [4][3] about pivots for wing colours
+  DG_POWER = 36,         // [3]
+  DG_URES = 39,          // [4] min |u-us|^2 inside the mask
+  DG_EK = 43,            // [3] sum u_i^2
+  DG_EKF = 46,           // [3] sum ((1-mask) u_i)^2
+  DG_DISS = 49,          // [3] sum w_i^2
+  DG_DISSF = 52,         // [3]
+  DG_DIVMAX = 55,        // max div
+  DG_DIVMAXF = 56,       // max div (1-mask)
+  DG_MASKVOL = 57,       // sum mask
+  DG_MASKUS = 58,        // [3] sum mask us_i
+  DIAG_NV = 61
+};
+
+template <typename T> void launch_sphere(const GridGeom& g, const SphereArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s);
+template <typename T> void launch_cavity(const GridGeom& g, const CavityArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s);
+template <typename T> void launch_insect(const GridGeom& g, const InsectArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s);
+template <typename T> void launch_vortex_ring(const GridGeom& g, double x0, double y0, double z0, double r00, double a, double gamma0, void* vort, hipStream_t s);
+// spectral helpers on S-layout fields (3 components, component stride ncplx)
+template <typename T> void launch_biot_savart(const GridGeom& g, const double* kx, const double* ky, const double* kz, const void* vk, void* uk, hipStream_t s);
+template <typename T> void launch_divergence(const GridGeom& g, const double* kx, const double* ky, const double* kz, const void* uk, void* dk, hipStream_t s);
+template <typename T> void launch_set_mode0(const GridGeom& g, void* uk, int comp, double re, double im, hipStream_t s);
+template <typename T> int diag_blocks(const GridGeom& g);
+template <typename T> void launch_diag(const GridGeom& g, const DiagArgs& a, hipStream_t s);
+// precision conversion host-layout <-> device arrays
+template <typename T> void launch_from_f64(const double* src, void* dst, long n, hipStream_t s);
+template <typename T> void launch_to_f64(const void* src, double* dst, long n, hipStream_t s);
+void launch_zero(void* p, size_t bytes, hipStream_t s);
+
+}  // namespace flusi

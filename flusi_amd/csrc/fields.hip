@@ -1,0 +1,449 @@
+// Point-wise kernels: obstacle masks, initial vorticity, spectral helpers,
+// the output-time diagnostics reduction and the deterministic row reducer.
+#include "fields.h"
+#include "flusi_kernels.h"
+#include "fft_core.h"
+#include <stdexcept>
+#include <vector>
+
+namespace flusi {
+
+static constexpr double PI_D = 3.14159265358979323846264338327950288;
+
+__device__ __forceinline__ int color_clamp(uint8_t c) { return c < 4 ? (int)c : 4; }
+
+__device__ __forceinline__ double steps_d(double x, double t, double h) {
+  // module_insects.f90:408-422 steps / mask.f90:68-90 smoothstep (cosine)
+  if (x <= t - h) return 1.0;
+  if ((t - h) < x && x < (t + h)) return 0.5 * (1.0 + cos((x - t + h) * PI_D / (2.0 * h)));
+  return 0.0;
+}
+
+__device__ __forceinline__ void periodize_d(double* x, const GridGeom& g) {
+  // vars.f90:468-484
+  if (!g.periodic) return;
+  const double box[3] = {g.xl, g.yl, g.zl};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double v = x[i];
+    if (v < -box[i] / 2.0) x[i] = v + box[i];
+    if (v > box[i] / 2.0) x[i] = v - box[i];
+  }
+}
+
+__device__ __forceinline__ void matvec(const double* M, const double* v, double* o) {
+  o[0] = M[0] * v[0] + M[1] * v[1] + M[2] * v[2];
+  o[1] = M[3] * v[0] + M[4] * v[1] + M[5] * v[2];
+  o[2] = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
+}
+__device__ __forceinline__ void matTvec(const double* M, const double* v, double* o) {
+  o[0] = M[0] * v[0] + M[3] * v[1] + M[6] * v[2];
+  o[1] = M[1] * v[0] + M[4] * v[1] + M[7] * v[2];
+  o[2] = M[2] * v[0] + M[5] * v[1] + M[8] * v[2];
+}
+
+// ------------------------------------------------------------------ sphere
+template <typename T>
+__global__ void sphere_k(GridGeom g, SphereArgs a, T* mask, T* us, uint8_t* color) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.npts) return;
+  const int ix = p % g.nx, iy = (p / g.nx) % g.ny, iz = p / ((long)g.nx * g.ny);
+  const double x = ix * g.dx, y = iy * g.dy, z = iz * g.dz;
+  const double R = sqrt((x - a.x0) * (x - a.x0) + (y - a.y0) * (y - a.y0) + (z - a.z0) * (z - a.z0));
+  double m = 0.0; uint8_t c = 0;
+  if (R <= a.radius + a.safety) {
+    m = steps_d(R, a.radius, a.smoothing);
+    if (m > a.color_thr) c = 1;
+  }
+  mask[p] = (T)m; color[p] = c;
+  us[p] = T(0); us[g.npts + p] = T(0); us[2 * g.npts + p] = T(0);
+}
+
+template <typename T>
+void launch_sphere(const GridGeom& g, const SphereArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(sphere_k<T>, dim3((g.npts + nt - 1) / nt), dim3(nt), 0, s, g, a, (T*)mask, (T*)us, color);
+}
+
+// ------------------------------------------------------------------ cavity
+template <typename T>
+__global__ void cavity_k(GridGeom g, CavityArgs a, T* mask, T* us, uint8_t* color) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.npts) return;
+  const int ix = p % g.nx, iy = (p / g.nx) % g.ny, iz = p / ((long)g.nx * g.ny);
+  const int n = a.size;
+  if (ix <= n - 1 || ix >= g.nx - n || iy <= n - 1 || iy >= g.ny - n || iz <= n - 1 || iz >= g.nz - n) {
+    mask[p] = T(1); color[p] = 0;
+    us[p] = (T)a.ux; us[g.npts + p] = (T)a.uy; us[2 * g.npts + p] = (T)a.uz;
+  }
+}
+template <typename T>
+void launch_cavity(const GridGeom& g, const CavityArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(cavity_k<T>, dim3((g.npts + nt - 1) / nt), dim3(nt), 0, s, g, a, (T*)mask, (T*)us, color);
+}
+
+// ------------------------------------------------------------------ insect
+// module_insects.f90:338-399 Draw_Insect with delete_old_mask (:844-883),
+// draw_body_jerry (body_geometry.f90:511-595), drawsphere (:1657-1724),
+// draw_wing_twoellipses / _rectangular (wings_geometry.f90:527-712) and the
+// stage II body-motion transform (wings_geometry.f90:71-108).
+template <typename T>
+__global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* color) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.npts) return;
+  const int ix = p % g.nx, iy = (p / g.nx) % g.ny, iz = p / ((long)g.nx * g.ny);
+  const uint8_t c_old = color[p];
+  const bool old_wing = (c_old == 2 || c_old == 3);
+  double m;
+  uint8_t c;
+  bool mask_loaded = false;
+  if (a.full_reset) { m = 0.0; c = 0; mask_loaded = true; }
+  else if (old_wing) { m = 0.0; c = 0; mask_loaded = true; }
+  else { m = 0.0; c = c_old; }
+  double u_s[3] = {0.0, 0.0, 0.0};
+  double xg[3] = {ix * g.dx - a.xc[0], iy * g.dy - a.xc[1], iz * g.dz - a.xc[2]};
+  periodize_d(xg, g);
+  double xb[3];
+  matvec(a.Mb, xg, xb);
+  bool changed = a.full_reset || old_wing;
+
+  if (a.draw_body) {
+    const double ab = a.L_body / 2.0, b = a.b_body;
+    if (fabs(xb[1]) <= b + a.safety && fabs(xb[2]) <= b + a.safety && fabs(xb[0]) < a.L_body / 2 + a.safety) {
+      const double rx = xb[0] / ab;
+      if (rx * rx <= 1.0) {
+        const double R = sqrt(xb[1] * xb[1] + xb[2] * xb[2]);
+        const double R0 = sqrt(b * b * (1.0 - rx * rx));
+        if (R < R0 + a.safety) {
+          if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
+          m = fmax(steps_d(R, R0, a.smooth), m);
+          c = 1; changed = true;
+        }
+      }
+    }
+    for (int sidx = 0; sidx < a.nspheres; ++sidx) {
+      if (ix < a.sph_lo[sidx][0] || ix > a.sph_hi[sidx][0] || iy < a.sph_lo[sidx][1] || iy > a.sph_hi[sidx][1] ||
+          iz < a.sph_lo[sidx][2] || iz > a.sph_hi[sidx][2]) continue;
+      double xs[3] = {ix * g.dx - a.sph_c[sidx][0], iy * g.dy - a.sph_c[sidx][1], iz * g.dz - a.sph_c[sidx][2]};
+      periodize_d(xs, g);
+      const double R = sqrt(xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2]);
+      if (R <= a.sph_r[sidx] + a.safety) {
+        if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
+        const double tmp = steps_d(R, a.sph_r[sidx], a.smooth);
+        if (tmp >= m) { m = tmp; c = 1; changed = true; }
+      }
+    }
+  }
+
+  for (int w = 0; w < 2; ++w) {
+    if (!a.draw_wing[w]) continue;
+    const double d[3] = {xb[0] - a.piv[w][0], xb[1] - a.piv[w][1], xb[2] - a.piv[w][2]};
+    double xw[3];
+    matvec(a.Mw[w], d, xw);
+    if (!(xw[1] >= -a.safety && xw[1] <= a.L_span + a.safety)) continue;
+    if (!(fabs(xw[2]) <= 0.5 * a.thickness + a.safety)) continue;
+    double x_top, x_bot;
+    if (a.wing_shape == 1) {
+      const double ab = 0.5 * a.L_span;
+      const double q = 1.0 - ((xw[1] - ab) * (xw[1] - ab)) / (ab * ab);
+      if (q >= 0.0) { x_top = sqrt((a.b_top * a.b_top) * q); x_bot = -sqrt((a.b_bot * a.b_bot) * q); }
+      else { x_top = 0.0; x_bot = 0.0; }
+    } else {
+      x_top = a.b_top; x_bot = -a.b_bot;
+    }
+    if (!(xw[0] > x_bot - a.safety && xw[0] < x_top + a.safety)) continue;
+    const double yt = (xw[1] < 0.0) ? steps_d(-xw[1], 0.0, a.smooth) : steps_d(xw[1], a.L_span, a.smooth);
+    const double zt = steps_d(fabs(xw[2]), 0.5 * a.thickness, a.smooth);
+    const double xt = (xw[0] < 0.0) ? steps_d(-xw[0], -x_bot, a.smooth) : steps_d(xw[0], x_top, a.smooth);
+    const double mt = zt * yt * xt;
+    if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
+    if (m < mt && mt > 0.0) {
+      m = mt; c = (uint8_t)a.color_w[w]; changed = true;
+      const double* r = a.rot[w];
+      const double v[3] = {r[1] * xw[2] - r[2] * xw[1], r[2] * xw[0] - r[0] * xw[2], r[0] * xw[1] - r[1] * xw[0]};
+      matTvec(a.Mw[w], v, u_s);
+    }
+  }
+  const bool new_wing = (c == 2 || c == 3);
+  if (new_wing && m > 0.0) {
+    const double v[3] = {a.vb[0] + a.rot_body_b[1] * xb[2] - a.rot_body_b[2] * xb[1],
+                         a.vb[1] + a.rot_body_b[2] * xb[0] - a.rot_body_b[0] * xb[2],
+                         a.vb[2] + a.rot_body_b[0] * xb[1] - a.rot_body_b[1] * xb[0]};
+    const double t[3] = {u_s[0] + v[0], u_s[1] + v[1], u_s[2] + v[2]};
+    matTvec(a.Mb, t, u_s);
+  }
+  if (changed) { mask[p] = (T)m; color[p] = c; }
+  if (a.full_reset || old_wing || new_wing) {
+    us[p] = (T)u_s[0]; us[g.npts + p] = (T)u_s[1]; us[2 * g.npts + p] = (T)u_s[2];
+  }
+}
+
+template <typename T>
+void launch_insect(const GridGeom& g, const InsectArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(insect_k<T>, dim3((g.npts + nt - 1) / nt), dim3(nt), 0, s, g, a, (T*)mask, (T*)us, color);
+}
+
+// ------------------------------------------------------------- vortex ring
+// inicond/hyd/init_fields_fsi.f90:381-435 (vorticity in physical space)
+template <typename T>
+__global__ void vortex_ring_k(GridGeom g, double x0, double y0, double z0, double r00, double a, double gamma0, T* vort) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.npts) return;
+  const int ix = p % g.nx, iy = (p / g.nx) % g.ny, iz = p / ((long)g.nx * g.ny);
+  const double x = ix * g.dx, y = iy * g.dy, z = iz * g.dz;
+  const double r = sqrt((y - y0) * (y - y0) + (z - z0) * (z - z0));
+  const double om = (gamma0 / (PI_D * a * a)) * exp(-((x - x0) * (x - x0) + (r - r00) * (r - r00)) / (a * a));
+  double wy = 0.0, wz = 0.0;
+  if (fabs(r) > 1.0e-12) { wy = -om * ((z - z0) / r); wz = om * ((y - y0) / r); }
+  vort[p] = T(0); vort[g.npts + p] = (T)wy; vort[2 * g.npts + p] = (T)wz;
+}
+template <typename T>
+void launch_vortex_ring(const GridGeom& g, double x0, double y0, double z0, double r00, double a, double gamma0, void* vort, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(vortex_ring_k<T>, dim3((g.npts + nt - 1) / nt), dim3(nt), 0, s, g, x0, y0, z0, r00, a, gamma0, (T*)vort);
+}
+
+// --------------------------------------------------------- spectral helpers
+template <typename T>
+__global__ void biot_savart_k(GridGeom g, const double* kx, const double* ky, const double* kz,
+                              const cplx<T>* vk, cplx<T>* uk) {
+  // basic_operators.f90:1081-1126
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.ncplx) return;
+  const int iz = p % g.nz, iy = (p / g.nz) % g.ny, ix = p / ((long)g.nz * g.ny);
+  const double k1 = kx[ix], k2_ = ky[iy], k3 = kz[iz];
+  const double kk = k1 * k1 + k2_ * k2_ + k3 * k3;
+  const long n = g.ncplx;
+  if (kk >= 1.0e-13) {
+    double sr[3], si[3];
+    for (int c = 0; c < 3; ++c) { sr[c] = (double)vk[c * n + p].x / kk; si[c] = (double)vk[c * n + p].y / kk; }
+    const double cr[3] = {k2_ * sr[2] - k3 * sr[1], k3 * sr[0] - k1 * sr[2], k1 * sr[1] - k2_ * sr[0]};
+    const double ci[3] = {k2_ * si[2] - k3 * si[1], k3 * si[0] - k1 * si[2], k1 * si[1] - k2_ * si[0]};
+    for (int c = 0; c < 3; ++c) uk[c * n + p] = mk<T>((T)(-ci[c]), (T)cr[c]);
+  } else {
+    for (int c = 0; c < 3; ++c) uk[c * n + p] = mk<T>(T(0), T(0));
+  }
+}
+template <typename T>
+void launch_biot_savart(const GridGeom& g, const double* kx, const double* ky, const double* kz, const void* vk, void* uk, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(biot_savart_k<T>, dim3((g.ncplx + nt - 1) / nt), dim3(nt), 0, s, g, kx, ky, kz,
+                     (const cplx<T>*)vk, (cplx<T>*)uk);
+}
+
+template <typename T>
+__global__ void divergence_k(GridGeom g, const double* kx, const double* ky, const double* kz,
+                             const cplx<T>* uk, cplx<T>* dk) {
+  // basic_operators.f90:707-734
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.ncplx) return;
+  const int iz = p % g.nz, iy = (p / g.nz) % g.ny, ix = p / ((long)g.nz * g.ny);
+  const long n = g.ncplx;
+  const double re = kx[ix] * (double)uk[p].x + ky[iy] * (double)uk[n + p].x + kz[iz] * (double)uk[2 * n + p].x;
+  const double im = kx[ix] * (double)uk[p].y + ky[iy] * (double)uk[n + p].y + kz[iz] * (double)uk[2 * n + p].y;
+  dk[p] = mk<T>((T)(-im), (T)re);
+}
+template <typename T>
+void launch_divergence(const GridGeom& g, const double* kx, const double* ky, const double* kz, const void* uk, void* dk, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(divergence_k<T>, dim3((g.ncplx + nt - 1) / nt), dim3(nt), 0, s, g, kx, ky, kz,
+                     (const cplx<T>*)uk, (cplx<T>*)dk);
+}
+
+template <typename T>
+__global__ void set_mode0_k(cplx<T>* uk, long off, T re, T im) { uk[off] = mk<T>(re, im); }
+template <typename T>
+void launch_set_mode0(const GridGeom& g, void* uk, int comp, double re, double im, hipStream_t s) {
+  hipLaunchKernelGGL(set_mode0_k<T>, dim3(1), dim3(1), 0, s, (cplx<T>*)uk, (long)comp * g.ncplx, (T)re, (T)im);
+}
+
+// --------------------------------------------------------------- diagnostics
+static constexpr int DIAG_NT = 256;
+static constexpr int DIAG_PPT = 8;  // points per thread
+
+template <typename T>
+__global__ __launch_bounds__(DIAG_NT) void diag_k(GridGeom g, DiagArgs a) {
+  const T* u = (const T*)a.u; const T* w = (const T*)a.w; const T* dv = (const T*)a.div;
+  const T* mask = (const T*)a.mask; const T* us = (const T*)a.us;
+  const long n = g.npts;
+  double acc[DIAG_NV];
+#pragma unroll
+  for (int v = 0; v < DIAG_NV; ++v) acc[v] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[DG_URES + c] = 9.0e9;
+  acc[DG_DIVMAX] = -1.0e300; acc[DG_DIVMAXF] = -1.0e300;
+  const long base = ((long)blockIdx.x * DIAG_NT) * DIAG_PPT + threadIdx.x;
+  for (int q = 0; q < DIAG_PPT; ++q) {
+    const long p = base + (long)q * DIAG_NT;
+    if (p >= n) break;
+    const int ix = p % g.nx, iy = (p / g.nx) % g.ny, iz = p / ((long)g.nx * g.ny);
+    const double m = (double)mask[p];
+    const double uu[3] = {(double)u[p], (double)u[n + p], (double)u[2 * n + p]};
+    const double ww[3] = {(double)w[p], (double)w[n + p], (double)w[2 * n + p]};
+    const double ss[3] = {(double)us[p], (double)us[n + p], (double)us[2 * n + p]};
+    const double du[3] = {uu[0] - ss[0], uu[1] - ss[1], uu[2] - ss[2]};
+    const double pen[3] = {-m * du[0] * a.eps_inv, -m * du[1] * a.eps_inv, -m * du[2] * a.eps_inv};
+    const int col = color_clamp(a.color[p]);
+    double lev[3] = {ix * g.dx - a.x0[0], iy * g.dy - a.x0[1], iz * g.dz - a.x0[2]};
+    periodize_d(lev, g);
+    const double t0[3] = {-(lev[1] * pen[2] - lev[2] * pen[1]), -(lev[2] * pen[0] - lev[0] * pen[2]),
+                          -(lev[0] * pen[1] - lev[1] * pen[0])};
+    if (a.insect && (col == 2 || col == 3)) {
+      const double* pv = (col == 2) ? a.piv_l : a.piv_r;
+      lev[0] -= pv[0]; lev[1] -= pv[1]; lev[2] -= pv[2];
+      periodize_d(lev, g);
+    }
+    const double tp[3] = {-(lev[1] * pen[2] - lev[2] * pen[1]), -(lev[2] * pen[0] - lev[0] * pen[2]),
+                          -(lev[0] * pen[1] - lev[1] * pen[0])};
+    const double d2 = du[0] * du[0] + du[1] * du[1] + du[2] * du[2];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (col == c) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          acc[DG_FORCE + 3 * c + i] -= pen[i];
+          acc[DG_TORQUE0 + 3 * c + i] += t0[i];
+          acc[DG_TORQUE + 3 * c + i] += tp[i];
+        }
+        if (m > 0.0) acc[DG_URES + c] = fmin(acc[DG_URES + c], d2);
+      }
+    }
+    const double fm = 1.0 - m;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      acc[DG_POWER + i] += ss[i] * pen[i];
+      acc[DG_EK + i] += uu[i] * uu[i];
+      acc[DG_EKF + i] += (uu[i] * fm) * (uu[i] * fm);
+      acc[DG_DISS + i] += ww[i] * ww[i];
+      acc[DG_DISSF + i] += (ww[i] * fm) * (ww[i] * fm);
+      acc[DG_MASKUS + i] += m * ss[i];
+    }
+    const double d = (double)dv[p];
+    acc[DG_DIVMAX] = fmax(acc[DG_DIVMAX], d);
+    acc[DG_DIVMAXF] = fmax(acc[DG_DIVMAXF], d * fm);
+    acc[DG_MASKVOL] += m;
+  }
+  __shared__ double red[DIAG_NT / 64];
+  const int tid = threadIdx.x;
+  for (int v = 0; v < DIAG_NV; ++v) {
+    const bool is_min = (v >= DG_URES && v < DG_URES + 4);
+    const bool is_max = (v == DG_DIVMAX || v == DG_DIVMAXF);
+    double x = acc[v];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(x, o, 64);
+      x = is_min ? fmin(x, y) : (is_max ? fmax(x, y) : x + y);
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = x;
+    __syncthreads();
+    if (tid == 0) {
+      double r = red[0];
+      for (int i = 1; i < DIAG_NT / 64; ++i) r = is_min ? fmin(r, red[i]) : (is_max ? fmax(r, red[i]) : r + red[i]);
+      a.partial[(long)blockIdx.x * DIAG_NV + v] = r;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T> int diag_blocks(const GridGeom& g) {
+  const long per = (long)DIAG_NT * DIAG_PPT;
+  return (int)((g.npts + per - 1) / per);
+}
+template <typename T>
+void launch_diag(const GridGeom& g, const DiagArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(diag_k<T>, dim3(diag_blocks<T>(g)), dim3(DIAG_NT), 0, s, g, a);
+}
+
+// ------------------------------------------------------------ row reducer
+// Combines per-block partial rows in a fixed order (bitwise reproducible):
+// Neumaier-compensated sums, or max/min.
+__global__ void reduce_rows_k(const double* partial, int nrows, int nv, const int* kinds, double* out) {
+  __shared__ double sv[1024];
+  __shared__ double sc[1024];
+  const int tid = threadIdx.x;
+  for (int v = 0; v < nv; ++v) {
+    const int kind = kinds[v];
+    double s = (kind == 1) ? -1.0e308 : (kind == 2 ? 1.0e308 : 0.0), comp = 0.0;
+    for (int r = tid; r < nrows; r += blockDim.x) {
+      const double x = partial[(long)r * nv + v];
+      if (kind == 1) s = fmax(s, x);
+      else if (kind == 2) s = fmin(s, x);
+      else {
+        const double t = s + x;
+        comp += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+        s = t;
+      }
+    }
+    sv[tid] = s; sc[tid] = comp;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (tid < w) {
+        if (kind == 1) sv[tid] = fmax(sv[tid], sv[tid + w]);
+        else if (kind == 2) sv[tid] = fmin(sv[tid], sv[tid + w]);
+        else {
+          const double a = sv[tid], b = sv[tid + w];
+          const double t = a + b;
+          const double e = (fabs(a) >= fabs(b)) ? (a - t) + b : (b - t) + a;
+          sv[tid] = t; sc[tid] = sc[tid] + sc[tid + w] + e;
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) out[v] = (kind == 0) ? sv[0] + sc[0] : sv[0];
+    __syncthreads();
+  }
+}
+
+void launch_reduce_rows(const double* partial, int nrows, int nv, const int* kinds_dev, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_rows_k, dim3(1), dim3(1024), 0, s, partial, nrows, nv, kinds_dev, out);
+}
+
+// ------------------------------------------------------------ conversions
+template <typename T>
+__global__ void from_f64_k(const double* src, T* dst, long n) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[p] = (T)src[p];
+}
+template <typename T>
+__global__ void to_f64_k(const T* src, double* dst, long n) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[p] = (double)src[p];
+}
+template <typename T> void launch_from_f64(const double* src, void* dst, long n, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(from_f64_k<T>, dim3((n + nt - 1) / nt), dim3(nt), 0, s, src, (T*)dst, n);
+}
+template <typename T> void launch_to_f64(const void* src, double* dst, long n, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(to_f64_k<T>, dim3((n + nt - 1) / nt), dim3(nt), 0, s, (const T*)src, dst, n);
+}
+
+__global__ void zero_k(uint32_t* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+void launch_zero(void* p, size_t bytes, hipStream_t s) {
+  // a fill kernel rather than hipMemsetAsync, so the call may sit inside a captured graph
+  const size_t n = bytes / 4;
+  const int nt = 256;
+  if (n) hipLaunchKernelGGL(zero_k, dim3((n + nt - 1) / nt), dim3(nt), 0, s, (uint32_t*)p, n);
+}
+
+#define INST(T)                                                                                              \
+  template void launch_sphere<T>(const GridGeom&, const SphereArgs&, void*, void*, uint8_t*, hipStream_t);   \
+  template void launch_cavity<T>(const GridGeom&, const CavityArgs&, void*, void*, uint8_t*, hipStream_t);   \
+  template void launch_insect<T>(const GridGeom&, const InsectArgs&, void*, void*, uint8_t*, hipStream_t);   \
+  template void launch_vortex_ring<T>(const GridGeom&, double, double, double, double, double, double, void*, hipStream_t); \
+  template void launch_biot_savart<T>(const GridGeom&, const double*, const double*, const double*, const void*, void*, hipStream_t); \
+  template void launch_divergence<T>(const GridGeom&, const double*, const double*, const double*, const void*, void*, hipStream_t); \
+  template void launch_set_mode0<T>(const GridGeom&, void*, int, double, double, hipStream_t);               \
+  template int diag_blocks<T>(const GridGeom&);                                                              \
+  template void launch_diag<T>(const GridGeom&, const DiagArgs&, hipStream_t);                               \
+  template void launch_from_f64<T>(const double*, void*, long, hipStream_t);                                 \
+  template void launch_to_f64<T>(const void*, double*, long, hipStream_t);
+INST(float)
+INST(double)
+
+}  // namespace flusi

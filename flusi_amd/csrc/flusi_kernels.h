@@ -1,0 +1,122 @@
+// Kernel argument blocks and launcher declarations shared by the transform
+// passes (passes.hip), the field kernels (fields.hip) and the solver host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flusi {
+
+// Element offsets of a 3-D complex array walked as (a = FFT axis,
+// b = tile axis, c = outer axis).
+struct Strides { long a, b, c; };
+
+enum TileOp { OP_COPY = 0, OP_DERIV = 1, OP_DERIV_MINUS = 2 };
+
+struct TileJob {
+  const void* in0;   // complex field, layout given by Strides in
+  const void* in1;   // second operand of OP_DERIV_MINUS
+  void* out;
+  int op;            // out = in0 | i k_a in0 | i k_a in0 - in1
+};
+
+struct TilePassArgs {
+  TileJob jobs[8];
+  int njobs;
+  int nb, nc;          // extents of the tile axis and the outer axis
+  Strides sin, sout;
+  const double* kline; // wavenumbers along the FFT axis (OP_DERIV*)
+  double scale;        // applied to every stored value
+  const void* tw;      // L twiddles exp(-2 pi i m / L), element type of the pass
+};
+
+// x pass modes
+enum XMode {
+  XM_RHS = 0,       // 6 spectral fields -> u x w - chi (u - us) [+ sponge] -> 3 [6] spectral
+  XM_INV_UW = 1,    // 6 spectral fields -> u (3) and w (3) in physical space
+  XM_INV = 2,       // nf spectral fields -> nf physical fields
+  XM_FWD = 3,       // nf physical fields -> nf spectral fields
+};
+
+struct XPassArgs {
+  const void* in;      // nf spectral (P layout) or physical fields, field stride fstride_in
+  void* out;           // outputs, field stride fstride_out
+  long fstride_in, fstride_out;
+  int nf;              // fields for XM_INV / XM_FWD
+  int nlines;          // ny*nz
+  int nx, ny, nxh;
+  const double* kx;    // nxh wavenumbers
+  const void* twM;     // M = nx/2 twiddles
+  const void* twN;     // nxh twiddles exp(-2 pi i k / nx)
+  // RHS extras
+  const void* mask;    // N reals (element type of the pass)
+  const void* us;      // 3N reals, component stride N
+  long npts;           // N
+  double eps_inv;      // 1/eta
+  int sponge;          // 0 none, 1 cavity, 2 outlet_x
+  int sponge_thickness;
+  double sponge_eps_inv;
+  double sponge_mask_thr;
+  int nz;
+  double* partial;     // per block: umax^2, sum chi(u-us) x,y,z (4 doubles)
+};
+
+// z-forward pass with the fused spectral epilogue (projection, explicit
+// diffusion, dealiasing, stepper update, zero-mode forcing, Parseval sum)
+enum StepMode {
+  SM_RHS_ONLY = 0,   // out0 = nlk (projected)
+  SM_EULER = 1,      // AB2 start-up Euler step
+  SM_AB2 = 2,
+  SM_RK2_1 = 3,
+  SM_RK2_2 = 4,
+  SM_RK4_1 = 5,
+  SM_RK4_2 = 6,
+  SM_RK4_3 = 7,
+  SM_RK4_4 = 8,
+};
+
+struct ZStepArgs {
+  const void* nq;        // 3 (6 with sponge) Q-layout fields, field stride fs
+  long fs;
+  int nx, ny, nz, nxh;
+  const double* kx; const double* ky; const double* kz;
+  const double* kxt2; const double* kyt2; const double* kzt2;  // dealias predicate terms
+  const void* tw;        // nz twiddles
+  double scale;          // 1/(nx ny nz)
+  int mode;
+  int sponge;
+  int dealias_on;        // iDealias
+  double nu;
+  // state (S layout, 3 fields each, field stride fs)
+  void* u;               // uk (AB2/RK2), u_n (RK4)
+  const void* udiff;     // field the explicit diffusion term is taken from (stage input)
+  void* ustage;          // RK4 next-stage input (written)
+  void* acc;             // RK4 accumulator
+  void* r0;              // AB2: nlk(n0) out; RK2: N0 (out in stage 1, in stage 2)
+  const void* r1;        // AB2: nlk(n1) in
+  void* rhs_out;         // SM_RHS_ONLY output
+  double dt, b10, b11;
+  double vis_dt_new;     // IF exponent dt (<0: factor is zero)
+  double vis_dt_old;     // RK2 stale IF (<0: zero)
+  int explicit_diffusion;
+  // zero mode forcing
+  int meanflow_dynamic[3];
+  const double* pen_sum; // device sum chi(u-us) over the grid (3 doubles)
+  double dynamic_scale;  // dx dy dz * startup / m_fluid (negated applied)
+  int set_mean[3];       // post-step zero-mode overwrite (fixed/sinusoidal)
+  double mean_value[3];
+  double* energy_partial;  // per block Parseval partial sum (final stages only)
+};
+
+// launchers (passes.hip)
+template <typename T> void launch_tile_pass(const TilePassArgs& a, int L, int dir, bool contig_in, hipStream_t s);
+template <typename T> void launch_x_pass(const XPassArgs& a, int mode, hipStream_t s);
+template <typename T> void launch_zstep(const ZStepArgs& a, hipStream_t s);
+int x_pass_blocks(int nx, int nlines, bool fp64);
+int zstep_blocks(int nz, int ny, int nxh, bool fp64);
+bool fft_length_supported(int n);
+
+// reductions (fields.hip)
+// out[v] = combine over rows of partial[row*nv + v]; kind[v]: 0 = Neumaier sum, 1 = max, 2 = min
+void launch_reduce_rows(const double* partial, int nrows, int nv, const int* kinds_host, double* out, hipStream_t s);
+
+}  // namespace flusi

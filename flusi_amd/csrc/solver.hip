@@ -1,0 +1,1032 @@
+// Solver context: device state, the RHS pass pipeline, the AB2/RK2/RK4
+// steppers, create_mask, write_integrals and the C ABI of include/flusi_hip.h.
+#include "../../include/flusi_hip.h"
+#include "fft_core.h"
+#include "fields.h"
+#include "flusi_kernels.h"
+#include "host_params.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace flusi {
+
+#define HIPCHECK(x)                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) throw FlusiError(900001, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static void kcheck(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw FlusiError(900002, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// fluid_time_step.f90:1051-1060 truncate: write(str,'(es7.1)') a; read(str,*) a
+static double truncate_dt(double a) {
+  char buf[64];
+  std::snprintf(buf, sizeof(buf), "%.1E", a);
+  return std::strtod(buf, nullptr);
+}
+
+// vars.f90:429-462 time_for_output
+static bool time_for_output(double time, double dt, int it, double tfrequ, int ifrequ, double tmax, double tfirst,
+                            bool intelligent) {
+  if (time < tfirst) return false;
+  if (intelligent) {
+    const double t1 = std::ceil(time / tfrequ) * tfrequ, t2 = std::floor(time / tfrequ) * tfrequ;
+    return std::fabs(time - t1) <= 1e-6 || std::fabs(time - t2) <= 1e-6 || (it % ifrequ) == 0 ||
+           std::fabs(time - tmax) <= 1e-6;
+  }
+  const double md = time - std::floor(time / tfrequ) * tfrequ;
+  return (md < dt) || (it % ifrequ) == 0 || (time == tmax);
+}
+
+// vars.f90:408-422
+static double startup_conditioner(double time, double trel, double tau) {
+  const double t = time - trel;
+  if (time <= trel) return 0.0;
+  if (time > trel && time < trel + tau) return (t * t * t) / (-0.5 * tau * tau * tau) + 3.0 * (t * t) / (tau * tau);
+  return 1.0;
+}
+
+static int fortran_nint(double x) { return x >= 0 ? (int)std::floor(x + 0.5) : -(int)std::floor(-x + 0.5); }
+
+static const char* kTimerNames[FLUSI_TIMER_SLOTS] = {"ZINV", "YINV", "XPASS", "YFWD", "ZSTEP", "MASK", "DIAG", "OTHER"};
+enum { TM_ZINV, TM_YINV, TM_X, TM_YFWD, TM_ZSTEP, TM_MASK, TM_DIAG, TM_OTHER };
+
+class SolverBase {
+ public:
+  virtual ~SolverBase() = default;
+  std::string last_error;
+  Params p;
+  std::unique_ptr<IniFile> ini;
+  std::string base_dir, out_dir;
+  int nx = 0, ny = 0, nz = 0, nxh = 0;
+  long npts = 0, nc = 0;
+  double time = 0.0, dt0 = 0.0, dt1 = 0.0;
+  int it = 0, n0 = 0, n1 = 1;
+  double vis_dt = -1.0;  // dt of the stored integrating factor, < 0: factor is zero
+  bool fields_initialized = false;
+  Insect insect;
+  bool is_insect = false;
+  Vec3 last_power4_xyz{0, 0, 0};
+  double last_power_total = 0.0;
+
+  virtual void init_fields() = 0;
+  virtual void set_uk(const double* uk) = 0;
+  virtual void get_uk(double* uk) = 0;
+  virtual void get_u(double* u, bool vort) = 0;
+  virtual void cal_nlk(double time, double* nlk, double* u) = 0;
+  virtual void fluid_timestep() = 0;
+  virtual void create_mask(double time) = 0;
+  virtual void get_mask(double* mask, double* us, int16_t* color) = 0;
+  virtual void write_integrals(double time, double* out) = 0;
+  virtual double energy_k() = 0;
+  virtual void cal_vis(double dt, double* vis) = 0;
+  virtual void fft1(const double* f, double* fk) = 0;
+  virtual void ifft1(const double* fk, double* f) = 0;
+  virtual void timers(int enable, double* ms) = 0;
+
+  // time_step.f90:57-167 loop body
+  void step() {
+    dt0 = dt1;
+    fluid_timestep();
+    time = time + dt1;
+    it += 1;
+    if (time_for_output(time, dt1, it, p.tintegral, p.itdrag, p.tmax, 0.0, p.intelligent_dt == "yes"))
+      write_integrals(time, nullptr);
+  }
+
+  void append(const std::string& name, const std::vector<double>& row, const char* header) {
+    if (out_dir.empty()) return;
+    const std::string path = out_dir + "/" + name;
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) throw FlusiError(900010, "cannot open " + path);
+    if (std::ftell(f) == 0 && header) std::fprintf(f, "%s\n", header);
+    for (double v : row) std::fprintf(f, "%15.8E ", v);
+    std::fprintf(f, "\n");
+    std::fclose(f);
+  }
+};
+
+template <typename T>
+class Solver : public SolverBase {
+ public:
+  using C = typename V2<T>::t;
+  static constexpr bool FP64 = sizeof(T) == 8;
+  hipStream_t stream = nullptr;
+  // device memory
+  void* uk = nullptr;
+  void* reg[2] = {nullptr, nullptr};
+  void* Q = nullptr;
+  void* P = nullptr;
+  void* mask = nullptr;
+  void* us = nullptr;
+  uint8_t* color = nullptr;
+  void* phys = nullptr;     // 7 real fields for diagnostics (u, w, div)
+  double* tab = nullptr;    // kx ky kz kxt2 kyt2 kzt2
+  double *kx_d, *ky_d, *kz_d, *kxt2_d, *kyt2_d, *kzt2_d;
+  void* tw_z = nullptr; void* tw_y = nullptr; void* tw_M = nullptr; void* tw_N = nullptr;
+  double* xpart = nullptr; int nxblocks = 0;
+  double* zpart = nullptr; int nzblocks = 0;
+  double* dpart = nullptr; int ndblocks = 0;
+  double* red = nullptr;    // small results
+  int* kinds = nullptr;     // reduction kinds: [0..3] x pass, [4] energy, [8..8+DIAG_NV) diag
+  double* stage = nullptr; size_t stage_n = 0;
+  std::vector<double> kx_h, ky_h, kz_h, kxt2_h, kyt2_h, kzt2_h;
+  GridGeom geom{};
+  bool timing = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  double tm_ms[FLUSI_TIMER_SLOTS] = {0};
+
+  ~Solver() override {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& e : ev) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
+    void* ptrs[] = {uk, reg[0], reg[1], Q, P, mask, us, color, phys, tab, tw_z, tw_y, tw_M, tw_N,
+                    xpart, zpart, dpart, red, kinds, stage};
+    for (void* q : ptrs) if (q) (void)hipFree(q);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void* dalloc(size_t bytes) {
+    void* q = nullptr;
+    HIPCHECK(hipMalloc(&q, bytes < 256 ? 256 : bytes));
+    return q;
+  }
+
+  bool fft_ok = false;
+  void need_fft() const {
+    if (!fft_ok)
+      throw FlusiError(900003, "grid " + std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz) +
+                                   " has no FFT plan (nx/2, ny, nz must each be 2^a or 3*2^a between 8 and 512, "
+                                   "nx <= 512); only mask generation is available");
+  }
+
+  void setup() {
+    nx = p.nx; ny = p.ny; nz = p.nz; nxh = nx / 2 + 1;
+    fft_ok = nx % 2 == 0 && nx / 2 >= 8 && nx / 2 <= 256 && fft_length_supported(nx / 2) &&
+             fft_length_supported(ny) && fft_length_supported(nz);
+    npts = (long)nx * ny * nz;
+    nc = (long)nxh * ny * nz;
+    geom = GridGeom{nx, ny, nz, nxh, npts, nc, p.dx(), p.dy(), p.dz(), p.xl, p.yl, p.zl, p.periodic ? 1 : 0};
+    HIPCHECK(hipStreamCreate(&stream));
+    const size_t cs = sizeof(C) * nc;
+    mask = dalloc(sizeof(T) * npts);
+    us = dalloc(3 * sizeof(T) * npts);
+    color = (uint8_t*)dalloc((npts + 3) / 4 * 4);
+    launch_zero(mask, sizeof(T) * npts, stream);
+    launch_zero(us, 3 * sizeof(T) * npts, stream);
+    launch_zero(color, (npts + 3) / 4 * 4, stream);
+    if (fft_ok) {
+      uk = dalloc(3 * cs);
+      reg[0] = dalloc(3 * cs);
+      reg[1] = dalloc(3 * cs);
+      Q = dalloc(6 * cs);
+      P = dalloc(6 * cs);
+      launch_zero(uk, 3 * cs, stream);
+      launch_zero(reg[0], 3 * cs, stream);
+      launch_zero(reg[1], 3 * cs, stream);
+    }
+    kcheck("zero state");
+    // wavenumbers (cof_p3dfft.f90:661-680) and dealias terms (vis.f90:23-53)
+    const double sx = 2.0 * PI / p.xl, sy = 2.0 * PI / p.yl, sz = 2.0 * PI / p.zl;
+    const double kxt = (2.0 / 3.0) * double(nx / 2 - 1), kyt = (2.0 / 3.0) * double(ny / 2 - 1),
+                 kzt = (2.0 / 3.0) * double(nz / 2 - 1);
+    kx_h.resize(nxh); ky_h.resize(ny); kz_h.resize(nz);
+    kxt2_h.resize(nxh); kyt2_h.resize(ny); kzt2_h.resize(nz);
+    auto wrap = [](int i, int n) { return ((i + n / 2) % n) - n / 2; };
+    for (int i = 0; i < nxh; ++i) { kx_h[i] = sx * double(i); double t = (kx_h[i] / sx) / kxt; kxt2_h[i] = t * t; }
+    for (int i = 0; i < ny; ++i) { ky_h[i] = sy * double(wrap(i, ny)); double t = (ky_h[i] / sy) / kyt; kyt2_h[i] = t * t; }
+    for (int i = 0; i < nz; ++i) { kz_h[i] = sz * double(wrap(i, nz)); double t = (kz_h[i] / sz) / kzt; kzt2_h[i] = t * t; }
+    const size_t ntab = 2 * ((size_t)nxh + ny + nz);
+    tab = (double*)dalloc(ntab * sizeof(double));
+    std::vector<double> h;
+    h.insert(h.end(), kx_h.begin(), kx_h.end());
+    h.insert(h.end(), ky_h.begin(), ky_h.end());
+    h.insert(h.end(), kz_h.begin(), kz_h.end());
+    h.insert(h.end(), kxt2_h.begin(), kxt2_h.end());
+    h.insert(h.end(), kyt2_h.begin(), kyt2_h.end());
+    h.insert(h.end(), kzt2_h.begin(), kzt2_h.end());
+    HIPCHECK(hipMemcpy(tab, h.data(), ntab * sizeof(double), hipMemcpyHostToDevice));
+    kx_d = tab; ky_d = kx_d + nxh; kz_d = ky_d + ny; kxt2_d = kz_d + nz; kyt2_d = kxt2_d + nxh; kzt2_d = kyt2_d + ny;
+    if (fft_ok) {
+      tw_z = make_tw(nz, nz); tw_y = make_tw(ny, ny); tw_M = make_tw(nx / 2, nx / 2); tw_N = make_tw(nxh, nx);
+      nxblocks = x_pass_blocks(nx, ny * nz, FP64);
+      xpart = (double*)dalloc(sizeof(double) * 4 * nxblocks);
+      nzblocks = zstep_blocks(nz, ny, nxh, FP64);
+      zpart = (double*)dalloc(sizeof(double) * nzblocks);
+      ndblocks = diag_blocks<T>(geom);
+      dpart = (double*)dalloc(sizeof(double) * DIAG_NV * ndblocks);
+    }
+    red = (double*)dalloc(sizeof(double) * 128);
+    std::vector<int> k(8 + DIAG_NV, 0);
+    k[0] = 1;  // umax^2: max
+    for (int v = 0; v < DIAG_NV; ++v) {
+      if (v >= DG_URES && v < DG_URES + 4) k[8 + v] = 2;
+      if (v == DG_DIVMAX || v == DG_DIVMAXF) k[8 + v] = 1;
+    }
+    kinds = (int*)dalloc(sizeof(int) * k.size());
+    HIPCHECK(hipMemcpy(kinds, k.data(), sizeof(int) * k.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
+  void* make_tw(int n, int period) {
+    std::vector<C> t(n);
+    for (int m = 0; m < n; ++m) {
+      const double a = -2.0 * PI * double(m) / double(period);
+      t[m].x = (T)std::cos(a);
+      t[m].y = (T)std::sin(a);
+    }
+    void* d = dalloc(sizeof(C) * n);
+    HIPCHECK(hipMemcpy(d, t.data(), sizeof(C) * n, hipMemcpyHostToDevice));
+    return d;
+  }
+
+  // ------------------------------------------------------------ timing
+  template <typename F> void timed(int slot, F&& f) {
+    if (!timing) { f(); return; }
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a)); HIPCHECK(hipEventCreate(&b));
+    HIPCHECK(hipEventRecord(a, stream));
+    f();
+    HIPCHECK(hipEventRecord(b, stream));
+    ev.push_back({slot, {a, b}});
+  }
+  void timers(int enable, double* ms) override {
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (auto& e : ev) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, e.second.first, e.second.second);
+      tm_ms[e.first] += t;
+      (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second);
+    }
+    ev.clear();
+    if (ms) for (int i = 0; i < FLUSI_TIMER_SLOTS; ++i) ms[i] = tm_ms[i];
+    if (enable >= 0) { timing = enable != 0; for (double& v : tm_ms) v = 0.0; }
+  }
+
+  // ------------------------------------------------------------ passes
+  C* cf(void* base, int f) { return reinterpret_cast<C*>(base) + (long)f * nc; }
+
+  void zinv(const TileJob* jobs, int nj) {  // S -> Q
+    TilePassArgs a{};
+    for (int j = 0; j < nj; ++j) a.jobs[j] = jobs[j];
+    a.njobs = nj; a.nb = ny; a.nc = nxh;
+    a.sin = Strides{1, nz, (long)ny * nz};
+    a.sout = Strides{ny, 1, (long)nz * ny};
+    a.kline = kz_d; a.scale = 1.0; a.tw = tw_z;
+    timed(TM_ZINV, [&] { launch_tile_pass<T>(a, nz, +1, true, stream); });
+    kcheck("zinv");
+  }
+  void yinv(const TileJob* jobs, int nj) {  // Q -> P
+    TilePassArgs a{};
+    for (int j = 0; j < nj; ++j) a.jobs[j] = jobs[j];
+    a.njobs = nj; a.nb = nxh; a.nc = nz;
+    a.sin = Strides{1, (long)nz * ny, ny};
+    a.sout = Strides{nxh, 1, (long)ny * nxh};
+    a.kline = ky_d; a.scale = 1.0; a.tw = tw_y;
+    timed(TM_YINV, [&] { launch_tile_pass<T>(a, ny, +1, true, stream); });
+    kcheck("yinv");
+  }
+  void yfwd(int nf) {  // P -> Q
+    TilePassArgs a{};
+    for (int j = 0; j < nf; ++j) a.jobs[j] = TileJob{cf(P, j), nullptr, cf(Q, j), OP_COPY};
+    a.njobs = nf; a.nb = nxh; a.nc = nz;
+    a.sin = Strides{nxh, 1, (long)ny * nxh};
+    a.sout = Strides{1, (long)nz * ny, ny};
+    a.kline = ky_d; a.scale = 1.0; a.tw = tw_y;
+    timed(TM_YFWD, [&] { launch_tile_pass<T>(a, ny, -1, false, stream); });
+    kcheck("yfwd");
+  }
+  void zfwd_plain(int nf, void* dst) {  // Q -> S with 1/N (cof_p3dfft.f90:52-56)
+    TilePassArgs a{};
+    for (int j = 0; j < nf; ++j) a.jobs[j] = TileJob{cf(Q, j), nullptr, cf(dst, j), OP_COPY};
+    a.njobs = nf; a.nb = ny; a.nc = nxh;
+    a.sin = Strides{ny, 1, (long)nz * ny};
+    a.sout = Strides{1, nz, (long)ny * nz};
+    a.kline = kz_d; a.scale = 1.0 / (double(nx) * double(ny) * double(nz)); a.tw = tw_z;
+    timed(TM_ZSTEP, [&] { launch_tile_pass<T>(a, nz, -1, false, stream); });
+    kcheck("zfwd");
+  }
+  XPassArgs xargs() {
+    XPassArgs a{};
+    a.nlines = ny * nz; a.nx = nx; a.ny = ny; a.nz = nz; a.nxh = nxh;
+    a.kx = kx_d; a.twM = tw_M; a.twN = tw_N;
+    a.npts = npts;
+    return a;
+  }
+  // the five fields of the z-inverse pass and the six of the y-inverse pass:
+  // Q: 0 U_x, 1 U_y, 2 U_z, 3 D_x = F_z^-1(i kz u_x), 4 D_y = F_z^-1(i kz u_y)
+  // P: 0 V_x, 1 V_y, 2 V_z, 3 w_x, 4 dz u_x, 5 dy u_x
+  void inv_uw(const void* u_in) {
+    const C* u = reinterpret_cast<const C*>(u_in);
+    TileJob zj[5] = {{u, nullptr, cf(Q, 0), OP_COPY}, {u, nullptr, cf(Q, 3), OP_DERIV},
+                     {u + nc, nullptr, cf(Q, 1), OP_COPY}, {u + nc, nullptr, cf(Q, 4), OP_DERIV},
+                     {u + 2 * nc, nullptr, cf(Q, 2), OP_COPY}};
+    zinv(zj, 5);
+    TileJob yj[6] = {{cf(Q, 0), nullptr, cf(P, 0), OP_COPY}, {cf(Q, 0), nullptr, cf(P, 5), OP_DERIV},
+                     {cf(Q, 1), nullptr, cf(P, 1), OP_COPY}, {cf(Q, 2), nullptr, cf(P, 2), OP_COPY},
+                     {cf(Q, 2), cf(Q, 4), cf(P, 3), OP_DERIV_MINUS}, {cf(Q, 3), nullptr, cf(P, 4), OP_COPY}};
+    yinv(yj, 6);
+  }
+
+  int sponge_code() const {
+    if (p.iVorticitySponge != "yes") return 0;
+    if (p.iSpongeType == "cavity") return 1;
+    if (p.iSpongeType == "outlet_x") return 2;
+    throw FlusiError(3335, "iSpongeType not supported: " + p.iSpongeType);
+  }
+
+  // rhs.f90:2-115 up to the forward transform: mask, inverse passes, x pass,
+  // and the reduction of max|u|^2 and sum chi (u - us) into red[0..3]
+  void pre_rhs(const void* u_in, double t) {
+    if (p.iMoving == 1 && p.iPenalization == 1) create_mask(t);
+    if (p.eps < 1.0e-11) throw FlusiError(77363800, "Value of eps is very small, maybe even zero.");
+    inv_uw(u_in);
+    XPassArgs a = xargs();
+    a.in = P; a.out = P; a.fstride_in = nc; a.fstride_out = nc; a.nf = 6;
+    a.mask = mask; a.us = us; a.eps_inv = 1.0 / p.eps;
+    a.sponge = sponge_code();
+    a.sponge_thickness = p.sponge_thickness;
+    a.sponge_eps_inv = a.sponge ? 1.0 / p.eps_sponge : 0.0;
+    a.sponge_mask_thr = (a.sponge == 2) ? (double)1.0e-12f : 1.0e-12;
+    a.partial = xpart;
+    timed(TM_X, [&] { launch_x_pass<T>(a, XM_RHS, stream); });
+    kcheck("x pass");
+    launch_reduce_rows(xpart, nxblocks, 4, kinds, red, stream);
+    kcheck("reduce x");
+  }
+
+  ZStepArgs zargs(int mode) {
+    ZStepArgs z{};
+    z.nq = Q; z.fs = nc; z.nx = nx; z.ny = ny; z.nz = nz; z.nxh = nxh;
+    z.kx = kx_d; z.ky = ky_d; z.kz = kz_d; z.kxt2 = kxt2_d; z.kyt2 = kyt2_d; z.kzt2 = kzt2_d;
+    z.tw = tw_z; z.scale = 1.0 / (double(nx) * double(ny) * double(nz));
+    z.mode = mode; z.sponge = sponge_code() ? 1 : 0; z.dealias_on = p.iDealias == 1; z.nu = p.nu;
+    z.explicit_diffusion = (p.method == "RK4") ? 1 : 0;
+    z.vis_dt_new = -1.0; z.vis_dt_old = -1.0;
+    z.pen_sum = red + 1;
+    double sc = p.dx() * p.dy() * p.dz();
+    if (p.startup_conditioner == "yes") sc *= startup_conditioner(time, p.T_release_meanflow, p.tau_meanflow);
+    z.dynamic_scale = sc / p.m_fluid;
+    for (int c = 0; c < 3; ++c) z.meanflow_dynamic[c] = p.iMeanFlow[c] == "dynamic";
+    return z;
+  }
+  void post_rhs(ZStepArgs& z) {
+    yfwd(z.sponge ? 6 : 3);
+    const bool fin = z.mode == SM_EULER || z.mode == SM_AB2 || z.mode == SM_RK2_2 || z.mode == SM_RK4_4;
+    if (fin) {
+      // set_mean_flow (fluid_time_step.f90:1066-1090) with the time of the step start
+      for (int c = 0; c < 3; ++c) {
+        z.set_mean[c] = 0;
+        if (p.iMeanFlow[c] == "fixed") { z.set_mean[c] = 1; z.mean_value[c] = p.umean[c]; }
+        else if (p.iMeanFlow[c] == "sinusoidal") {
+          z.set_mean[c] = 1; z.mean_value[c] = p.umean[c] * std::sin(2.0 * PI * p.umean_freq * time);
+        }
+      }
+      z.energy_partial = zpart;
+    }
+    timed(TM_ZSTEP, [&] { launch_zstep<T>(z, stream); });
+    kcheck("zstep");
+  }
+
+  double host_umax() {
+    double v = 0.0;
+    HIPCHECK(hipMemcpyAsync(&v, red, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    return std::sqrt(v);
+  }
+
+  // fluid_time_step.f90:917-1042 adjust_dt
+  double adjust_dt(double t) {
+    if (p.dt_fixed > 0.0) return p.dt_fixed;
+    const double umax = host_umax();
+    if (std::isnan(umax)) throw FlusiError(100011, "Evolved field contains a NAN: aborting run.");
+    if (umax >= 1.0e3) throw FlusiError(100012, "Umax is very big, surely this is an error");
+    const double h = std::min(p.dx(), std::min(p.dy(), p.dz()));
+    double d = (umax >= 1.0e-8) ? h * p.cfl / umax : 1.0e-3;
+    d = truncate_dt(d);
+    if (p.dt_max > 0.0) d = std::min(d, p.dt_max);
+    if (p.iPenalization > 0) d = std::min(p.cfl_eta * p.eps, d);
+    if (p.method == "RK4") d = std::min(d, 0.5 * h * h / p.nu);
+    if (p.intelligent_dt == "yes") {
+      if (t >= p.tsave_first) {
+        d = std::min(d, 0.98 * p.tsave);
+        const double tt = std::ceil(t / p.tsave) * p.tsave;
+        if (t + d > tt && std::fabs(t - tt) >= 1.0e-6) d = tt - t;
+      }
+      d = std::min(d, 0.98 * p.tintegral);
+      double tt = std::ceil(t / p.tintegral) * p.tintegral;
+      if (t + d > tt && std::fabs(t - tt) >= 1.0e-6) d = tt - t;
+      tt = p.tmax;
+      if (t + d > tt && std::fabs(t - tt) >= 1.0e-6) d = p.tmax - t;
+    }
+    append("dt.t", {t, d, d * umax / h, d * p.nu / (h * h), d / p.eps},
+           "%          time              dt             CFL         viscous    penalization");
+    return d;
+  }
+
+  void record_ekin() {
+    if (out_dir.empty()) return;
+    launch_reduce_rows(zpart, nzblocks, 1, kinds + 4, red + 8, stream);
+    double e = 0.0;
+    HIPCHECK(hipMemcpyAsync(&e, red + 8, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    append("ekin.t", {time + dt1, e * p.xl * p.yl * p.zl}, "%          time            ekin");
+  }
+
+  // ------------------------------------------------------------ steppers
+  void fluid_timestep() override {
+    need_fft();
+    if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    const std::string& m = p.method;
+    if (m == "AB2") {
+      if (it == 0) {  // fluid_time_step.f90:675-748 euler_startup
+        pre_rhs(uk, time);
+        dt1 = adjust_dt(time);
+        if (dt1 != dt0) vis_dt = dt1;
+        ZStepArgs z = zargs(SM_EULER);
+        z.u = uk; z.r0 = reg[n0]; z.dt = dt1; z.vis_dt_new = vis_dt;
+        post_rhs(z);
+      } else {        // :752-832 adamsbashforth
+        pre_rhs(uk, time);
+        double d = adjust_dt(time);
+        if (d > p.tmax - time) d = p.tmax - time;
+        dt1 = d;
+        ZStepArgs z = zargs(SM_AB2);
+        z.b10 = dt1 / dt0 * (0.5 * dt1 + dt0);
+        z.b11 = -0.5 * dt1 * dt1 / dt0;
+        if (dt1 != dt0) vis_dt = dt1;
+        z.u = uk; z.r0 = reg[n0]; z.r1 = reg[n1]; z.dt = dt1; z.vis_dt_new = vis_dt;
+        post_rhs(z);
+      }
+    } else if (m == "RK2") {  // :519-598
+      pre_rhs(uk, time);
+      dt1 = adjust_dt(time);
+      const double old = vis_dt;
+      if (dt1 != dt0) vis_dt = dt1;
+      ZStepArgs z = zargs(SM_RK2_1);
+      z.u = uk; z.r0 = reg[0]; z.dt = dt1; z.vis_dt_old = old; z.vis_dt_new = vis_dt;
+      post_rhs(z);
+      pre_rhs(uk, time);
+      ZStepArgs z2 = zargs(SM_RK2_2);
+      z2.u = uk; z2.r0 = reg[0]; z2.dt = dt1;
+      post_rhs(z2);
+    } else if (m == "RK4") {  // :602-671 (explicit diffusion, accumulator register)
+      pre_rhs(uk, time);
+      dt1 = adjust_dt(time);
+      const int modes[4] = {SM_RK4_1, SM_RK4_2, SM_RK4_3, SM_RK4_4};
+      const double ts[4] = {time, time + 0.5 * dt1, time + 0.5 * dt1, time + dt1};
+      for (int s = 0; s < 4; ++s) {
+        if (s > 0) pre_rhs(reg[0], ts[s]);
+        ZStepArgs z = zargs(modes[s]);
+        z.u = uk; z.udiff = (s == 0) ? uk : reg[0]; z.ustage = reg[0]; z.acc = reg[1]; z.dt = dt1;
+        post_rhs(z);
+      }
+    } else {
+      throw FlusiError(10001, "Error! iTimeMethodFluid unknown. Abort.");
+    }
+    record_ekin();
+    std::swap(n0, n1);  // time_step.f90:72
+  }
+
+  void cal_nlk(double t, double* nlk, double* u) override {
+    need_fft();
+    if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    pre_rhs(uk, t);
+    ZStepArgs z = zargs(SM_RHS_ONLY);
+    z.rhs_out = P; z.udiff = uk;
+    post_rhs(z);
+    if (nlk) copy_spectral_out(P, nlk);
+    if (u) get_u(u, false);
+  }
+
+  // ------------------------------------------------------------ masks
+  void create_mask(double t) override {
+    if (p.iMask == "Insect" && (p.iMoving != 1 || p.iPenalization != 1))
+      throw FlusiError(4453, "insects.f90::DrawInsect: the parameters iMoving or iPenalization are wrong.");
+    timed(TM_MASK, [&] {
+      if (p.iPenalization == 1) {
+        if (p.iMask == "sphere" || p.iMask == "Sphere") {
+          SphereArgs a{};
+          a.x0 = p.x0; a.y0 = p.y0; a.z0 = p.z0; a.radius = 0.5 * p.length;
+          const double h = std::max(p.dx(), std::max(p.dy(), p.dz()));
+          a.smoothing = 1.5 * h; a.safety = 2.0 * 1.5 * h; a.color_thr = (double)1.0e-12f;
+          launch_sphere<T>(geom, a, mask, us, color, stream);
+        } else if (p.iMask == "Insect" || p.iMask == "insect") {
+          insect.update(t);
+          InsectArgs a = insect_args();
+          launch_insect<T>(geom, a, mask, us, color, stream);
+        } else if (p.iMask == "none" || p.iMask == "empty" || p.iMask == "no") {
+        } else {
+          throw FlusiError(3333, "create_mask_fsi(): unkown mask function iMask");
+        }
+      }
+      if (p.iCavity != "no" && p.iPenalization == 1) {
+        CavityArgs c{};
+        c.size = p.cavity_size;
+        if (p.iCavity == "yes") { c.ux = c.uy = c.uz = 0.0; }
+        else if (p.iCavity == "freestream") { c.ux = p.umean[0]; c.uy = p.umean[1]; c.uz = p.umean[2]; }
+        else throw FlusiError(3334, "iCavity not supported: " + p.iCavity);
+        launch_cavity<T>(geom, c, mask, us, color, stream);
+      }
+    });
+    kcheck("create_mask");
+  }
+
+  InsectArgs insect_args() {
+    Insect& I = insect;
+    InsectArgs a{};
+    const double h = std::max(p.dx(), std::max(p.dy(), p.dz()));
+    I.smooth = 1.0 * h;
+    I.safety = 3.5 * I.smooth;
+    // delete_old_mask: a resting body keeps its mask after the first call
+    a.full_reset = I.cleaned_already_once ? 0 : 1;
+    I.cleaned_already_once = true;
+    a.draw_body = 0;
+    if (!I.body_already_drawn) {
+      a.draw_body = (I.BodyType != "nobody") ? 1 : 0;
+      I.body_already_drawn = true;
+      if (I.BodyType == "jerry" || I.BodyType == "Jerry") {
+        // body_geometry.f90:535-545 hard-coded Jerry geometry
+        I.R_head = 0.125; I.R_eye = 0.0625;
+        I.x_pivot_r_b = {0.05, -0.2165, 0.0};
+        I.x_pivot_l_b = {0.05, +0.2165, 0.0};
+        I.b_body = 0.1; I.L_body = 1.0;
+        I.x_head = {0.5 * I.L_body, 0.0, 0.0};
+        const double se = std::sin(45.0 * PI / 180.0) * I.R_head * 0.8;
+        I.x_eye_r = {I.x_head[0] + se, I.x_head[1] + se, I.x_head[2] + se};
+        I.x_eye_l = {I.x_head[0] + se, I.x_head[1] - se, I.x_head[2] + se};
+      }
+    }
+    const InsectMotion& s = I.st;
+    a.body_type = (I.BodyType == "jerry" || I.BodyType == "Jerry") ? 1 : 2;
+    for (int i = 0; i < 3; ++i) a.xc[i] = s.xc[i];
+    for (int i = 0; i < 9; ++i) a.Mb[i] = s.Mb[i];
+    a.b_body = I.b_body; a.L_body = I.L_body; a.smooth = I.smooth; a.safety = I.safety;
+    a.nspheres = 0;
+    if (a.draw_body && a.body_type == 1) {
+      const Vec3 parts[3] = {I.x_head, I.x_eye_l, I.x_eye_r};
+      const double rr[3] = {I.R_head, I.R_eye, I.R_eye};
+      const double box[3] = {p.xl, p.yl, p.zl};
+      const double ddx[3] = {p.dx(), p.dy(), p.dz()};
+      const int nn[3] = {nx, ny, nz};
+      const double mind = std::min(ddx[0], std::min(ddx[1], ddx[2]));
+      for (int k = 0; k < 3; ++k) {
+        Vec3 c = mat_vec(transpose(s.Mb), parts[k]);
+        for (int i = 0; i < 3; ++i) c[i] += s.xc[i];
+        for (int i = 0; i < 3; ++i) if (c[i] < 0.0) c[i] += box[i];
+        for (int i = 0; i < 3; ++i) if (c[i] >= box[i]) c[i] -= box[i];
+        const int ns = fortran_nint((rr[k] + I.safety) / mind);
+        for (int i = 0; i < 3; ++i) {
+          a.sph_c[k][i] = c[i];
+          const int ci = fortran_nint(c[i] / ddx[i]);
+          a.sph_lo[k][i] = std::max(ci - ns, 0);
+          a.sph_hi[k][i] = std::min(ci + ns, nn[i] - 1);
+        }
+        a.sph_r[k] = rr[k];
+      }
+      a.nspheres = 3;
+    }
+    a.wing_shape = (I.WingShape == "TwoEllipses") ? 1 : 2;
+    a.L_span = I.L_span; a.b_top = I.b_top; a.b_bot = I.b_bot; a.thickness = I.WingThickness;
+    a.draw_wing[0] = I.RightWing == "yes"; a.draw_wing[1] = I.LeftWing == "yes";
+    for (int i = 0; i < 9; ++i) { a.Mw[0][i] = s.Mwr[i]; a.Mw[1][i] = s.Mwl[i]; }
+    for (int i = 0; i < 3; ++i) {
+      a.piv[0][i] = I.x_pivot_r_b[i]; a.piv[1][i] = I.x_pivot_l_b[i];
+      a.rot[0][i] = s.rot_rel_r_w[i]; a.rot[1][i] = s.rot_rel_l_w[i];
+    }
+    a.color_w[0] = 3; a.color_w[1] = 2;
+    const Vec3 vb = mat_vec(s.Mb, s.vc);
+    for (int i = 0; i < 3; ++i) { a.vb[i] = vb[i]; a.rot_body_b[i] = s.rot_body_b[i]; }
+    return a;
+  }
+
+  void get_mask(double* m, double* u_s, int16_t* col) override {
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (m) copy_real_out(mask, m, npts);
+    if (u_s) copy_real_out(us, u_s, 3 * npts);
+    if (col) {
+      std::vector<uint8_t> c(npts);
+      HIPCHECK(hipMemcpy(c.data(), color, npts, hipMemcpyDeviceToHost));
+      for (long i = 0; i < npts; ++i) col[i] = c[i];
+    }
+  }
+
+  // ------------------------------------------------------------ host copies
+  double* staging(size_t n) {
+    if (n > stage_n) {
+      if (stage) HIPCHECK(hipFree(stage));
+      stage = (double*)dalloc(n * sizeof(double));
+      stage_n = n;
+    }
+    return stage;
+  }
+  void copy_real_out(const void* src, double* dst, long n) {
+    if (FP64) { HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, stream)); }
+    else {
+      double* s = staging(n);
+      launch_to_f64<T>(src, s, n, stream);
+      HIPCHECK(hipMemcpyAsync(dst, s, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  void copy_real_in(const double* src, void* dst, long n) {
+    if (FP64) { HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, stream)); }
+    else {
+      double* s = staging(n);
+      HIPCHECK(hipMemcpyAsync(s, src, n * sizeof(double), hipMemcpyHostToDevice, stream));
+      launch_from_f64<T>(s, dst, n, stream);
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  void copy_spectral_out(const void* src, double* dst) { copy_real_out(src, dst, 6 * nc); }
+
+  void set_uk(const double* h) override {
+    need_fft();
+ copy_real_in(h, uk, 6 * nc); fields_initialized = true; }
+  void get_uk(double* h) override {
+    need_fft();
+ copy_real_out(uk, h, 6 * nc); }
+
+  void ensure_phys() { if (!phys) phys = dalloc(7 * sizeof(T) * npts); }
+  T* pf(int f) { return reinterpret_cast<T*>(phys) + (long)f * npts; }
+
+  void inv_uw_phys(const void* u_in) {
+    inv_uw(u_in);
+    ensure_phys();
+    XPassArgs a = xargs();
+    a.in = P; a.out = phys; a.fstride_in = nc; a.fstride_out = npts; a.nf = 6;
+    timed(TM_X, [&] { launch_x_pass<T>(a, XM_INV_UW, stream); });
+    kcheck("x inverse uw");
+  }
+  // 1 or 3 spectral S-layout fields -> physical
+  void ifft_fields(const void* src, int nf, void* dst) {
+    const C* s = reinterpret_cast<const C*>(src);
+    TileJob zj[3], yj[3];
+    for (int f = 0; f < nf; ++f) {
+      zj[f] = TileJob{s + (long)f * nc, nullptr, cf(Q, f), OP_COPY};
+      yj[f] = TileJob{cf(Q, f), nullptr, cf(P, f), OP_COPY};
+    }
+    zinv(zj, nf);
+    yinv(yj, nf);
+    XPassArgs a = xargs();
+    a.in = P; a.out = dst; a.fstride_in = nc; a.fstride_out = npts; a.nf = nf;
+    timed(TM_X, [&] { launch_x_pass<T>(a, XM_INV, stream); });
+    kcheck("x inverse");
+  }
+  // 1 or 3 physical fields -> S-layout spectral (times 1/N)
+  void fft_fields(const void* src, int nf, void* dst) {
+    XPassArgs a = xargs();
+    a.in = src; a.out = P; a.fstride_in = npts; a.fstride_out = nc; a.nf = nf;
+    timed(TM_X, [&] { launch_x_pass<T>(a, XM_FWD, stream); });
+    kcheck("x forward");
+    yfwd(nf);
+    zfwd_plain(nf, dst);
+  }
+
+  void get_u(double* u, bool vort) override {
+    need_fft();
+    inv_uw_phys(uk);
+    copy_real_out(vort ? (void*)pf(3) : (void*)pf(0), u, 3 * npts);
+  }
+
+  void fft1(const double* f, double* fk) override {
+    need_fft();
+    ensure_phys();
+    copy_real_in(f, pf(0), npts);
+    fft_fields(pf(0), 1, reg_scratch());
+    copy_real_out(reg_scratch(), fk, 2 * nc);
+  }
+  void ifft1(const double* fk, double* f) override {
+    need_fft();
+    ensure_phys();
+    copy_real_in(fk, reg_scratch(), 2 * nc);
+    ifft_fields(reg_scratch(), 1, pf(0));
+    copy_real_out(pf(0), f, npts);
+  }
+  void* reg_scratch() { return cf(P, 3); }  // S-layout scratch outside the RHS pipeline
+
+  // ------------------------------------------------------------ init
+  void init_fields() override {
+    // init_fields_fsi.f90:33-48
+    if (!fft_ok && p.inicond != "quiescent") need_fft();
+    time = 0.0; dt1 = p.tsave; dt0 = p.tsave; it = 0; vis_dt = -1.0;
+    const size_t cs = sizeof(C) * nc;
+    if (fft_ok) {
+      launch_zero(uk, 3 * cs, stream);
+      launch_zero(reg[0], 3 * cs, stream);
+      launch_zero(reg[1], 3 * cs, stream);
+    }
+    if (p.inicond == "VortexRing") {
+      // init_fields_fsi.f90:381-435 (core radius 0.82*Size unless core_a is given)
+      const double r00 = p.length;
+      const double a = p.core_a > 0.0 ? p.core_a : 0.82 * r00;
+      ensure_phys();
+      launch_vortex_ring<T>(geom, p.x0, p.y0, p.z0, r00, a, p.omega1, phys, stream);
+      kcheck("vortex ring");
+      fft_fields(phys, 3, reg[1]);
+      launch_biot_savart<T>(geom, kx_d, ky_d, kz_d, reg[1], uk, stream);
+      kcheck("biot-savart");
+      launch_zero(reg[1], 3 * cs, stream);
+      set_mean_flow(0.0);
+    } else if (p.inicond == "MeanFlow") {
+      need_fft();
+      for (int c = 0; c < 3; ++c) launch_set_mode0<T>(geom, uk, c, p.umean[c], 0.0, stream);
+    } else if (p.inicond == "quiescent") {
+    } else {
+      throw FlusiError(1, "inicond not supported: " + p.inicond);
+    }
+    n1 = 1; n0 = 0;
+    if (p.iMask == "Insect" && p.iPenalization == 1) {
+      is_insect = true;
+      insect = Insect();
+      insect.init(*ini, p, base_dir);
+    }
+    if (p.iPenalization == 1) create_mask(time);
+    HIPCHECK(hipStreamSynchronize(stream));
+    fields_initialized = true;
+  }
+  void set_mean_flow(double t) {
+    for (int c = 0; c < 3; ++c) {
+      if (p.iMeanFlow[c] == "fixed") launch_set_mode0<T>(geom, uk, c, p.umean[c], 0.0, stream);
+      else if (p.iMeanFlow[c] == "sinusoidal")
+        launch_set_mode0<T>(geom, uk, c, p.umean[c] * std::sin(2.0 * PI * p.umean_freq * t), 0.0, stream);
+    }
+  }
+
+  // ------------------------------------------------------------ diagnostics
+  double energy_k() override {
+    need_fft();
+    std::vector<double> h(6 * nc);
+    get_uk(h.data());
+    double e = 0.0;
+    for (int c = 0; c < 3; ++c)
+      for (int ix = 0; ix < nxh; ++ix) {
+        const double w = (ix == 0 || ix == nx / 2) ? 0.5 : 1.0;
+        for (int iy = 0; iy < ny; ++iy)
+          for (int iz = 0; iz < nz; ++iz) {
+            const long k = (long)c * nc + ((long)ix * ny + iy) * nz + iz;
+            e += w * (h[2 * k] * h[2 * k] + h[2 * k + 1] * h[2 * k + 1]);
+          }
+      }
+    return e * p.xl * p.yl * p.zl;
+  }
+
+  void cal_vis(double dt, double* vis) override {
+    // vis.f90:9-66, host side (only needed at the boundary)
+    for (int ix = 0; ix < nxh; ++ix)
+      for (int iy = 0; iy < ny; ++iy)
+        for (int iz = 0; iz < nz; ++iz) {
+          const double kx2 = kx_h[ix] * kx_h[ix], ky2 = ky_h[iy] * ky_h[iy], kz2 = kz_h[iz] * kz_h[iz];
+          const bool alias = ((kxt2_h[ix] + kyt2_h[iy]) + kzt2_h[iz]) >= 1.0;
+          vis[((long)ix * ny + iy) * nz + iz] = (alias && p.iDealias == 1) ? 0.0 : std::exp(-dt * p.nu * ((kx2 + ky2) + kz2));
+        }
+  }
+
+  // integrals.f90:48-238 write_integrals_fsi
+  void write_integrals(double t, double* out) override {
+    need_fft();
+    if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    const bool ins = is_insect;
+    if (ins) append("kinematics.t", insect.kinematics_row(t), kin_header());
+    inv_uw_phys(uk);
+    const bool forces = p.compute_forces == 1 && p.iPenalization == 1;
+    if (p.compute_forces == 1 && p.iMoving == 1) create_mask(t);
+    // divergence (basic_operators.f90:707-734) -> physical
+    void* dk = reg_scratch();
+    launch_divergence<T>(geom, kx_d, ky_d, kz_d, uk, dk, stream);
+    kcheck("divergence");
+    ifft_fields(dk, 1, pf(6));
+    DiagArgs d{};
+    d.u = pf(0); d.w = pf(3); d.div = pf(6); d.mask = mask; d.us = us; d.color = color;
+    d.eps_inv = 1.0 / p.eps;
+    d.x0[0] = p.x0; d.x0[1] = p.y0; d.x0[2] = p.z0;
+    d.insect = ins ? 1 : 0;
+    if (ins) for (int i = 0; i < 3; ++i) { d.piv_l[i] = insect.st.x_pivot_l_g[i]; d.piv_r[i] = insect.st.x_pivot_r_g[i]; }
+    d.partial = dpart;
+    timed(TM_DIAG, [&] { launch_diag<T>(geom, d, stream); });
+    kcheck("diag");
+    launch_reduce_rows(dpart, ndblocks, DIAG_NV, kinds + 8, red + 16, stream);
+    kcheck("reduce diag");
+    std::vector<double> r(DIAG_NV);
+    HIPCHECK(hipMemcpyAsync(r.data(), red + 16, sizeof(double) * DIAG_NV, hipMemcpyDeviceToHost, stream));
+    std::vector<C> m0(3);
+    for (int c = 0; c < 3; ++c)
+      HIPCHECK(hipMemcpyAsync(&m0[c], cf(uk, c), sizeof(C), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    const double dv = p.dx() * p.dy() * p.dz();
+    double o[FLUSI_INTEGRALS_NV] = {0};
+    double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
+    for (int c = 1; c < 4; ++c)
+      for (int i = 0; i < 3; ++i) { F[i] += r[DG_FORCE + 3 * c + i] * dv; Tq[i] += r[DG_TORQUE0 + 3 * c + i] * dv; }
+    if (forces) {
+      double pw[3];
+      for (int i = 0; i < 3; ++i) pw[i] = r[DG_POWER + i] * dv;
+      last_power_total = pw[0] + pw[1] + pw[2];
+      last_power4_xyz = {pw[0], pw[1], pw[2]};
+    }
+    for (int i = 0; i < 3; ++i) { o[i] = F[i]; o[3 + i] = Tq[i]; }
+    o[6] = last_power_total; o[7] = last_power4_xyz[0]; o[8] = last_power4_xyz[1]; o[9] = last_power4_xyz[2];
+    const double ekf[3] = {0.5 * dv * r[DG_EKF], 0.5 * dv * r[DG_EKF + 1], 0.5 * dv * r[DG_EKF + 2]};
+    const double ek[3] = {0.5 * dv * r[DG_EK], 0.5 * dv * r[DG_EK + 1], 0.5 * dv * r[DG_EK + 2]};
+    const double dsf[3] = {p.nu * dv * r[DG_DISSF], p.nu * dv * r[DG_DISSF + 1], p.nu * dv * r[DG_DISSF + 2]};
+    const double ds[3] = {p.nu * dv * r[DG_DISS], p.nu * dv * r[DG_DISS + 1], p.nu * dv * r[DG_DISS + 2]};
+    auto put4 = [&](int off, const double* v3, double tot) { o[off] = tot; o[off + 1] = v3[0]; o[off + 2] = v3[1]; o[off + 3] = v3[2]; };
+    put4(10, ekf, 0.5 * dv * (r[DG_EKF] + r[DG_EKF + 1] + r[DG_EKF + 2]));
+    put4(14, dsf, p.nu * dv * (r[DG_DISSF] + r[DG_DISSF + 1] + r[DG_DISSF + 2]));
+    put4(18, ek, 0.5 * dv * (r[DG_EK] + r[DG_EK + 1] + r[DG_EK + 2]));
+    put4(22, ds, p.nu * dv * (r[DG_DISS] + r[DG_DISS + 1] + r[DG_DISS + 2]));
+    o[26] = r[DG_DIVMAX]; o[27] = r[DG_DIVMAXF];
+    o[28] = r[DG_MASKVOL] * dv;
+    for (int i = 0; i < 3; ++i) o[29 + i] = r[DG_MASKUS + i] * dv;
+    for (int c = 1; c < 4; ++c)
+      for (int i = 0; i < 3; ++i) { o[32 + 3 * (c - 1) + i] = r[DG_FORCE + 3 * c + i] * dv; o[44 + 3 * (c - 1) + i] = r[DG_TORQUE + 3 * c + i] * dv; }
+    for (int c = 0; c < 6; ++c) o[53 + c] = std::sqrt(c < 4 ? r[DG_URES + c] : 9.0e9);
+    if (ins) {
+      // module_insects.f90:469-586 aero_power / inert_power
+      const InsectMotion& s = insect.st;
+      double ap_l = 0, ap_r = 0;
+      for (int i = 0; i < 3; ++i) { ap_l -= o[44 + 3 + i] * s.rot_rel_l_g[i]; ap_r -= o[44 + 6 + i] * s.rot_rel_r_g[i]; }
+      auto ipow = [&](const Vec3& rw, const Vec3& rdt) {
+        const Insect& I = insect;
+        const double a0 = I.Jxx * rdt[0] + I.Jxy * rdt[1], a1 = I.Jxy * rdt[0] + I.Jyy * rdt[1], a2 = I.Jzz * rdt[2];
+        const double b0 = I.Jxx * rw[0] + I.Jxy * rw[1], b1 = I.Jxy * rw[0] + I.Jyy * rw[1], b2 = I.Jzz * rw[2];
+        return rw[0] * (a0 + rw[1] * b2 - rw[2] * b1) + rw[1] * (a1 + rw[2] * b0 - rw[0] * b2) + rw[2] * (a2 + rw[0] * b1 - rw[1] * b0);
+      };
+      const double ip_l = ipow(s.rot_rel_l_w, s.rot_dt_l_w), ip_r = ipow(s.rot_rel_r_w, s.rot_dt_r_w);
+      o[59] = 0.0 + ap_l + ap_r;
+      o[60] = ip_r + ip_l;
+      if (forces) {
+        append("forces.t", {t, F[0], F[1], F[2], 0, 0, 0, Tq[0], Tq[1], Tq[2], 0, 0, 0, o[59], o[60]}, forces_header(true));
+        const double apc[3] = {0.0, ap_l, ap_r}, ipc[3] = {0.0, ip_l, ip_r};
+        for (int c = 1; c <= 3; ++c) {
+          std::vector<double> row = {t};
+          for (int i = 0; i < 3; ++i) row.push_back(o[32 + 3 * (c - 1) + i]);
+          row.insert(row.end(), {0, 0, 0});
+          for (int i = 0; i < 3; ++i) row.push_back(o[44 + 3 * (c - 1) + i]);
+          row.insert(row.end(), {0, 0, 0, apc[c - 1], ipc[c - 1]});
+          append("forces_part" + std::to_string(c) + ".t", row, forces_header(true));
+        }
+      }
+    } else if (forces) {
+      append("forces.t", {t, F[0], F[1], F[2], 0, 0, 0, Tq[0], Tq[1], Tq[2], 0, 0, 0}, forces_header(false));
+    }
+    if (forces) append("u_residual.t", {t, o[53], o[54], o[55], o[56], o[57], o[58]}, "%          time     u_residual(0:5)");
+    append("divu.t", {t, o[26], o[27]}, "%          time         max_div   max_div_fluid");
+    std::vector<double> erow = {t};
+    for (int i = 10; i < 26; ++i) erow.push_back(o[i]);
+    for (int i = 6; i < 10; ++i) erow.push_back(o[i]);
+    append("energy.t", erow, "%          time         E_kin_f       E_kin_x_f       E_kin_y_f       E_kin_z_f          diss_f        diss_x_f        diss_y_f        diss_z_f       E_kin_tot     E_kin_tot_x     E_kin_tot_y     E_kin_tot_z        diss_tot      diss_tot_x      diss_tot_y      diss_tot_z      flux_penal");
+    append("meanflow.t", {t, (double)m0[0].x, (double)m0[1].x, (double)m0[2].x}, "%          time          uxmean          uymean          uzmean");
+    append("mask_volume.t", {t, o[28], o[29], o[30], o[31]}, "%          time     mask_volume      usx_integral    usy_integral    usz_integral");
+    if (out) std::memcpy(out, o, sizeof(o));
+  }
+  static const char* kin_header() {
+    return "%          time       xc_body_g         yc_body         zc_body             psi            beta           gamma      eta_stroke         alpha_l           phi_l         theta_l         alpha_r           phi_r         theta_r         rot_l_x         rot_l_y         rot_l_z         rot_r_x         rot_r_y         rot_r_z      rot_dt_l_x      rot_dt_l_y      rot_dt_l_z      rot_dt_r_x      rot_dt_r_y      rot_dt_r_z";
+  }
+  static const char* forces_header(bool ins) {
+    return ins ? "%          time          Forcex          Forcey          Forcez     Forcex_unst     Forcey_unst     Forcez_unst         Momentx         Momenty         Momentz    Momentx_unst    Momenty_unst    Momentz_unst        Powerx     Inertial_power"
+               : "%          time          Forcex          Forcey          Forcez     Forcex_unst     Forcey_unst     Forcez_unst         Momentx         Momenty         Momentz    Momentx_unst    Momenty_unst    Momentz_unst";
+  }
+};
+
+}  // namespace flusi
+
+// =========================================================================== C ABI
+struct flusi_ctx {
+  std::unique_ptr<flusi::SolverBase> s;
+  std::string err;
+};
+
+using flusi::FlusiError;
+
+template <typename F> static int guarded(flusi_ctx* c, F&& f) {
+  if (!c || !c->s) return 900000;
+  try {
+    f();
+    return 0;
+  } catch (const FlusiError& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return 900099;
+  }
+}
+
+extern "C" {
+
+int flusi_init(const char* ini_path, int precision, flusi_ctx** out) {
+  if (!out) return 900000;
+  *out = nullptr;
+  auto ctx = new flusi_ctx();
+  try {
+    if (precision == FLUSI_FP64) ctx->s.reset(new flusi::Solver<double>());
+    else if (precision == FLUSI_FP32_MIXED) ctx->s.reset(new flusi::Solver<float>());
+    else throw FlusiError(900005, "precision must be FLUSI_FP64 or FLUSI_FP32_MIXED");
+    flusi::SolverBase* s = ctx->s.get();
+    s->ini.reset(new flusi::IniFile(ini_path));
+    s->p = flusi::load_params(*s->ini);
+    std::string path(ini_path);
+    const size_t sl = path.find_last_of('/');
+    s->base_dir = (sl == std::string::npos) ? "." : path.substr(0, sl);
+    if (precision == FLUSI_FP64) static_cast<flusi::Solver<double>*>(s)->setup();
+    else static_cast<flusi::Solver<float>*>(s)->setup();
+  } catch (const FlusiError& e) {
+    ctx->err = e.what();
+    const int code = e.code;
+    *out = ctx;  // keep the context so flusi_last_error can report
+    ctx->s.reset();
+    return code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    *out = ctx;
+    ctx->s.reset();
+    return 900099;
+  }
+  *out = ctx;
+  return 0;
+}
+
+void flusi_free(flusi_ctx* ctx) { delete ctx; }
+
+const char* flusi_last_error(const flusi_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int flusi_get_dims(const flusi_ctx* ctx, int* nx, int* ny, int* nz) {
+  if (!ctx || !ctx->s) return 900000;
+  *nx = ctx->s->nx; *ny = ctx->s->ny; *nz = ctx->s->nz;
+  return 0;
+}
+
+int flusi_set_output_dir(flusi_ctx* ctx, const char* dir) {
+  return guarded(ctx, [&] { ctx->s->out_dir = dir ? dir : ""; });
+}
+
+int flusi_init_fields(flusi_ctx* ctx) { return guarded(ctx, [&] { ctx->s->init_fields(); }); }
+int flusi_set_uk(flusi_ctx* ctx, const double* uk) { return guarded(ctx, [&] { ctx->s->set_uk(uk); }); }
+int flusi_get_uk(flusi_ctx* ctx, double* uk) { return guarded(ctx, [&] { ctx->s->get_uk(uk); }); }
+int flusi_get_u(flusi_ctx* ctx, double* u) { return guarded(ctx, [&] { ctx->s->get_u(u, false); }); }
+int flusi_get_vorticity(flusi_ctx* ctx, double* w) { return guarded(ctx, [&] { ctx->s->get_u(w, true); }); }
+int flusi_cal_nlk(flusi_ctx* ctx, double time, int it, double* nlk, double* u) {
+  (void)it;
+  return guarded(ctx, [&] { ctx->s->cal_nlk(time, nlk, u); });
+}
+int flusi_fluid_timestep(flusi_ctx* ctx, double* time, int* it, double* dt0, double* dt1) {
+  return guarded(ctx, [&] {
+    flusi::SolverBase* s = ctx->s.get();
+    if (time) s->time = *time;
+    if (it) s->it = *it;
+    if (dt0) s->dt0 = *dt0;
+    s->fluid_timestep();
+    if (dt1) *dt1 = s->dt1;
+  });
+}
+int flusi_step(flusi_ctx* ctx) { return guarded(ctx, [&] { ctx->s->step(); }); }
+int flusi_run(flusi_ctx* ctx, int nsteps) {
+  return guarded(ctx, [&] {
+    flusi::SolverBase* s = ctx->s.get();
+    if (!s->fields_initialized) s->init_fields();
+    s->write_integrals(s->time, nullptr);
+    int n = 0;
+    while (s->time < s->p.tmax && s->it <= s->p.nt && (nsteps <= 0 || n < nsteps)) { s->step(); ++n; }
+  });
+}
+int flusi_advance(flusi_ctx* ctx, int nsteps) {
+  return guarded(ctx, [&] { for (int i = 0; i < nsteps; ++i) ctx->s->step(); });
+}
+int flusi_get_time(const flusi_ctx* ctx, double* time, int* it, double* dt0, double* dt1) {
+  if (!ctx || !ctx->s) return 900000;
+  if (time) *time = ctx->s->time;
+  if (it) *it = ctx->s->it;
+  if (dt0) *dt0 = ctx->s->dt0;
+  if (dt1) *dt1 = ctx->s->dt1;
+  return 0;
+}
+int flusi_create_mask(flusi_ctx* ctx, double time) { return guarded(ctx, [&] { ctx->s->create_mask(time); }); }
+int flusi_get_mask(flusi_ctx* ctx, double* mask, double* us, int16_t* color) {
+  return guarded(ctx, [&] { ctx->s->get_mask(mask, us, color); });
+}
+int flusi_write_integrals(flusi_ctx* ctx, double time, double* out) {
+  return guarded(ctx, [&] { ctx->s->write_integrals(time, out); });
+}
+int flusi_energy_k(flusi_ctx* ctx, double* E) { return guarded(ctx, [&] { *E = ctx->s->energy_k(); }); }
+int flusi_cal_vis(flusi_ctx* ctx, double dt, double* vis) { return guarded(ctx, [&] { ctx->s->cal_vis(dt, vis); }); }
+int flusi_fft(flusi_ctx* ctx, const double* f, double* fk) { return guarded(ctx, [&] { ctx->s->fft1(f, fk); }); }
+int flusi_ifft(flusi_ctx* ctx, const double* fk, double* f) { return guarded(ctx, [&] { ctx->s->ifft1(fk, f); }); }
+int flusi_timers(flusi_ctx* ctx, int enable, double* ms, const char** names) {
+  return guarded(ctx, [&] {
+    ctx->s->timers(enable, ms);
+    if (names) for (int i = 0; i < FLUSI_TIMER_SLOTS; ++i) names[i] = flusi::kTimerNames[i];
+  });
+}
+
+}  // extern "C"

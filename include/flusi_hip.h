@@ -1,0 +1,110 @@
+/*
+ * flusi_hip.h -- C ABI of the MI355X implementation of FLUSI's fluid hot path.
+ *
+ * Every entry point returns 0 on success or the reference's abort code
+ * (vars.f90:327-336 abort(code,msg)); flusi_last_error() returns the message.
+ * Arrays crossing the boundary are plain host arrays in the reference's
+ * single-rank layouts (see below); the step loop keeps all fields resident on
+ * the GPU.  One context per GPU, one HIP stream, single host thread, not
+ * re-entrant (as the reference, which is one thread per MPI rank).
+ *
+ * Layouts (0-based, first index fastest, as the Fortran arrays):
+ *   real field       u(ix,iy,iz,comp)   nx*ny*nz doubles per component
+ *   spectral field   uk(iz,iy,ix,comp)  complex(8) interleaved re/im,
+ *                    nz*ny*(nx/2+1) values per component, ix <-> kx >= 0
+ *   mask_color       int16 (ix,iy,iz)   (vars.f90:9)
+ */
+#ifndef FLUSI_HIP_H
+#define FLUSI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct flusi_ctx flusi_ctx;
+
+#define FLUSI_FP64 0        /* plain FP64 solver (the reference's pr=8) */
+#define FLUSI_FP32_MIXED 1  /* FP32 FFTs/storage, FP64 pointwise, IF and reductions */
+
+/* Reads the .ini (params.f90:3-44 get_params, ini_files_parser.f90) and
+ * allocates the device state (flusi.f90:112-286 Start_Simulation). */
+int flusi_init(const char* ini_path, int precision, flusi_ctx** ctx);
+void flusi_free(flusi_ctx* ctx);
+const char* flusi_last_error(const flusi_ctx* ctx);
+int flusi_get_dims(const flusi_ctx* ctx, int* nx, int* ny, int* nz);
+
+/* Directory for the .t time series (forces.t, energy.t, ekin.t, dt.t,
+ * kinematics.t, ...); NULL or "" disables file output. */
+int flusi_set_output_dir(flusi_ctx* ctx, const char* dir);
+
+/* inicond/init_fields.f90:2-114 (VortexRing, MeanFlow, quiescent), the
+ * insect initialisation (insect_init_clean.f90:1-250) and the first
+ * create_mask. */
+int flusi_init_fields(flusi_ctx* ctx);
+
+/* State copies in the reference layouts (uk: 3 components). */
+int flusi_set_uk(flusi_ctx* ctx, const double* uk);
+int flusi_get_uk(flusi_ctx* ctx, double* uk);
+/* u = ifft3(uk) (cof_p3dfft.f90:122-145), 3 real components. */
+int flusi_get_u(flusi_ctx* ctx, double* u);
+/* vorticity = ifft3(curl(uk)) (basic_operators.f90:185-215). */
+int flusi_get_vorticity(flusi_ctx* ctx, double* vort);
+
+/* rhs.f90:2-115 cal_nlk: projected right-hand side of the current uk at
+ * `time` (mask regenerated when iMoving=1).  nlk: 3 spectral components or
+ * NULL; u: physical velocity (3 components) or NULL. */
+int flusi_cal_nlk(flusi_ctx* ctx, double time, int it, double* nlk, double* u);
+
+/* fluid_time_step.f90:34-150 FluidTimestep for the configured stepper
+ * (AB2 with euler_startup at it=0, RK2, RK4): advances uk from `time` by one
+ * step and returns dt1; dt0 is the previous step (as in time_step.f90:61). */
+int flusi_fluid_timestep(flusi_ctx* ctx, double* time, int* it, double* dt0, double* dt1);
+
+/* One iteration of time_step.f90:57-167: dt0=dt1, FluidTimestep, swap
+ * n0/n1, time+=dt1, it+=1, write_integrals on the tintegral/itdrag cadence. */
+int flusi_step(flusi_ctx* ctx);
+/* time_step.f90:1-180: initial write_integrals, then steps while
+ * time<Tmax and it<=nt; nsteps>0 stops after nsteps iterations. */
+int flusi_run(flusi_ctx* ctx, int nsteps);
+/* nsteps iterations of flusi_step without the initial write_integrals. */
+int flusi_advance(flusi_ctx* ctx, int nsteps);
+int flusi_get_time(const flusi_ctx* ctx, double* time, int* it, double* dt0, double* dt1);
+
+/* mask.f90:2-31 create_mask at `time` (sphere, tethered insect, cavity). */
+int flusi_create_mask(flusi_ctx* ctx, double time);
+int flusi_get_mask(flusi_ctx* ctx, double* mask, double* us, int16_t* color);
+
+/* integrals.f90:48-238 write_integrals_fsi at `time` from the current uk:
+ * cal_drag (drag.f90:17-454), energies, divergence, mean flow, mask volume.
+ * out (may be NULL) receives FLUSI_INTEGRALS_NV values:
+ *   [0..2] Force (colours 1..5)   [3..5] Torque about (x0,y0,z0)
+ *   [6..9] penalisation power (total, x, y, z)
+ *   [10..13] E_kin_f (tot,x,y,z) [14..17] diss_f  [18..21] E_kin  [22..25] diss
+ *   [26] max div  [27] max div in fluid  [28] mask volume  [29..31] sum mask*us dV
+ *   [32..43] per-part force (body, left, right)   [44..52] per-part torque
+ *   [53..58] sqrt(u_residual) colours 0..5  [59] aero power [60] inertial power */
+#define FLUSI_INTEGRALS_NV 61
+int flusi_write_integrals(flusi_ctx* ctx, double time, double* out);
+
+/* integrals.f90:522-552 compute_energies_k of the current uk. */
+int flusi_energy_k(flusi_ctx* ctx, double* E);
+
+/* vis.f90:9-66 cal_vis (real, spectral layout). */
+int flusi_cal_vis(flusi_ctx* ctx, double dt, double* vis);
+
+/* cof_p3dfft.f90:33-83 fft (R2C times 1/N) and ifft (C2R, unnormalised) of
+ * one field in the reference layouts, through the device passes. */
+int flusi_fft(flusi_ctx* ctx, const double* f, double* fk);
+int flusi_ifft(flusi_ctx* ctx, const double* fk, double* f);
+
+/* Device time in milliseconds accumulated per pass since the last reset
+ * (module_timing.f90 toc slots); names[i] are static strings. */
+#define FLUSI_TIMER_SLOTS 8
+int flusi_timers(flusi_ctx* ctx, int enable, double* ms, const char** names);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLUSI_HIP_H */

@@ -1,0 +1,1471 @@
+"""FP64 CPU restatement of FLUSI's fluid hot path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the parity oracle for the HIP implementation in ``flusi_amd``.
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import it, and only as the checker; the product path never
+calls into it.
+
+Every routine restates one reference routine (file:line given in its
+docstring, paths relative to the reference's ``src/``).  Arrays use the
+reference's single-rank memory layouts, written as C-order numpy shapes:
+
+* physical real field   (ix,iy,iz), ix fastest  -> numpy shape (nz, ny, nx)
+* spectral half field   (iz,iy,ix), iz fastest  -> numpy shape (nx/2+1, ny, nz)
+* vector fields carry a leading component axis of length 3.
+
+The FFT is pocketfft as shipped in scipy 1.15.3 / numpy 2.2.6 (third-party, not
+the reference's P3DFFT/FFTW).  Its normalisation is set to the reference's:
+forward multiplied by 1/N, inverse unnormalised (cof_p3dfft.f90:33-83).  It is
+pinned by the closed-form checks of the reference's own start-up self test
+(fft_unit_test.f90:81-253) and brute-force DFTs in tests/test_oracle_core.py.
+
+Pinning status of each function (what fails if it is wrong) is listed in
+DESIGN.md; functions without an independent check say "parity unpinned" in
+their docstring.
+"""
+from __future__ import annotations
+
+import copy
+import math
+import os
+import re
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import scipy.fft as sfft
+
+PI = 3.1415926535897932384626433832795028841971693993751058209749445923078164
+
+_WORKERS = int(os.environ.get("FLUSI_ORACLE_WORKERS", "1"))
+
+
+def set_workers(n: int) -> None:
+    """Number of pocketfft threads the oracle uses (1 = scalar restatement)."""
+    global _WORKERS
+    _WORKERS = max(1, int(n))
+
+
+class FlusiAbort(RuntimeError):
+    """Raised where the reference calls abort(code, msg) (vars.f90:327-336)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"abort {code}: {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------------------
+# INI parser (ini_files_parser.f90:249-319 read_ini_file, :825-891 GetValue,
+# :337-373 param_dbl with the '*dx' unit suffix)
+# ---------------------------------------------------------------------------
+class IniFile:
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise FlusiAbort(1, f"file {path} not found")
+        self.path = path
+        self.lines: List[str] = []
+        with open(path, "r") as fh:
+            for raw in fh.read().splitlines():
+                line = raw.lstrip()
+                if ";" in line:
+                    line = line[: line.index(";") + 1]
+                if line[:1] in (";", "!", "#", "%"):
+                    line = ""
+                self.lines.append(line)
+        self.dx: Optional[float] = None
+
+    def get_value(self, section: str, keyword: str) -> str:
+        found = False
+        for line in self.lines:
+            if line[:1] in ("#", ";", "!", "%"):
+                continue
+            if line.startswith("[" + section + "]"):
+                found = True
+            elif line[:1] == "[":
+                found = False
+            if found and line.startswith(keyword + "="):
+                i1 = line.index("=") + 1
+                if ";" in line:
+                    return line[i1: line.index(";")].strip()
+                rest = line[i1:]
+                sp = rest.find(" ")
+                return rest if sp < 0 else rest[:sp]
+        return ""
+
+    @staticmethod
+    def _fortran_real(tok: str) -> float:
+        tok = tok.strip().replace("d", "e").replace("D", "E")
+        tok = re.split(r"[\s,]+", tok)[0]
+        return float(tok)
+
+    def dbl(self, section: str, keyword: str, default: float) -> float:
+        v = self.get_value(section, keyword)
+        if v == "":
+            return float(default)
+        if "*dx" not in v:
+            return self._fortran_real(v)
+        if self.dx is None:
+            raise FlusiAbort(1, "relative value read before dx is set")
+        return self._fortran_real(v[: v.index("*dx")]) * self.dx
+
+    def int(self, section: str, keyword: str, default: int) -> int:
+        v = self.get_value(section, keyword)
+        if v == "":
+            return int(default)
+        return int(re.split(r"[\s,]+", v.strip())[0])
+
+    def str(self, section: str, keyword: str, default: str) -> str:
+        v = self.get_value(section, keyword)
+        return default if v == "" else v
+
+    def vct(self, section: str, keyword: str, default) -> np.ndarray:
+        v = self.get_value(section, keyword)
+        if v == "":
+            return np.array(default, dtype=np.float64)
+        toks = [t for t in re.split(r"[\s,]+", v.strip()) if t]
+        vals = [self._fortran_real(t) for t in toks]
+        return np.array(vals[: len(default)], dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# Parameters (params.f90:51-243 get_params_common, :247-372 get_params_fsi)
+# ---------------------------------------------------------------------------
+@dataclass
+class Params:
+    nx: int = 4
+    ny: int = 4
+    nz: int = 4
+    xl: float = 2 * PI
+    yl: float = 2 * PI
+    zl: float = 2 * PI
+    length: float = 0.0
+    nt: int = 9999999
+    iTimeMethodFluid: str = "AB2"
+    intelligent_dt: str = "no"
+    tmax: float = 1e9
+    cfl: float = 0.1
+    cfl_eta: float = 0.99
+    dt_max: float = 0.0
+    dt_fixed: float = 0.0
+    nu: float = 1e-2
+    inicond: str = "none"
+    omega1: float = 0.0
+    core_a: float = 0.0       # build extension: vortex-ring core radius (0 -> 0.82*Size)
+    iDealias: int = 1
+    iPenalization: int = 0
+    iMoving: int = 0
+    iMask: str = "none"
+    eps: float = 1e-2
+    periodic: bool = False
+    tsave: float = 9e9
+    tintegral: float = 0.01
+    tsave_first: float = 0.0
+    itdrag: int = 99999
+    iCavity: str = "no"
+    cavity_size: int = 0
+    compute_forces: int = 1
+    thick_wall: float = 0.2
+    iVorticitySponge: str = "no"
+    iSpongeType: str = "top_cover"
+    eps_sponge: float = 0.0
+    sponge_thickness: int = 0
+    x0: float = 0.0
+    y0: float = 0.0
+    z0: float = 0.0
+    iMeanFlow: List[str] = field(default_factory=lambda: ["free", "free", "free"])
+    m_fluid: float = 999e9
+    umean: List[float] = field(default_factory=lambda: [1.0, 1.0, 1.0])
+    umean_freq: float = 0.0
+    iMeanFlowStartupConditioner: str = "no"
+    tau_meanflow: float = 0.0
+    T_release_meanflow: float = 0.0
+    ini_path: str = ""
+
+    @property
+    def dx(self):
+        return self.xl / self.nx
+
+    @property
+    def dy(self):
+        return self.yl / self.ny
+
+    @property
+    def dz(self):
+        return self.zl / self.nz
+
+
+def get_params(path: str) -> Tuple[Params, IniFile]:
+    """params.f90:3-44 get_params, keys and defaults of get_params_common/_fsi."""
+    ini = IniFile(path)
+    p = Params(ini_path=path)
+    p.nx = ini.int("Resolution", "nx", 4)
+    p.ny = ini.int("Resolution", "ny", 4)
+    p.nz = ini.int("Resolution", "nz", 4)
+    p.xl = ini.dbl("Geometry", "xl", 6.283185307179586)
+    p.yl = ini.dbl("Geometry", "yl", 6.283185307179586)
+    p.zl = ini.dbl("Geometry", "zl", 6.283185307179586)
+    ini.dx = p.xl / p.nx
+    p.length = ini.dbl("Geometry", "Size", 0.0)
+    p.nt = ini.int("Time", "nt", 9999999)
+    p.iTimeMethodFluid = ini.str("Time", "iTimeMethodFluid", "AB2")
+    p.intelligent_dt = ini.str("Time", "intelligent_dt", "no")
+    p.tmax = ini.dbl("Time", "Tmax", 1e9)
+    p.cfl = ini.dbl("Time", "CFL", 0.1)
+    p.cfl_eta = ini.dbl("Time", "CFL_eta", 0.99)
+    p.dt_max = ini.dbl("Time", "dt_max", 0.0)
+    p.dt_fixed = ini.dbl("Time", "dt_fixed", 0.0)
+    p.nu = ini.dbl("ReynoldsNumber", "nu", 1e-2)
+    p.inicond = ini.str("InitialCondition", "inicond", "none")
+    p.omega1 = ini.dbl("InitialCondition", "omega1", 0.0)
+    p.core_a = ini.dbl("InitialCondition", "core_a", 0.0)
+    p.iDealias = ini.int("Dealiasing", "iDealias", 1)
+    p.iPenalization = ini.int("Penalization", "iPenalization", 0)
+    p.iMoving = ini.int("Penalization", "iMoving", 0)
+    p.iMask = ini.str("Penalization", "iMask", "none")
+    p.eps = ini.dbl("Penalization", "eps", 1e-2)
+    p.periodic = ini.str("Penalization", "periodic", "no") == "yes"
+    p.tsave = ini.dbl("Saving", "tsave", 9e9)
+    p.tintegral = ini.dbl("Saving", "tintegral", 0.01)
+    p.tsave_first = ini.dbl("Saving", "tsave_first", 0.0)
+    p.itdrag = ini.int("Saving", "itdrag", 99999)
+    p.iCavity = ini.str("Penalization", "iCavity", "no")
+    p.cavity_size = ini.int("Penalization", "cavity_size", 0)
+    p.compute_forces = ini.int("Penalization", "compute_forces", 1)
+    p.thick_wall = ini.dbl("Penalization", "thick_wall", 0.2)
+    p.iVorticitySponge = ini.str("Sponge", "iVorticitySponge", "no")
+    p.iSpongeType = ini.str("Sponge", "iSpongeType", "top_cover")
+    p.eps_sponge = ini.dbl("Sponge", "eps_sponge", 0.0)
+    p.sponge_thickness = ini.int("Sponge", "sponge_thickness", 0)
+    p.x0 = ini.dbl("Geometry", "x0", p.xl / 2)
+    p.y0 = ini.dbl("Geometry", "y0", p.yl / 2)
+    p.z0 = ini.dbl("Geometry", "z0", p.zl / 2)
+    mf = [ini.str("MeanFlow", "iMeanFlow_" + c, "free") for c in "xyz"]
+    if ini.str("MeanFlow", "iMeanFlow", "unused") == "1":
+        mf = ["fixed", "fixed", "fixed"]
+    p.iMeanFlow = mf
+    p.m_fluid = ini.dbl("MeanFlow", "m_fluid", 999e9)
+    p.umean = [ini.dbl("MeanFlow", c, 1.0) for c in ("ux", "uy", "uz")]
+    p.umean_freq = ini.dbl("MeanFlow", "umean_freq", 0.0)
+    p.iMeanFlowStartupConditioner = ini.str("MeanFlow", "iMeanFlowStartupConditioner", "no")
+    p.tau_meanflow = ini.dbl("MeanFlow", "tau_meanflow", 0.0)
+    p.T_release_meanflow = ini.dbl("MeanFlow", "T_release_meanflow", 0.0)
+    return p, ini
+
+
+# ---------------------------------------------------------------------------
+# Grid, wavenumbers, FFT
+# ---------------------------------------------------------------------------
+class Grid:
+    """Wavenumbers of cof_p3dfft.f90:661-680 and the dealias/IF predicate of
+    vis.f90:23-53 / basic_operators.f90:964-984, all in the spectral layout."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        nx, ny, nz = p.nx, p.ny, p.nz
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nxh = nx // 2 + 1
+        self.scalex = 2.0 * PI / p.xl
+        self.scaley = 2.0 * PI / p.yl
+        self.scalez = 2.0 * PI / p.zl
+        ix = np.arange(self.nxh)
+        iy = np.arange(ny)
+        iz = np.arange(nz)
+        self.kx1 = self.scalex * ix.astype(np.float64)
+        self.ky1 = self.scaley * (np.mod(iy + ny // 2, ny) - ny // 2).astype(np.float64)
+        self.kz1 = self.scalez * (np.mod(iz + nz // 2, nz) - nz // 2).astype(np.float64)
+        self.KX = self.kx1[:, None, None]
+        self.KY = self.ky1[None, :, None]
+        self.KZ = self.kz1[None, None, :]
+        self.K2 = (self.KX * self.KX + self.KY * self.KY) + self.KZ * self.KZ
+        kxt = (2.0 / 3.0) * float(nx // 2 - 1)
+        kyt = (2.0 / 3.0) * float(ny // 2 - 1)
+        kzt = (2.0 / 3.0) * float(nz // 2 - 1)
+        kxt2 = ((self.kx1 / self.scalex) / kxt) ** 2
+        kyt2 = ((self.ky1 / self.scaley) / kyt) ** 2
+        kzt2 = ((self.kz1 / self.scalez) / kzt) ** 2
+        self.alias = ((kxt2[:, None, None] + kyt2[None, :, None]) + kzt2[None, None, :]) >= 1.0
+        self.dx, self.dy, self.dz = p.dx, p.dy, p.dz
+        self.x = np.arange(nx) * self.dx
+        self.y = np.arange(ny) * self.dy
+        self.z = np.arange(nz) * self.dz
+
+
+def fft(g: Grid, f: np.ndarray) -> np.ndarray:
+    """cof_p3dfft.f90:33-60: R2C of a real (nz,ny,nx) field times 1/N."""
+    out = sfft.rfftn(f, axes=(0, 1, 2), norm="forward", workers=_WORKERS)
+    return np.ascontiguousarray(out.transpose(2, 1, 0))
+
+
+def ifft(g: Grid, fk: np.ndarray) -> np.ndarray:
+    """cof_p3dfft.f90:65-83: unnormalised C2R (imag of the kx=0 and kx=nx/2
+    entries of each x line is ignored, as in FFTW's c2r)."""
+    a = fk.transpose(2, 1, 0)
+    return sfft.irfftn(a, s=(g.nz, g.ny, g.nx), axes=(0, 1, 2), norm="forward",
+                       workers=_WORKERS)
+
+
+def fft3(g: Grid, u: np.ndarray) -> np.ndarray:
+    """cof_p3dfft.f90:88-117."""
+    return np.stack([fft(g, u[i]) for i in range(3)])
+
+
+def ifft3(g: Grid, uk: np.ndarray) -> np.ndarray:
+    """cof_p3dfft.f90:122-145."""
+    return np.stack([ifft(g, uk[i]) for i in range(3)])
+
+
+# ---------------------------------------------------------------------------
+# Spectral operators
+# ---------------------------------------------------------------------------
+def cal_vis(g: Grid, dt: float, nu: float) -> np.ndarray:
+    """vis.f90:9-66: exp(-dt*nu*k^2), zero outside the 2/3 ellipsoid."""
+    kx2 = g.kx1[:, None, None] ** 2
+    ky2 = g.ky1[None, :, None] ** 2
+    kz2 = g.kz1[None, None, :] ** 2
+    vis = np.exp(-dt * nu * ((kx2 + ky2) + kz2))
+    if g.p.iDealias == 1:
+        vis = np.where(g.alias, 0.0, vis)
+    return vis
+
+
+def dealias(g: Grid, fk: np.ndarray) -> np.ndarray:
+    """basic_operators.f90:1032-1066 dealias3 (acts on every component)."""
+    out = fk.copy()
+    out[..., g.alias] = 0.0
+    return out
+
+
+def curl(g: Grid, uk: np.ndarray) -> np.ndarray:
+    """basic_operators.f90:185-215 curl3: i k x u."""
+    o = np.empty_like(uk)
+    o[0] = 1j * (g.KY * uk[2] - g.KZ * uk[1])
+    o[1] = 1j * (g.KZ * uk[0] - g.KX * uk[2])
+    o[2] = 1j * (g.KX * uk[1] - g.KY * uk[0])
+    return o
+
+
+def divergence(g: Grid, uk: np.ndarray) -> np.ndarray:
+    """basic_operators.f90:707-734: i k . u."""
+    return 1j * (g.KX * uk[0] + g.KY * uk[1] + g.KZ * uk[2])
+
+
+def add_grad_pressure(g: Grid, nlk: np.ndarray) -> np.ndarray:
+    """rhs.f90:575-615: N - k (k.N)/k^2 for k^2 != 0 (zero mode untouched)."""
+    k2 = g.K2
+    nz = k2 != 0.0
+    safe = np.where(nz, k2, 1.0)
+    qk = (g.KX * nlk[0] + g.KY * nlk[1] + g.KZ * nlk[2]) / safe
+    out = nlk.copy()
+    out[0] = np.where(nz, nlk[0] - g.KX * qk, nlk[0])
+    out[1] = np.where(nz, nlk[1] - g.KY * qk, nlk[1])
+    out[2] = np.where(nz, nlk[2] - g.KZ * qk, nlk[2])
+    return out
+
+
+def add_explicit_diffusion(g: Grid, uk: np.ndarray, nlk: np.ndarray, nu: float) -> np.ndarray:
+    """rhs.f90:441-468: N - nu k^2 u."""
+    return nlk - nu * g.K2[None] * uk
+
+
+def vorticity2velocity(g: Grid, vork: np.ndarray) -> np.ndarray:
+    """basic_operators.f90:1081-1126: i k x (w/k^2), zero where k^2 < 1e-13."""
+    k2 = g.K2
+    ok = k2 >= 1e-13
+    safe = np.where(ok, k2, 1.0)
+    sp = vork / safe[None]
+    uk = np.empty_like(vork)
+    uk[0] = 1j * (g.KY * sp[2] - g.KZ * sp[1])
+    uk[1] = 1j * (g.KZ * sp[0] - g.KX * sp[2])
+    uk[2] = 1j * (g.KX * sp[1] - g.KY * sp[0])
+    uk[:, ~ok] = 0.0
+    return uk
+
+
+def energy_k(g: Grid, uk: np.ndarray) -> float:
+    """integrals.f90:522-552 compute_energies_k (Parseval, half weight at
+    ix=0 and ix=nx/2), times xl*yl*zl."""
+    w = np.ones(g.nxh)
+    w[0] = 0.5
+    if g.nx // 2 < g.nxh:
+        w[g.nx // 2] = 0.5
+    s = (uk.real ** 2 + uk.imag ** 2).sum(axis=(0, 2, 3))
+    return float((w * s).sum()) * g.p.xl * g.p.yl * g.p.zl
+
+
+def compute_energies(g: Grid, u: np.ndarray) -> Tuple[float, float, float, float]:
+    """integrals.f90:434-480."""
+    dv = g.dx * g.dy * g.dz
+    ex = float((u[0] * u[0]).sum())
+    ey = float((u[1] * u[1]).sum())
+    ez = float((u[2] * u[2]).sum())
+    return 0.5 * dv * (ex + ey + ez), 0.5 * dv * ex, 0.5 * dv * ey, 0.5 * dv * ez
+
+
+def field_max_magnitude(u: np.ndarray) -> float:
+    """basic_operators.f90:847-872."""
+    return float(np.sqrt(np.max(u[0] * u[0] + u[1] * u[1] + u[2] * u[2])))
+
+
+# ---------------------------------------------------------------------------
+# Smooth steps (mask.f90:68-90 smoothstep; module_insects.f90:408-422 steps)
+# ---------------------------------------------------------------------------
+def steps(x, t, h):
+    x = np.asarray(x, dtype=np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    mid = 0.5 * (1.0 + np.cos((x - t + h) * PI / (2.0 * h)))
+    return np.where(x <= t - h, 1.0, np.where((t - h < x) & (x < t + h), mid, 0.0))
+
+
+smoothstep = steps
+
+
+def periodize(x, box, periodic: bool):
+    """vars.f90:468-484 periodize_coordinate (x is a (3,...) array)."""
+    if not periodic:
+        return x
+    out = [np.asarray(c, dtype=np.float64).copy() for c in x]
+    for i in range(3):
+        c = out[i]
+        c = np.where(x[i] < -box[i] / 2.0, x[i] + box[i], c)
+        c = np.where(x[i] > box[i] / 2.0, x[i] - box[i], c)
+        out[i] = c
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Rotation matrices (rotation_matrices.f90:4-37), passive convention
+# ---------------------------------------------------------------------------
+def Rx(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[1.0, 0.0, 0.0], [0.0, c, s], [0.0, -s, c]])
+
+
+def Ry(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, 0.0, -s], [0.0, 1.0, 0.0], [s, 0.0, c]])
+
+
+def Rz(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, s, 0.0], [-s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def deg2rad(d):
+    """vars.f90:255-260."""
+    return d * PI / 180.0
+
+
+def fseries_eval(time, a0, ai, bi):
+    """module_helpers.f90:30-55."""
+    f = 2.0 * PI
+    u = 0.5 * a0
+    u_dt = 0.0
+    for i in range(1, len(ai) + 1):
+        s = math.sin(f * float(i) * time)
+        c = math.cos(f * float(i) * time)
+        u = u + ai[i - 1] * c + bi[i - 1] * s
+        u_dt = u_dt + f * float(i) * (-ai[i - 1] * s + bi[i - 1] * c)
+    return u, u_dt
+
+
+def _f32(vals):
+    """Fortran default-real literals (single precision) promoted to double."""
+    return [float(np.float32(v)) for v in vals]
+
+
+class WingKinematics:
+    def __init__(self, protocol: str, infile: str = ""):
+        self.protocol = protocol
+        self.infile = infile
+        self.initialized = False
+        self.a0 = {"phi": 0.0, "alpha": 0.0, "theta": 0.0}
+        self.ai = {"phi": [], "alpha": [], "theta": []}
+        self.bi = {"phi": [], "alpha": [], "theta": []}
+        self.type = "fourier"
+        self.units = "degree"
+        self.convention = "flusi"
+
+    def _load_file(self, base_dir: str):
+        """wings_motion.f90:98-171 (from_file, Fourier coefficients)."""
+        path = self.infile
+        if not os.path.isabs(path):
+            path = os.path.join(base_dir, path)
+        if ".ini" not in self.infile:
+            raise FlusiAbort(2030, "old kinematics file format")
+        kf = IniFile(path)
+        self.type = kf.str("kinematics", "type", "none")
+        self.units = kf.str("kinematics", "units", "degree")
+        self.convention = kf.str("kinematics", "convention", "flusi")
+        if self.type not in ("Fourier", "fourier", "FOURIER"):
+            raise FlusiAbort(77771, "only type=fourier kinematics are supported")
+        for ang in ("phi", "alpha", "theta"):
+            n = kf.int("kinematics", "nfft_" + ang, 0)
+            self.a0[ang] = kf.dbl("kinematics", "a0_" + ang, 0.0)
+            self.ai[ang] = list(kf.vct("kinematics", "ai_" + ang, [0.0] * n)) if n else []
+            self.bi[ang] = list(kf.vct("kinematics", "bi_" + ang, [0.0] * n)) if n else []
+        self.initialized = True
+
+    def evaluate(self, time: float, base_dir: str):
+        """wings_motion.f90:69-305 FlappingMotion for the protocols in scope.
+        Returns phi, alpha, theta, phi_dt, alpha_dt, theta_dt in radians."""
+        pr = self.protocol
+        if pr == "from_file":
+            if not self.initialized:
+                self._load_file(base_dir)
+            phi, phi_dt = fseries_eval(time, self.a0["phi"], self.ai["phi"], self.bi["phi"])
+            alpha, alpha_dt = fseries_eval(time, self.a0["alpha"], self.ai["alpha"], self.bi["alpha"])
+            theta, theta_dt = fseries_eval(time, self.a0["theta"], self.ai["theta"], self.bi["theta"])
+            if self.units in ("degree", "DEGREE", "Degree"):
+                phi, alpha, theta = deg2rad(phi), deg2rad(alpha), deg2rad(theta)
+                phi_dt, alpha_dt, theta_dt = deg2rad(phi_dt), deg2rad(alpha_dt), deg2rad(theta_dt)
+            elif self.units not in ("radiant", "RADIANT", "Radiant"):
+                raise FlusiAbort(1718, "units must be degree or radiant")
+            if self.convention in ("SISC", "sisc", "siam-sisc"):
+                theta, theta_dt = -theta, -theta_dt
+            elif self.convention not in ("FLUSI", "flusi", "Flusi", "FluSI"):
+                raise FlusiAbort(1719, "convention must be flusi or sisc")
+            return phi, alpha, theta, phi_dt, alpha_dt, theta_dt
+        if pr == "Drosophila_hovering_fry":
+            a0p, a0a, a0t = _f32([25.4649398, -0.3056968, -17.8244658])
+            aip = _f32([71.1061858, 2.1685448, -0.1986978, 0.6095268, -0.0311298,
+                        -0.1255648, -0.0867778, 0.0543518, 0.0, 0.0])
+            bip = _f32([5.4547058, -3.5461688, 0.6260698, 0.1573728, -0.0360498, -0.0205348,
+                        -0.0083818, -0.0076848, 0.0, 0.0])
+            aia = _f32([3.3288788, 0.6303878, -10.9780518, 2.1123398, -3.2301198,
+                        -1.4473158, 0.6141758, -0.3071608, 0.1458498, 0.0848308])
+            bia = _f32([67.5430838, 0.6566888, 9.9226018, 3.9183988, -2.6882828, 0.6433518,
+                        -0.8792398, -0.4817838, 0.0300078, -0.1015118])
+            ait = _f32([-3.9750378, -8.2808998, 0.0611208, 0.3906598, -0.4488778, 0.120087,
+                        0.0717048, -0.0699578, 0.0, 0.0])
+            bit = _f32([-2.2839398, -3.5213068, 1.9296668, -1.0832488, -0.3011748, 0.1786648,
+                        -0.1228608, 0.0004808, 0.0, 0.0])
+            phi, phi_dt = fseries_eval(time, a0p, aip, bip)
+            alpha, alpha_dt = fseries_eval(time, a0a, aia, bia)
+            theta, theta_dt = fseries_eval(time, a0t, ait, bit)
+            return (deg2rad(phi), deg2rad(alpha), deg2rad(theta),
+                    deg2rad(phi_dt), deg2rad(alpha_dt), deg2rad(theta_dt))
+        if pr == "flapper_dickinson":
+            bi_alpha = [48.23094285611071, 0.0, 10.224154661301371, 0.0, 2.1623763046726396,
+                        0.0, 0.05049394424178093, 0.0, -0.17550942623071494,
+                        0.0, -0.06634193748204852, 0.0, -0.008925020495896451,
+                        0.0, 0.0011292567942149407, 0.0, 6.471071566666472e-4,
+                        0.0, 1.0018757795834964e-4, 0.0, 3.0105550216312524e-6,
+                        0.0, -1.237567150768195e-6, 0.0, -1.988004402010933e-7,
+                        0.0, -1.10165545174181e-8, 0.0, 2.4135650975460306e-10]
+            tadv = float(np.float32(0.08))
+            f = 2.0 * PI
+            alpha = 0.0
+            alpha_dt = 0.0
+            for i in range(1, 30):
+                s = math.sin(f * float(i) * (time + tadv))
+                c = math.cos(f * float(i) * (time + tadv))
+                alpha = alpha + bi_alpha[i - 1] * s
+                alpha_dt = alpha_dt + f * float(i) * bi_alpha[i - 1] * c
+            alphacdeg = 90.0 - 40.0
+            alpha = alphacdeg / 40.0 * alpha
+            alpha_dt = alphacdeg / 40.0 * alpha_dt
+            alpha, alpha_dt = deg2rad(alpha), deg2rad(alpha_dt)
+            ai_phi = [63.24528806534019, 0.0, 5.753991800610726, 0.0, 1.3887974015525626,
+                      0.0, 0.3889856512386744, 0.0, 0.10577402496901325,
+                      0.0, 0.026061339604144987, 0.0, 0.005623376646981709,
+                      0.0, 0.001042285996467963, 0.0, 1.639611509380189e-4,
+                      0.0, 2.1716252827442023e-5, 0.0, 2.408190194815521e-6,
+                      0.0, 2.2268710288534648e-7, 0.0, 1.7118916093759426e-8,
+                      0.0, 1.0914870312823793e-9, 0.0, 5.76135101855556e-11,
+                      0.0, 2.513944479978149e-12]
+            phi = 0.0
+            phi_dt = 0.0
+            for i in range(1, 32):
+                s = math.sin(f * float(i) * time)
+                c = math.cos(f * float(i) * time)
+                phi = phi + ai_phi[i - 1] * c
+                phi_dt = phi_dt + f * float(i) * (-ai_phi[i - 1] * s)
+            phicdeg = 180.0
+            phi = phicdeg / 180.0 * phi
+            phi_dt = phicdeg / 180.0 * phi_dt
+            return deg2rad(phi), alpha, 0.0, deg2rad(phi_dt), alpha_dt, 0.0
+        if pr == "none":
+            return 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
+        raise FlusiAbort(10624, f"flapping motion protocol {pr} not in scope")
+
+
+class Insect:
+    """Tethered insect state (module_insects.f90:89-227 type diptera) with
+    insect_init (insect_init_clean.f90:1-250)."""
+
+    def __init__(self, ini: IniFile, p: Params, base_dir: str):
+        self.base_dir = base_dir
+        self.p = p
+        dxr = p.dx
+        self.WingShape = ini.str("Insects", "WingShape", "none")
+        self.b_top = ini.dbl("Insects", "b_top", 0.0)
+        self.b_bot = ini.dbl("Insects", "b_bot", 0.0)
+        self.L_span = ini.dbl("Insects", "L_span", 0.0)
+        fr = ini.str("Insects", "FlappingMotion_right", "none")
+        fl = ini.str("Insects", "FlappingMotion_left", "none")
+        infile = ini.str("Insects", "infile", "none.in")
+        self.kine_r = self._kine(fr, infile)
+        self.kine_l = self._kine(fl, infile)
+        self.BodyType = ini.str("Insects", "BodyType", "ellipsoid")
+        self.BodyMotion = ini.str("Insects", "BodyMotion", "yes")
+        self.LeftWing = ini.str("Insects", "LeftWing", "yes")
+        self.RightWing = ini.str("Insects", "RightWing", "yes")
+        self.b_body = ini.dbl("Insects", "b_body", 0.1)
+        self.L_body = ini.dbl("Insects", "L_body", 1.0)
+        self.R_head = ini.dbl("Insects", "R_head", 0.1)
+        self.R_eye = ini.dbl("Insects", "R_eye", 1.0)
+        self.WingThickness = ini.dbl("Insects", "WingThickness", 4.0 * dxr)
+        self.x0 = ini.vct("Insects", "x0", [0.5 * p.xl, 0.5 * p.yl, 0.5 * p.zl])
+        self.yawpitchroll_0 = ini.vct("Insects", "yawpitchroll_0", [0.0, 0.0, 0.0]) * (PI / 180.0)
+        self.eta0 = ini.dbl("Insects", "eta0", 0.0) * (PI / 180.0)
+        self.Jxx = ini.dbl("Insects", "Jxx", 0.0)
+        self.Jyy = ini.dbl("Insects", "Jyy", 0.0)
+        self.Jzz = ini.dbl("Insects", "Jzz", 0.0)
+        self.Jxy = ini.dbl("Insects", "Jxy", 0.0)
+        self.smooth = 1.0 * dxr
+        self.safety = 3.5 * self.smooth
+        self.x_head = ini.vct("Insects", "x_head", [0.5 * self.L_body, 0.0, 0.0])
+        d = self.x_head + math.sin(45.0 * PI / 180.0) * self.R_head * 0.8 * np.array([1.0, 1.0, 1.0])
+        self.x_eye_r = ini.vct("Insects", "x_eye_r", d)
+        d = self.x_head + math.sin(45.0 * PI / 180.0) * self.R_head * 0.8 * np.array([1.0, -1.0, 1.0])
+        self.x_eye_l = ini.vct("Insects", "x_eye_l", d)
+        self.x_pivot_l_b = ini.vct("Insects", "x_pivot_l", [0.0, +self.b_body, 0.0])
+        self.x_pivot_r_b = ini.vct("Insects", "x_pivot_r", [0.0, -self.b_body, 0.0])
+        self.color_body, self.color_l, self.color_r = 1, 2, 3
+        self.body_moves = "yes"
+        self.body_already_drawn = False
+        self.cleaned_already_once = False
+        self.time = 0.0
+        if self.BodyMotion not in ("tethered",):
+            raise FlusiAbort(10625, f"BodyMotion {self.BodyMotion} not in scope")
+        self.update(0.0)
+
+    @staticmethod
+    def _kine(prot: str, infile: str) -> WingKinematics:
+        if "from_file::" in prot:
+            return WingKinematics("from_file", prot[prot.index("from_file::") + 11:])
+        if prot == "from_file":
+            return WingKinematics("from_file", infile)
+        return WingKinematics(prot)
+
+    # module_insects.f90:281-333 Update_Insect
+    def _motion_state(self, time):
+        st = {}
+        # body_motion.f90:147-160 (tethered)
+        st["psi"] = self.yawpitchroll_0[2]
+        st["beta"] = self.yawpitchroll_0[1]
+        st["gamma"] = self.yawpitchroll_0[0]
+        st["psi_dt"] = st["beta_dt"] = st["gamma_dt"] = 0.0
+        st["xc"] = self.x0.copy()
+        st["vc"] = np.zeros(3)
+        (st["phi_r"], st["alpha_r"], st["theta_r"], st["phi_dt_r"], st["alpha_dt_r"],
+         st["theta_dt_r"]) = self.kine_r.evaluate(time, self.base_dir)
+        (st["phi_l"], st["alpha_l"], st["theta_l"], st["phi_dt_l"], st["alpha_dt_l"],
+         st["theta_dt_l"]) = self.kine_l.evaluate(time, self.base_dir)
+        st["eta"] = self.eta0  # stroke_plane.f90:7-37
+        st["M_body"] = Rx(st["psi"]) @ (Ry(st["beta"]) @ Rz(st["gamma"]))
+        self._wing_ang_vel(st)
+        return st
+
+    @staticmethod
+    def _wing_ang_vel(st):
+        """module_insects.f90:650-750 wing_angular_velocities (+ body, :604-636)."""
+        psi, beta, gamma = st["psi"], st["beta"], st["gamma"]
+        psi_dt, beta_dt, gamma_dt = st["psi_dt"], st["beta_dt"], st["gamma_dt"]
+        st["rot_body_g"] = np.array([psi_dt * math.cos(beta) * math.cos(gamma) - beta_dt * math.sin(gamma),
+                                     beta_dt * math.cos(gamma) + psi_dt * math.cos(beta) * math.sin(gamma),
+                                     gamma_dt - psi_dt * math.sin(beta)])
+        st["rot_body_b"] = np.array([psi_dt - gamma_dt * math.sin(beta),
+                                     beta_dt * math.cos(psi) + gamma_dt * math.cos(beta) * math.sin(psi),
+                                     gamma_dt * math.cos(beta) * math.cos(psi) - beta_dt * math.sin(psi)])
+        eta = st["eta"]
+        Ms_l = Ry(eta)
+        Ms_r = Rx(PI) @ Ry(eta)
+        M1l, M2l, M3l = Ry(st["alpha_l"]), Rz(st["theta_l"]), Rx(st["phi_l"])
+        M1r, M2r, M3r = Ry(-st["alpha_r"]), Rz(st["theta_r"]), Rx(-st["phi_r"])
+        Mwl = M1l @ (M2l @ (M3l @ Ms_l))
+        Mwr = M1r @ (M2r @ (M3r @ Ms_r))
+        st["M_wing_l"], st["M_wing_r"] = Mwl, Mwr
+        ra_l = np.array([0.0, st["alpha_dt_l"], 0.0])
+        rt_l = np.array([0.0, 0.0, st["theta_dt_l"]])
+        rp_l = np.array([st["phi_dt_l"], 0.0, 0.0])
+        ra_r = np.array([0.0, -st["alpha_dt_r"], 0.0])
+        rt_r = np.array([0.0, 0.0, st["theta_dt_r"]])
+        rp_r = np.array([-st["phi_dt_r"], 0.0, 0.0])
+        st["rot_rel_l_w"] = Mwl @ (Ms_l.T @ (M3l.T @ (rp_l + M2l.T @ (rt_l + M1l.T @ ra_l))))
+        st["rot_rel_r_w"] = Mwr @ (Ms_r.T @ (M3r.T @ (rp_r + M2r.T @ (rt_r + M1r.T @ ra_r))))
+        Mb = st["M_body"]
+        st["rot_rel_l_b"] = Mwl.T @ st["rot_rel_l_w"]
+        st["rot_rel_r_b"] = Mwr.T @ st["rot_rel_r_w"]
+        st["rot_rel_l_g"] = Mb.T @ st["rot_rel_l_b"]
+        st["rot_rel_r_g"] = Mb.T @ st["rot_rel_r_b"]
+
+    def update(self, time: float):
+        st = self._motion_state(time)
+        # module_insects.f90:752-799 wing_angular_accel (forward difference, dt=1e-8)
+        dt = 1.0e-8
+        st2 = self._motion_state(time + dt)
+        Mb = st["M_body"]
+        st["rot_dt_l_w"] = st["M_wing_l"] @ (Mb @ ((st2["rot_rel_l_g"] - st["rot_rel_l_g"]) / dt))
+        st["rot_dt_r_w"] = st["M_wing_r"] @ (Mb @ ((st2["rot_rel_r_g"] - st["rot_rel_r_g"]) / dt))
+        st["M_body_inv"] = Mb.T
+        st["x_pivot_l_g"] = Mb.T @ self.x_pivot_l_b
+        st["x_pivot_r_g"] = Mb.T @ self.x_pivot_r_b
+        self.st = st
+        self.time = time
+        self.body_moves = "no"  # tethered (body_motion.f90:158)
+
+    def kinematics_row(self, time):
+        """module_insects.f90:1139-1154 write_kinematics_file columns."""
+        s = self.st
+        return [time, *s["xc"], s["psi"], s["beta"], s["gamma"], s["eta"],
+                s["alpha_l"], s["phi_l"], s["theta_l"], s["alpha_r"], s["phi_r"], s["theta_r"],
+                *s["rot_rel_l_w"], *s["rot_rel_r_w"], *s["rot_dt_l_w"], *s["rot_dt_r_w"]]
+
+    # ---- drawing -------------------------------------------------------
+    def draw(self, g: Grid, mask, color, us):
+        """module_insects.f90:338-399 Draw_Insect, in place on (mask, color, us)."""
+        p = self.p
+        ddx = (g.dx, g.dy, g.dz)
+        self.smooth = 1.0 * max(ddx)
+        self.safety = 3.5 * self.smooth
+        # delete_old_mask (module_insects.f90:844-883)
+        if self.body_moves == "no" and self.cleaned_already_once:
+            w = (color == self.color_l) | (color == self.color_r)
+            mask[w] = 0.0
+            color[w] = 0
+            us[...] = 0.0
+        else:
+            mask[...] = 0.0
+            color[...] = 0
+            us[...] = 0.0
+        self.cleaned_already_once = True
+        if self.body_moves == "no":
+            if not self.body_already_drawn:
+                self._draw_body(g, mask, color, us)
+                self.body_already_drawn = True
+        else:
+            self._draw_body(g, mask, color, us)
+        self._draw_wings(g, mask, color, us)
+
+    def _xglob(self, g: Grid):
+        s = self.st
+        X = (g.x[None, None, :] - s["xc"][0]) + np.zeros((g.nz, g.ny, 1))
+        Y = (g.y[None, :, None] - s["xc"][1]) + np.zeros((g.nz, 1, g.nx))
+        Z = (g.z[:, None, None] - s["xc"][2]) + np.zeros((1, g.ny, g.nx))
+        return periodize([X, Y, Z], (self.p.xl, self.p.yl, self.p.zl), self.p.periodic)
+
+    @staticmethod
+    def _matvec(M, v):
+        return [M[i, 0] * v[0] + M[i, 1] * v[1] + M[i, 2] * v[2] for i in range(3)]
+
+    def _draw_body(self, g, mask, color, us):
+        """body_geometry.f90:5-122 draw_insect_body; :511-595 draw_body_jerry.
+        'ellipsoid_plain' (build extension) is Jerry's ellipsoid with b_body and
+        L_body from the ini file and no head or eyes."""
+        bt = self.BodyType
+        if bt == "nobody":
+            return
+        if bt in ("jerry", "Jerry"):
+            self.R_head = 0.125
+            self.R_eye = 0.0625
+            self.x_pivot_r_b = np.array([0.05, -0.2165, 0.0])
+            self.x_pivot_l_b = np.array([0.05, +0.2165, 0.0])
+            self.b_body = 0.1
+            self.L_body = 1.0
+            self.x_head = np.array([0.5 * self.L_body, 0.0, 0.0])
+            self.x_eye_r = self.x_head + math.sin(45.0 * PI / 180.0) * self.R_head * 0.8 * np.array([1.0, 1.0, 1.0])
+            self.x_eye_l = self.x_head + math.sin(45.0 * PI / 180.0) * self.R_head * 0.8 * np.array([1.0, -1.0, 1.0])
+        elif bt != "ellipsoid_plain":
+            raise FlusiAbort(10623, "Insect%BodyType unknown..." + bt)
+        Mb = self.st["M_body"]
+        xb = self._matvec(Mb, self._xglob(g))
+        a_body = self.L_body / 2.0
+        b = self.b_body
+        sel = ((np.abs(xb[1]) <= b + self.safety) & (np.abs(xb[2]) <= b + self.safety)
+               & (np.abs(xb[0]) < self.L_body / 2 + self.safety) & ((xb[0] / a_body) ** 2 <= 1.0))
+        R = np.sqrt(xb[1] ** 2 + xb[2] ** 2)
+        with np.errstate(invalid="ignore"):
+            R0 = np.sqrt(b ** 2 * (1.0 - (xb[0] / a_body) ** 2))
+        sel &= R < R0 + self.safety
+        mask[sel] = np.maximum(steps(R[sel], R0[sel], self.smooth), mask[sel])
+        color[sel] = self.color_body
+        if bt in ("jerry", "Jerry"):
+            xc = self.st["xc"]
+            for xp, rr in ((self.x_head, self.R_head), (self.x_eye_l, self.R_eye), (self.x_eye_r, self.R_eye)):
+                self._drawsphere(g, xc + Mb.T @ xp, rr, mask, color)
+        # stage II is skipped for a resting body (body_moves == "no")
+
+    def _drawsphere(self, g, xc, R0, mask, color):
+        """body_geometry.f90:1657-1724 drawsphere."""
+        p = self.p
+        xc = np.array(xc, dtype=np.float64)
+        box = (p.xl, p.yl, p.zl)
+        for i in range(3):
+            if xc[i] < 0.0:
+                xc[i] += box[i]
+        for i in range(3):
+            if xc[i] >= box[i]:
+                xc[i] -= box[i]
+        ddx = (g.dx, g.dy, g.dz)
+        nsafe = _nint((R0 + self.safety) / min(ddx))
+        lo, hi = [], []
+        for i, n in enumerate((g.nx, g.ny, g.nz)):
+            c = _nint(xc[i] / ddx[i])
+            lo.append(max(c - nsafe, 0))
+            hi.append(min(c + nsafe, n - 1))
+        if any(l > h for l, h in zip(lo, hi)):
+            return
+        ix = np.arange(lo[0], hi[0] + 1)
+        iy = np.arange(lo[1], hi[1] + 1)
+        iz = np.arange(lo[2], hi[2] + 1)
+        X = (ix * g.dx - xc[0])[None, None, :] + np.zeros((len(iz), len(iy), 1))
+        Y = (iy * g.dy - xc[1])[None, :, None] + np.zeros((len(iz), 1, len(ix)))
+        Z = (iz * g.dz - xc[2])[:, None, None] + np.zeros((1, len(iy), len(ix)))
+        X, Y, Z = periodize([X, Y, Z], box, p.periodic)
+        R = np.sqrt(X * X + Y * Y + Z * Z)
+        sub = (slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
+        m = mask[sub]
+        c = color[sub]
+        tmp = steps(R, R0, self.smooth)
+        sel = (R <= R0 + self.safety) & (tmp >= m)
+        m[sel] = tmp[sel]
+        c[sel] = self.color_body
+
+    def _draw_wings(self, g, mask, color, us):
+        """wings_geometry.f90:5-110 draw_insect_wings (stage I right then left
+        wing, stage II transform of us to the global frame)."""
+        s = self.st
+        if self.RightWing == "yes":
+            self._draw_wing(g, mask, color, us, self.color_r, s["M_wing_r"], self.x_pivot_r_b, s["rot_rel_r_w"])
+        if self.LeftWing == "yes":
+            self._draw_wing(g, mask, color, us, self.color_l, s["M_wing_l"], self.x_pivot_l_b, s["rot_rel_l_w"])
+        sel = ((color == self.color_l) | (color == self.color_r)) & (mask > 0.0)
+        if not sel.any():
+            return
+        Mb = s["M_body"]
+        xg = self._xglob(g)
+        xb = [c[sel] for c in self._matvec(Mb, xg)]
+        vb = Mb @ s["vc"]
+        rb = s["rot_body_b"]
+        v = [vb[0] + rb[1] * xb[2] - rb[2] * xb[1],
+             vb[1] + rb[2] * xb[0] - rb[0] * xb[2],
+             vb[2] + rb[0] * xb[1] - rb[1] * xb[0]]
+        w = [us[i][sel] + v[i] for i in range(3)]
+        Mi = s["M_body_inv"]
+        for i in range(3):
+            us[i][sel] = Mi[i, 0] * w[0] + Mi[i, 1] * w[1] + Mi[i, 2] * w[2]
+
+    def _draw_wing(self, g, mask, color, us, cw, Mw, xpiv, rot):
+        """wings_geometry.f90:621-712 (TwoEllipses) and :527-610 (rectangular)."""
+        s = self.st
+        xb = self._matvec(s["M_body"], self._xglob(g))
+        xw = self._matvec(Mw, [xb[0] - xpiv[0], xb[1] - xpiv[1], xb[2] - xpiv[2]])
+        sel = (xw[1] >= -self.safety) & (xw[1] <= self.L_span + self.safety)
+        sel &= np.abs(xw[2]) <= 0.5 * self.WingThickness + self.safety
+        if self.WingShape == "TwoEllipses":
+            a = 0.5 * self.L_span
+            q = 1.0 - ((xw[1] - a) ** 2) / (a ** 2)
+            pos = q >= 0.0
+            qq = np.where(pos, q, 0.0)
+            x_top = np.where(pos, np.sqrt((self.b_top ** 2) * qq), 0.0)
+            x_bot = np.where(pos, -np.sqrt((self.b_bot ** 2) * qq), 0.0)
+        elif self.WingShape == "rectangular":
+            x_top = np.full_like(xw[0], self.b_top)
+            x_bot = np.full_like(xw[0], -self.b_bot)
+        else:
+            raise FlusiAbort(10626, f"WingShape {self.WingShape} not in scope")
+        sel &= (xw[0] > x_bot - self.safety) & (xw[0] < x_top + self.safety)
+        y_tmp = np.where(xw[1] < 0.0, steps(-xw[1], 0.0, self.smooth), steps(xw[1], self.L_span, self.smooth))
+        z_tmp = steps(np.abs(xw[2]), 0.5 * self.WingThickness, self.smooth)
+        x_tmp = np.where(xw[0] < 0.0, steps(-xw[0], -x_bot, self.smooth), steps(xw[0], x_top, self.smooth))
+        mt = z_tmp * y_tmp * x_tmp
+        sel &= (mask < mt) & (mt > 0.0)
+        mask[sel] = mt[sel]
+        color[sel] = cw
+        v = [rot[1] * xw[2] - rot[2] * xw[1], rot[2] * xw[0] - rot[0] * xw[2], rot[0] * xw[1] - rot[1] * xw[0]]
+        MT = Mw.T
+        for i in range(3):
+            us[i][sel] = (MT[i, 0] * v[0] + MT[i, 1] * v[1] + MT[i, 2] * v[2])[sel]
+
+    def aero_inert_power(self, torque_part):
+        """module_insects.f90:469-586 aero_power / inert_power."""
+        s = self.st
+        apow = {1: 0.0}
+        apow[2] = -float(np.dot(torque_part[2], s["rot_rel_l_g"]))
+        apow[3] = -float(np.dot(torque_part[3], s["rot_rel_r_g"]))
+        ipow = {1: 0.0}
+        for c, rw, rdt in ((2, s["rot_rel_l_w"], s["rot_dt_l_w"]), (3, s["rot_rel_r_w"], s["rot_dt_r_w"])):
+            a = np.array([self.Jxx * rdt[0] + self.Jxy * rdt[1], self.Jxy * rdt[0] + self.Jyy * rdt[1], self.Jzz * rdt[2]])
+            b = np.array([self.Jxx * rw[0] + self.Jxy * rw[1], self.Jxy * rw[0] + self.Jyy * rw[1], self.Jzz * rw[2]])
+            ipow[c] = (rw[0] * (a[0] + rw[1] * b[2] - rw[2] * b[1]) + rw[1] * (a[1] + rw[2] * b[0] - rw[0] * b[2])
+                       + rw[2] * (a[2] + rw[0] * b[1] - rw[1] * b[0]))
+        return apow, ipow, apow[1] + apow[2] + apow[3], ipow[3] + ipow[2]
+
+
+def _nint(x: float) -> int:
+    """Fortran NINT (round half away from zero)."""
+    return int(math.floor(x + 0.5)) if x >= 0 else -int(math.floor(-x + 0.5))
+
+
+# ---------------------------------------------------------------------------
+# Simulation state + RHS + steppers + diagnostics
+# ---------------------------------------------------------------------------
+class Simulation:
+    """FLUSI's single-rank fluid path: init_fields_fsi, cal_nlk, FluidTimestep,
+    adjust_dt, write_integrals_fsi and the time_step loop."""
+
+    def __init__(self, ini_path: str, workers: int = 1, write_dir: Optional[str] = None):
+        set_workers(workers)
+        self.p, self.ini = get_params(ini_path)
+        self.g = Grid(self.p)
+        g = self.g
+        self.base_dir = os.path.dirname(os.path.abspath(ini_path))
+        self.mask = np.zeros((g.nz, g.ny, g.nx))
+        self.color = np.zeros((g.nz, g.ny, g.nx), dtype=np.int16)
+        self.us = np.zeros((3, g.nz, g.ny, g.nx))
+        self.insect: Optional[Insect] = None
+        self.series: Dict[str, List[List[float]]] = {}
+        self.keys: Dict[str, List[float]] = {}
+        self.write_dir = write_dir
+        self.nrhs = 5 if self.p.iTimeMethodFluid == "RK4" else 2
+        self.last_integrals = {"penalization_power": [0.0, 0.0, 0.0, 0.0]}
+
+    # -- series helpers ---------------------------------------------------
+    def _emit(self, name: str, row):
+        self.series.setdefault(name, []).append([float(v) for v in row])
+
+    # -- masks ---------------------------------------------------------------
+    def create_mask(self, time: float):
+        """mask.f90:2-31 -> geometry/hyd/mask_fsi.f90:2-158."""
+        p, g = self.p, self.g
+        if p.iMask == "Insect" and (p.iMoving != 1 or p.iPenalization != 1):
+            raise FlusiAbort(4453, "iMoving or iPenalization are wrong")
+        if p.iPenalization == 1:
+            if p.iMask in ("sphere", "Sphere"):
+                self.draw_sphere()
+            elif p.iMask in ("Insect", "insect"):
+                self.insect.update(time)
+                self.insect.draw(g, self.mask, self.color, self.us)
+            elif p.iMask in ("none", "empty", "no"):
+                pass
+            else:
+                raise FlusiAbort(3333, "unknown mask function iMask")
+        if p.iCavity != "no" and p.iPenalization == 1:
+            self.add_cavity()
+
+    def draw_sphere(self):
+        """geometry/hyd/draw_sphere.f90:2-43."""
+        p, g = self.p, self.g
+        self.mask[...] = 0.0
+        self.color[...] = 0
+        self.us[...] = 0.0
+        nsm = 1.5
+        smoothing = nsm * max(g.dx, g.dy, g.dz)
+        safety = 2.0 * nsm * max(g.dx, g.dy, g.dz)
+        X = g.x[None, None, :]
+        Y = g.y[None, :, None]
+        Z = g.z[:, None, None]
+        R = np.sqrt((X - p.x0) ** 2 + (Y - p.y0) ** 2 + (Z - p.z0) ** 2)
+        sel = R <= 0.5 * p.length + safety
+        tmp = smoothstep(R, 0.5 * p.length, smoothing)
+        self.mask[sel] = tmp[sel]
+        self.color[sel & (tmp > float(np.float32(1.0e-12)))] = 1
+
+    def add_cavity(self):
+        """geometry/add_cavity.f90:8-111 for iCavity = yes | freestream."""
+        p, g = self.p, self.g
+        if p.iCavity == "yes":
+            u = (0.0, 0.0, 0.0)
+        elif p.iCavity == "freestream":
+            u = tuple(p.umean)
+        else:
+            raise FlusiAbort(3334, f"iCavity={p.iCavity} not in scope")
+        n = p.cavity_size
+        ix = np.arange(g.nx)[None, None, :]
+        iy = np.arange(g.ny)[None, :, None]
+        iz = np.arange(g.nz)[:, None, None]
+        sel = ((ix <= n - 1) | (ix >= g.nx - n) | (iy <= n - 1) | (iy >= g.ny - n)
+               | (iz <= n - 1) | (iz >= g.nz - n))
+        sel = np.broadcast_to(sel, self.mask.shape)
+        self.mask[sel] = 1.0
+        for i in range(3):
+            self.us[i][sel] = u[i]
+        self.color[sel] = 0
+
+    # -- sponge ----------------------------------------------------------------
+    def penalize_vort(self, vort_c: np.ndarray) -> np.ndarray:
+        """sponge.f90:94-329 penalize_vort (cavity, outlet_x)."""
+        p, g = self.p, self.g
+        th = p.sponge_thickness
+        eps_inv = 1.0 / p.eps_sponge
+        out = np.zeros_like(vort_c)
+        ix = np.arange(g.nx)[None, None, :]
+        iy = np.arange(g.ny)[None, :, None]
+        iz = np.arange(g.nz)[:, None, None]
+        if p.iSpongeType == "cavity":
+            fl = self.mask < 1.0e-12
+            sel = np.zeros(self.mask.shape, dtype=bool)
+            if g.nx > 4:
+                sel |= np.broadcast_to((ix <= th - 1) | (ix >= g.nx - 1 - th + 1), sel.shape)
+            sel |= np.broadcast_to((iy <= th - 1) | (iy >= g.ny - 1 - th + 1), sel.shape)
+            sel |= np.broadcast_to((iz <= th - 1) | (iz >= g.nz - 1 - th + 1), sel.shape)
+            sel &= fl
+        elif p.iSpongeType == "outlet_x":
+            fl = self.mask < float(np.float32(1e-12))
+            sel = fl & np.broadcast_to(ix >= g.nx - 1 - th + 1, fl.shape)
+        else:
+            raise FlusiAbort(3335, f"iSpongeType={p.iSpongeType} not in scope")
+        out[sel] = -vort_c[sel] * eps_inv
+        return out
+
+    def vorticity_sponge(self, vort: np.ndarray) -> np.ndarray:
+        """sponge.f90:22-83."""
+        g = self.g
+        wk = np.stack([fft(g, self.penalize_vort(vort[i])) for i in range(3)])
+        k2 = g.K2
+        nz = k2 != 0.0
+        safe = np.where(nz, k2, 1.0)
+        sp = wk / safe[None]
+        o = np.empty_like(wk)
+        o[0] = 1j * (g.KY * sp[2] - g.KZ * sp[1])
+        o[1] = 1j * (g.KZ * sp[0] - g.KX * sp[2])
+        o[2] = 1j * (g.KX * sp[1] - g.KY * sp[0])
+        o[:, ~nz] = 0.0
+        return o
+
+    # -- RHS -----------------------------------------------------------------
+    def cal_nlk(self, time: float, uk: np.ndarray):
+        """rhs.f90:2-115 cal_nlk and :147-330 cal_nlk_fsi (+ add_grad_pressure).
+        Returns (nlk, u) with u the physical velocity."""
+        p, g = self.p, self.g
+        if p.iMoving == 1 and p.iPenalization == 1:
+            self.create_mask(time)
+        if p.eps < 1.0e-11:
+            raise FlusiAbort(77363800, "Value of eps is very small, maybe even zero.")
+        eps_inv = 1.0 / p.eps
+        u = ifft3(g, uk)
+        vort = ifft3(g, curl(g, uk))
+        workc = self.vorticity_sponge(vort) if p.iVorticitySponge == "yes" else None
+        chi = self.mask * eps_inv
+        du = u - self.us
+        pen = chi[None] * du
+        f = pen.reshape(3, -1).sum(axis=1)
+        nl = np.empty_like(u)
+        nl[0] = u[1] * vort[2] - u[2] * vort[1] - pen[0]
+        nl[1] = u[2] * vort[0] - u[0] * vort[2] - pen[1]
+        nl[2] = u[0] * vort[1] - u[1] * vort[0] - pen[2]
+        nlk = fft3(g, nl)
+        if workc is not None:
+            nlk = nlk + workc
+        if "dynamic" in p.iMeanFlow:
+            f1 = f * (g.dx * g.dy * g.dz)
+            if p.iMeanFlowStartupConditioner == "yes":
+                f1 = f1 * startup_conditioner(time, p.T_release_meanflow, p.tau_meanflow)
+            for i in range(3):
+                if p.iMeanFlow[i] == "dynamic":
+                    nlk[i, 0, 0, 0] = -f1[i] / p.m_fluid
+        if "accelerate_to_unity" in p.iMeanFlow:
+            for i in range(3):
+                if p.iMeanFlow[i] == "accelerate_to_unity":
+                    nlk[i, 0, 0, 0] += max(0.0, 1.0 - uk[i, 0, 0, 0].real)
+        if p.iTimeMethodFluid == "RK4":
+            nlk = add_explicit_diffusion(g, uk, nlk, p.nu)
+        nlk = add_grad_pressure(g, nlk)
+        return nlk, u
+
+    # -- time step control ----------------------------------------------------
+    def adjust_dt(self, time: float, u: np.ndarray) -> float:
+        """fluid_time_step.f90:917-1042 adjust_dt (+ truncate :1051-1060)."""
+        p, g = self.p, self.g
+        if p.dt_fixed > 0.0:
+            return p.dt_fixed
+        umax = field_max_magnitude(u)
+        if math.isnan(umax):
+            raise FlusiAbort(100011, "Evolved field contains a NAN: aborting run.")
+        if umax >= 1.0e3:
+            raise FlusiAbort(100012, "Umax is very big, surely this is an error")
+        h = min(g.dx, g.dy, g.dz)
+        dt1 = h * p.cfl / umax if umax >= 1.0e-8 else 1.0e-3
+        dt1 = truncate(dt1)
+        if p.dt_max > 0.0:
+            dt1 = min(dt1, p.dt_max)
+        if p.iPenalization > 0:
+            dt1 = min(p.cfl_eta * p.eps, dt1)
+        if p.iTimeMethodFluid == "RK4":
+            dt1 = min(dt1, 0.5 * h ** 2 / p.nu)
+        if p.intelligent_dt == "yes":
+            if time >= p.tsave_first:
+                dt1 = min(dt1, 0.98 * p.tsave)
+                t = float(math.ceil(time / p.tsave)) * p.tsave
+                if time + dt1 > t and abs(time - t) >= 1.0e-6:
+                    dt1 = t - time
+            dt1 = min(dt1, 0.98 * p.tintegral)
+            t = float(math.ceil(time / p.tintegral)) * p.tintegral
+            if time + dt1 > t and abs(time - t) >= 1.0e-6:
+                dt1 = t - time
+            t = p.tmax
+            if time + dt1 > t and abs(time - t) >= 1.0e-6:
+                dt1 = p.tmax - time
+        self._emit("dt.t", [time, dt1, dt1 * umax / h, dt1 * p.nu / h ** 2, dt1 / p.eps])
+        return dt1
+
+    # -- initial conditions ---------------------------------------------------
+    def init_fields(self):
+        """inicond/init_fields.f90:2-114 -> hyd/init_fields_fsi.f90:1-48,
+        381-435 (VortexRing), 716-737 (MeanFlow, quiescent)."""
+        p, g = self.p, self.g
+        self.time = 0.0
+        self.dt1 = p.tsave
+        self.dt0 = p.tsave
+        self.it = 0
+        self.uk = np.zeros((3, g.nxh, g.ny, g.nz), dtype=np.complex128)
+        self.nlk = np.zeros((self.nrhs, 3, g.nxh, g.ny, g.nz), dtype=np.complex128)
+        self.vis = np.zeros((g.nxh, g.ny, g.nz))
+        self.vis_dt = None  # the dt the stored integrating factor belongs to
+        if p.inicond == "VortexRing":
+            r00 = p.length
+            a = p.core_a if p.core_a > 0.0 else 0.82 * r00
+            gamma0 = p.omega1
+            X = g.x[None, None, :]
+            Y = g.y[None, :, None]
+            Z = g.z[:, None, None]
+            r = np.sqrt((Y - p.y0) ** 2 + (Z - p.z0) ** 2) + np.zeros((1, 1, g.nx))
+            om = (gamma0 / (PI * a ** 2)) * np.exp(-((X - p.x0) ** 2 + (r - r00) ** 2) / (a ** 2))
+            vort = np.zeros((3, g.nz, g.ny, g.nx))
+            ok = np.abs(r) > 1.0e-12
+            rs = np.where(ok, r, 1.0)
+            vort[1] = np.where(ok, -om * ((Z - p.z0) / rs), 0.0)
+            vort[2] = np.where(ok, om * ((Y - p.y0) / rs), 0.0)
+            self.uk = vorticity2velocity(g, fft3(g, vort))
+            self.set_mean_flow(self.time)
+        elif p.inicond == "MeanFlow":
+            self.uk[:, 0, 0, 0] = p.umean
+        elif p.inicond == "quiescent":
+            pass
+        else:
+            raise FlusiAbort(1, f"inicond {p.inicond} not in scope")
+        self.n1 = 1
+        self.n0 = 1 - self.n1
+        if p.iMask == "Insect" and p.iPenalization == 1:
+            self.insect = Insect(self.ini, p, self.base_dir)
+        if p.iPenalization == 1:
+            self.create_mask(self.time)
+
+    def set_mean_flow(self, time):
+        """fluid_time_step.f90:1066-1090."""
+        p = self.p
+        for i in range(3):
+            if p.iMeanFlow[i] == "fixed":
+                self.uk[i, 0, 0, 0] = p.umean[i]
+            elif p.iMeanFlow[i] == "sinusoidal":
+                self.uk[i, 0, 0, 0] = p.umean[i] * math.sin(2.0 * PI * p.umean_freq * time)
+
+    # -- steppers ---------------------------------------------------------------
+    def _vis_for(self, dt):
+        """cal_vis is called iff dt1 != dt0 (the stored factor stays otherwise)."""
+        if dt != self.dt0:
+            self.vis = cal_vis(self.g, dt, self.p.nu)
+            self.vis_dt = dt
+
+    def euler_startup(self):
+        """fluid_time_step.f90:675-748."""
+        n0 = self.n0
+        nlk, u = self.cal_nlk(self.time, self.uk)
+        self.nlk[n0] = nlk
+        self.u = u
+        self.dt1 = self.adjust_dt(self.time, u)
+        self._vis_for(self.dt1)
+        self.uk = (self.uk + self.dt1 * self.nlk[n0]) * self.vis[None]
+        self.nlk[n0] = self.nlk[n0] * self.vis[None]
+
+    def adamsbashforth(self):
+        """fluid_time_step.f90:752-832."""
+        n0, n1 = self.n0, self.n1
+        nlk, u = self.cal_nlk(self.time, self.uk)
+        self.nlk[n0] = nlk
+        self.u = u
+        dt1 = self.adjust_dt(self.time, u)
+        if dt1 > self.p.tmax - self.time:
+            dt1 = self.p.tmax - self.time
+        self.dt1 = dt1
+        dt0 = self.dt0
+        b10 = dt1 / dt0 * (0.5 * dt1 + dt0)
+        b11 = -0.5 * dt1 * dt1 / dt0
+        self._vis_for(dt1)
+        self.uk = (self.uk + b10 * self.nlk[n0] + b11 * self.nlk[n1]) * self.vis[None]
+        self.nlk[n0] = self.nlk[n0] * self.vis[None]
+
+    def rungekutta2(self):
+        """fluid_time_step.f90:519-598 (stale integrating factor on N0, N1 at the
+        same time and neither IF-weighted nor dealiased)."""
+        nlk0, u = self.cal_nlk(self.time, self.uk)
+        self.u = u
+        self.dt1 = self.adjust_dt(self.time, u)
+        nlk0 = nlk0 * self.vis[None]
+        self._vis_for(self.dt1)
+        self.uk = self.uk * self.vis[None] + self.dt1 * nlk0
+        nlk1, u = self.cal_nlk(self.time, self.uk)
+        self.u = u
+        self.uk = self.uk + 0.5 * self.dt1 * (-nlk0 + nlk1)
+        self.nlk[0], self.nlk[1] = nlk0, nlk1
+
+    def rungekutta4(self):
+        """fluid_time_step.f90:602-671 (explicit diffusion, dealias every stage)."""
+        g = self.g
+        u0 = self.uk.copy()
+        k1, u = self.cal_nlk(self.time, self.uk)
+        k1 = dealias(g, k1)
+        self.u = u
+        dt = self.adjust_dt(self.time, u)
+        self.dt1 = dt
+        self.uk = u0 + 0.5 * dt * k1
+        k2, _ = self.cal_nlk(self.time + 0.5 * dt, self.uk)
+        k2 = dealias(g, k2)
+        self.uk = u0 + 0.5 * dt * k2
+        k3, _ = self.cal_nlk(self.time + 0.5 * dt, self.uk)
+        k3 = dealias(g, k3)
+        self.uk = u0 + dt * k3
+        k4, _ = self.cal_nlk(self.time + dt, self.uk)
+        k4 = dealias(g, k4)
+        self.uk = u0 + (dt / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+        self.uk = dealias(g, self.uk)
+
+    def fluid_timestep(self):
+        """fluid_time_step.f90:34-150 FluidTimestep (fsi branch)."""
+        m = self.p.iTimeMethodFluid
+        if m == "RK2":
+            self.rungekutta2()
+        elif m == "RK4":
+            self.rungekutta4()
+        elif m == "AB2":
+            if self.it == 0:
+                self.euler_startup()
+            else:
+                self.adamsbashforth()
+        else:
+            raise FlusiAbort(10001, "Error! iTimeMethodFluid unknown. Abort.")
+        self.set_mean_flow(self.time)
+        self._emit("ekin.t", [self.time + self.dt1, energy_k(self.g, self.uk)])
+
+    # -- diagnostics ------------------------------------------------------------
+    def cal_drag(self, time: float, u: np.ndarray):
+        """drag.f90:17-454 cal_drag (non-SX branch); fills forces.t,
+        forces_partN.t (insects) and u_residual.t rows."""
+        p, g = self.p, self.g
+        if p.iPenalization != 1:
+            return None
+        eps_inv = 1.0 / p.eps
+        du = u - self.us
+        pen = -self.mask[None] * du * eps_inv
+        col = self.color.astype(np.int64)
+        X = (g.x[None, None, :] - p.x0) + np.zeros(self.mask.shape)
+        Y = (g.y[None, :, None] - p.y0) + np.zeros(self.mask.shape)
+        Z = (g.z[:, None, None] - p.z0) + np.zeros(self.mask.shape)
+        X, Y, Z = periodize([X, Y, Z], (p.xl, p.yl, p.zl), p.periodic)
+        t0 = [-(Y * pen[2] - Z * pen[1]), -(Z * pen[0] - X * pen[2]), -(X * pen[1] - Y * pen[0])]
+        if p.iMask == "Insect":
+            ins = self.insect.st
+            Xp, Yp, Zp = X.copy(), Y.copy(), Z.copy()
+            for cc, piv in ((self.insect.color_l, ins["x_pivot_l_g"]), (self.insect.color_r, ins["x_pivot_r_g"])):
+                s = col == cc
+                Xp[s] -= piv[0]
+                Yp[s] -= piv[1]
+                Zp[s] -= piv[2]
+            Xp, Yp, Zp = periodize([Xp, Yp, Zp], (p.xl, p.yl, p.zl), p.periodic)
+        else:
+            Xp, Yp, Zp = X, Y, Z
+        tp = [-(Yp * pen[2] - Zp * pen[1]), -(Zp * pen[0] - Xp * pen[2]), -(Xp * pen[1] - Yp * pen[0])]
+        dv = g.dx * g.dy * g.dz
+        force = np.zeros((6, 3))
+        torque0 = np.zeros((6, 3))
+        torque = np.zeros((6, 3))
+        ures = np.full(6, 9.0e9)
+        d2 = du[0] ** 2 + du[1] ** 2 + du[2] ** 2
+        inside = self.mask > 0.0
+        for c in range(6):
+            s = col == c
+            if not s.any():
+                continue
+            for i in range(3):
+                force[c, i] = -pen[i][s].sum()
+                torque0[c, i] = t0[i][s].sum()
+                torque[c, i] = tp[i][s].sum()
+            si = s & inside
+            if si.any():
+                ures[c] = min(ures[c], float(d2[si].min()))
+        power_xyz = np.array([(self.us[i] * pen[i]).sum() for i in range(3)]) * dv
+        force *= dv
+        torque0 *= dv
+        torque *= dv
+        F = force[1:6].sum(axis=0)
+        T = torque0[1:6].sum(axis=0)
+        power = power_xyz[0] + power_xyz[1] + power_xyz[2]
+        self.last_integrals = {"penalization_power": [power, *power_xyz], "force": F, "torque": T,
+                               "force_part": force, "torque_part": torque, "u_residual": ures}
+        z3 = [0.0, 0.0, 0.0]
+        if p.iMask == "Insect":
+            apow, ipow, atot, itot = self.insect.aero_inert_power({2: torque[2], 3: torque[3]})
+            self._emit("forces.t", [time, *F, *z3, *T, *z3, atot, itot])
+            for c in (1, 2, 3):
+                self._emit(f"forces_part{c}.t", [time, *force[c], *z3, *torque[c], *z3, apow[c], ipow[c]])
+        else:
+            self._emit("forces.t", [time, *F, *z3, *T, *z3])
+        self._emit("u_residual.t", [time, *np.sqrt(ures)])
+        return self.last_integrals
+
+    def write_integrals(self, time: float):
+        """integrals.f90:48-238 write_integrals_fsi."""
+        p, g = self.p, self.g
+        if p.iMask == "Insect" and self.insect is not None:
+            self._emit("kinematics.t", self.insect.kinematics_row(time))
+        u = ifft3(g, self.uk)
+        if p.compute_forces == 1:
+            if p.iMoving == 1:
+                self.create_mask(time)
+            self.cal_drag(time, u)
+        div = ifft(g, divergence(g, self.uk))
+        self._emit("divu.t", [time, float(div.max()), float((div * (1.0 - self.mask)).max())])
+        ek = compute_energies(g, u)
+        ekf = compute_energies(g, u * (1.0 - self.mask)[None])
+        vort = ifft3(g, curl(g, self.uk))
+        di = compute_energies(g, vort)
+        dif = compute_energies(g, vort * (1.0 - self.mask)[None])
+        di = [2.0 * p.nu * v for v in di]
+        dif = [2.0 * p.nu * v for v in dif]
+        pp = self.last_integrals["penalization_power"]
+        self._emit("energy.t", [time, *ekf, *dif, *ek, *di, *pp])
+        self._emit("meanflow.t", [time, *self.uk[:, 0, 0, 0].real])
+        dv = g.dx * g.dy * g.dz
+        self._emit("mask_volume.t", [time, float(self.mask.sum()) * dv,
+                                     *[float((self.mask * self.us[i]).sum()) * dv for i in range(3)]])
+
+    def field_keys(self, time: float, name_it: Optional[str] = None):
+        """Key values (postprocessing/keyvalues.f90:7-80) of the fields
+        save_fields writes (velocity, vorticity, mask, us), computed on the
+        float32-rounded field as the default field_precision=single does."""
+        g = self.g
+        tag = name_it or "%06d" % int(math.floor(time * 1000.0))
+        u = ifft3(g, self.uk)
+        vort = ifft3(g, curl(g, self.uk))
+        out = {}
+        for nm, fld in (("ux", u[0]), ("uy", u[1]), ("uz", u[2]),
+                        ("vorx", vort[0]), ("vory", vort[1]), ("vorz", vort[2]),
+                        ("mask", self.mask), ("usx", self.us[0]), ("usy", self.us[1]), ("usz", self.us[2])):
+            out[f"{nm}_{tag}"] = keyvalues(fld, time, self.p)
+        self.keys.update(out)
+        return out
+
+    # -- main loop ---------------------------------------------------------------
+    def run(self, nsteps: Optional[int] = None, save_keys: bool = False):
+        """time_step.f90:1-180: the loop runs while time < tmax and it <= nt."""
+        p = self.p
+        self.init_fields()
+        if save_keys:
+            self.field_keys(self.time)
+        self.write_integrals(self.time)
+        nt = p.nt if nsteps is None else nsteps - 1
+        while self.time < p.tmax and self.it <= nt:
+            self.step()
+            if time_for_output(self.time, self.dt1, self.it, p.tintegral, p.itdrag, p.tmax, 0.0, p):
+                self.write_integrals(self.time)
+            if save_keys and time_for_output(self.time, self.dt1, self.it, p.tsave, 99999999, p.tmax,
+                                             p.tsave_first, p):
+                self.field_keys(self.time)
+        return self
+
+    def step(self):
+        self.dt0 = self.dt1
+        self.fluid_timestep()
+        self.n0, self.n1 = self.n1, self.n0
+        self.time = self.time + self.dt1
+        self.it += 1
+
+
+def startup_conditioner(time, time_release, time_tau):
+    """vars.f90:408-422."""
+    t = time - time_release
+    if time <= time_release:
+        return 0.0
+    if time > time_release and time < time_release + time_tau:
+        return (t ** 3) / (-0.5 * time_tau ** 3) + 3.0 * (t ** 2) / time_tau ** 2
+    return 1.0
+
+
+def time_for_output(time, dt, it, tfrequ, ifrequ, tmax, tfirst, p: Params) -> bool:
+    """vars.f90:429-462."""
+    if time < tfirst:
+        return False
+    if p.intelligent_dt == "yes":
+        tn1 = float(math.ceil(time / tfrequ)) * tfrequ
+        tn2 = float(math.floor(time / tfrequ)) * tfrequ
+        return (abs(time - tn1) <= 1e-6 or abs(time - tn2) <= 1e-6 or it % ifrequ == 0
+                or abs(time - tmax) <= 1e-6)
+    return (_fmod(time, tfrequ) < dt) or (it % ifrequ == 0) or (time == tmax)
+
+
+def _fmod(a, p):
+    """Fortran MODULO for reals (result has the sign of p)."""
+    return a - math.floor(a / p) * p
+
+
+def truncate(a: float) -> float:
+    """fluid_time_step.f90:1051-1060: write(str,'(es7.1)') a; read(str,*) a."""
+    return float("%.1E" % a)
+
+
+# ---------------------------------------------------------------------------
+# Fixture protocol (postprocessing/keyvalues.f90, compare_key.f90,
+# compare_timeseries.f90)
+# ---------------------------------------------------------------------------
+def keyvalues(field_xyz: np.ndarray, time: float, p: Params, single: bool = True) -> List[float]:
+    """keyvalues.f90:7-80 on a (nz,ny,nx) field (float32-rounded first when
+    the field was saved in single precision)."""
+    f = field_xyz.astype(np.float32).astype(np.float64) if single else field_xyz
+    nz, ny, nx = f.shape
+    x = np.arange(nx) * p.xl / nx
+    y = np.arange(ny) * p.yl / ny
+    z = np.arange(nz) * p.zl / nz
+    q = (x[None, None, :] * f + y[None, :, None] * f + z[:, None, None] * f).sum()
+    n = float(nx) * float(ny) * float(nz)
+    return [float(time), float(f.max()), float(f.min()), float(f.sum()) / n, float((f ** 2).sum()) / n, float(q) / n]
+
+
+def read_key(path: str) -> List[float]:
+    with open(path) as fh:
+        return [float(v) for v in fh.read().split()[:6]]
+
+
+def compare_key(present: List[float], reference: List[float]) -> Tuple[bool, List[float]]:
+    """compare_key.f90:60-99: relative error, absolute where |ref| < 1e-7; all < 1e-4."""
+    errs = []
+    for a, b in zip(present, reference):
+        errs.append(abs((b - a) / b) if abs(b) >= 1.0e-7 else abs(b - a))
+    return all(e < 1.0e-4 for e in errs), errs
+
+
+def read_timeseries(path: str) -> List[List[float]]:
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            s = line.strip()
+            if not s or s.startswith("%"):
+                continue
+            rows.append([float(v) for v in s.split()])
+    return rows
+
+
+def compare_timeseries(present: List[List[float]], reference: List[List[float]]) -> Tuple[bool, float]:
+    """compare_timeseries.f90:60-126: per entry, relative error where
+    |ref| > 1e-4 and |diff| > 1e-7 (else 0); fail if any exceeds 1e-4."""
+    worst = 0.0
+    for r1, r2 in zip(present, reference):
+        for a, b in zip(r1, r2):
+            d = a - b
+            e = abs(d / b) if (abs(b) > 1.0e-4 and abs(d) > 1.0e-7) else 0.0
+            worst = max(worst, e)
+    return worst <= 1.0e-4, worst
+
+
+def format_series_row(row, fmt="es15.8") -> str:
+    return " ".join("% .8E" % v for v in row)

@@ -1,0 +1,253 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle and with
+the reference's own fixtures.  Tolerances: FP64 runs within 1e-11..1e-9
+relative L2 of the oracle (round-off of reordered FFT sums); the FP32-mixed
+path within the single-precision FFT error (1e-5 for one RHS)."""
+import os
+
+import numpy as np
+import pytest
+
+import flusi_amd
+from oracle import flusi_oracle as fo
+from tests._util import CONFIGS, GOLDEN, rel_l2, rewrite_ini
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [(flusi_amd.FP64, 1e-12), (flusi_amd.FP32_MIXED, 2e-6)]
+
+
+def _base_ini(tmp_path, **kw):
+    defaults = dict(Time__dt_fixed="1.0e-3", Time__iTimeMethodFluid="AB2")
+    defaults.update(kw)
+    return rewrite_ini(os.path.join(GOLDEN, "vortex_ring1_AB2", "vortexring1_AB2.ini"), str(tmp_path), **defaults)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 24, 64), (16, 96, 8), (128, 64, 48)])
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_fft_ifft_match_oracle(tmp_path, shape, prec, tol):
+    nx, ny, nz = shape
+    ini = _base_ini(tmp_path, Resolution__nx=nx, Resolution__ny=ny, Resolution__nz=nz,
+                    Geometry__xl=1.3, Geometry__yl=2.2, Geometry__zl=0.9)
+    p, _ = fo.get_params(ini)
+    g = fo.Grid(p)
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((nz, ny, nx))
+    with flusi_amd.Flusi(ini, prec) as s:
+        fk = s.fft(f)
+        assert rel_l2(fk, fo.fft(g, f)) < tol * 5
+        back = s.ifft(fo.fft(g, f))
+        assert rel_l2(back, f) < tol * 5
+        # imaginary parts of the kx=0 / kx=nx/2 planes are ignored like FFTW's c2r
+        fk2 = fo.fft(g, f)
+        fk2[0] += 1j * 0.37
+        fk2[-1] -= 1j * 0.21
+        assert rel_l2(s.ifft(fk2), fo.ifft(g, fk2)) < tol * 5
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_vortex_ring_init_and_cal_nlk(tmp_path, prec, tol):
+    """init_fields_fsi.f90:381-435 + rhs.f90:2-115 on the 32^3 ring fixture."""
+    ini = os.path.join(GOLDEN, "vortex_ring1_AB2", "vortexring1_AB2.ini")
+    sim = fo.Simulation(ini)
+    sim.init_fields()
+    nlk_ref, u_ref = sim.cal_nlk(0.0, sim.uk)
+    with flusi_amd.Flusi(ini, prec) as s:
+        s.init_fields()
+        assert rel_l2(s.get_uk(), sim.uk) < tol * 10
+        nlk, u = s.cal_nlk(0.0, want_u=True)
+        assert rel_l2(nlk, nlk_ref) < tol * 20
+        assert rel_l2(u, u_ref) < tol * 10
+        assert rel_l2(s.get_vorticity(), fo.ifft3(sim.g, fo.curl(sim.g, sim.uk))) < tol * 10
+
+
+def _perturbed_state(sim, seed=5, amp=0.3):
+    """A smooth solenoidal random field added to the oracle's initial uk."""
+    g = sim.g
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((3, g.nxh, g.ny, g.nz)) + 1j * rng.standard_normal((3, g.nxh, g.ny, g.nz))
+    v *= np.exp(-0.5 * g.K2 / (4.0 * (2 * np.pi / g.p.xl) ** 2))[None]
+    v = fo.add_grad_pressure(g, v)
+    v[:, 0, 0, 0] = 0.0
+    # make it the spectrum of a real field
+    v = fo.fft3(g, fo.ifft3(g, v))
+    v *= amp / max(1e-30, np.sqrt(fo.energy_k(g, v)))
+    return sim.uk + v
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_sphere_cavity_sponge_cal_nlk(tmp_path, prec, tol):
+    """Penalised sphere + cavity vorticity sponge (48^3 fixture setup)."""
+    ini = os.path.join(GOLDEN, "sphere_sponge_RK2", "sphere_sponge_RK2.ini")
+    sim = fo.Simulation(ini)
+    sim.init_fields()
+    uk = _perturbed_state(sim)
+    nlk_ref, _ = sim.cal_nlk(0.0, uk)
+    with flusi_amd.Flusi(ini, prec) as s:
+        s.init_fields()
+        m, us, col = s.get_mask()
+        assert np.max(np.abs(m - sim.mask)) < 1e-6 if prec else np.max(np.abs(m - sim.mask)) < 1e-14
+        assert np.array_equal(col, sim.color.astype(np.int16))
+        s.set_uk(uk)
+        nlk = s.cal_nlk(0.0)
+        assert rel_l2(nlk, nlk_ref) < tol * 50
+
+
+def _run_both(ini, prec, nsteps, perturb=False):
+    sim = fo.Simulation(ini)
+    sim.init_fields()
+    s = flusi_amd.Flusi(ini, prec)
+    s.init_fields()
+    if perturb:
+        sim.uk = _perturbed_state(sim)
+        s.set_uk(sim.uk)
+    for _ in range(nsteps):
+        sim.step()
+        s.step()
+    return sim, s
+
+
+def test_ab2_vortex_ring_steps_fp64(tmp_path):
+    """euler_startup + 19 adamsbashforth steps with CFL-adapted dt."""
+    ini = rewrite_ini(os.path.join(GOLDEN, "vortex_ring1_AB2", "vortexring1_AB2.ini"), str(tmp_path),
+                      Saving__tintegral=100.0, Saving__tsave=100.0)
+    sim, s = _run_both(ini, flusi_amd.FP64, 20)
+    t, it, dt0, dt1 = s.get_time()
+    assert it == 20 and t == pytest.approx(sim.time, rel=1e-14) and dt1 == sim.dt1
+    assert rel_l2(s.get_uk(), sim.uk) < 1e-10
+    s.close()
+
+
+def test_rk2_sphere_sponge_steps_fp64(tmp_path):
+    """RK2 with the stale integrating factor at it=0, cavity sponge, fixed mean flow."""
+    ini = rewrite_ini(os.path.join(GOLDEN, "sphere_sponge_RK2", "sphere_sponge_RK2.ini"), str(tmp_path),
+                      Saving__tintegral=100.0, Saving__itdrag=100000, Saving__tsave=100.0)
+    sim, s = _run_both(ini, flusi_amd.FP64, 8, perturb=True)
+    assert rel_l2(s.get_uk(), sim.uk) < 1e-10
+    s.close()
+
+
+def _insect_ini(tmp_path, n=64, **kw):
+    over = dict(Resolution__nx=n, Resolution__ny=n, Resolution__nz=n, Time__nt=2,
+                Time__dt_fixed="4.0e-4", Penalization__eps="1.0e-2", Saving__tintegral=100.0,
+                Saving__itdrag=100000)
+    over.update(kw)
+    return rewrite_ini(os.path.join(CONFIGS, "cfg3_insect_512.ini"), str(tmp_path), **over)
+
+
+def test_rk4_insect_steps_fp64(tmp_path):
+    """RK4 with explicit diffusion, dealiasing and the insect mask regenerated
+    at t, t+dt/2, t+dt/2, t+dt (the flapping-insect workload at 64^3)."""
+    ini = _insect_ini(tmp_path)
+    sim, s = _run_both(ini, flusi_amd.FP64, 3, perturb=True)
+    m, us, col = s.get_mask()
+    assert np.max(np.abs(m - sim.mask)) < 1e-12
+    assert np.max(np.abs(us - sim.us)) < 1e-10
+    assert np.array_equal(col, sim.color.astype(np.int16))
+    assert rel_l2(s.get_uk(), sim.uk) < 1e-10
+    s.close()
+
+
+@pytest.mark.parametrize("prec", [flusi_amd.FP64, flusi_amd.FP32_MIXED])
+def test_jerry_masks_match_fixture(tmp_path, prec):
+    """Mask-only grid 128x64x86 (no FFT plan needed): create_mask at t = 0,
+    0.5, 1.0 against the reference fixture keys, with the wing-erosion quirk
+    of delete_old_mask carried across calls."""
+    d = os.path.join(GOLDEN, "insects_mask_jerry")
+    ini = os.path.join(d, "jerry.ini")
+    p, _ = fo.get_params(ini)
+    with flusi_amd.Flusi(ini, prec) as s:
+        s.init_fields()     # quiescent: only the insect set-up and the first mask
+        for t in (0.0, 0.5, 1.0):
+            s.create_mask(t)
+            m, us, _ = s.get_mask()
+            tag = "%06d" % int(t * 1000)
+            for nm, f in (("mask", m), ("usx", us[0]), ("usy", us[1]), ("usz", us[2])):
+                ok, errs = fo.compare_key(fo.keyvalues(f, t, p), fo.read_key(os.path.join(d, f"{nm}_{tag}.ref")))
+                assert ok, (nm, t, errs)
+
+
+def test_sphere_sponge_fixture_forces_fp64(tmp_path):
+    """The whole sphere_sponge_RK2 fixture run (354 RK2 steps, CFL-adapted dt)
+    through flusi_run; forces.t / dt.t / energy.t against the reference's files
+    with compare_timeseries.f90's rule."""
+    d = os.path.join(GOLDEN, "sphere_sponge_RK2")
+    out = tmp_path / "out"
+    out.mkdir()
+    with flusi_amd.Flusi(os.path.join(d, "sphere_sponge_RK2.ini"), flusi_amd.FP64, output_dir=str(out)) as s:
+        s.run()
+    for ts in ("forces.t", "dt.t", "energy.t", "meanflow.t"):
+        ref = fo.read_timeseries(os.path.join(d, ts))
+        got = fo.read_timeseries(str(out / ts))
+        n = min(len(ref), len(got))
+        assert n >= len(ref) - 1, ts
+        ncol = min(len(ref[0]), len(got[0]))
+        ok, worst = fo.compare_timeseries([r[:ncol] for r in got[:n]], [r[:ncol] for r in ref[:n]])
+        assert ok, (ts, worst)
+
+
+def test_vortex_ring1_fixture_run_fp64(tmp_path):
+    """vortex_ring1_AB2 fixture to T=0.5: dt.t, energy.t and the final keys."""
+    d = os.path.join(GOLDEN, "vortex_ring1_AB2")
+    out = tmp_path / "out"
+    out.mkdir()
+    ini = os.path.join(d, "vortexring1_AB2.ini")
+    p, _ = fo.get_params(ini)
+    with flusi_amd.Flusi(ini, flusi_amd.FP64, output_dir=str(out)) as s:
+        s.run()
+        t = s.get_time()[0]
+        u = s.get_u()
+        w = s.get_vorticity()
+    for ts in ("dt.t", "energy.t"):
+        ref = fo.read_timeseries(os.path.join(d, ts))
+        got = fo.read_timeseries(str(out / ts))
+        n = min(len(ref), len(got))
+        ok, worst = fo.compare_timeseries(got[:n], ref[:n])
+        assert ok, (ts, worst)
+    for nm, f in (("ux", u[0]), ("uy", u[1]), ("uz", u[2]), ("vorx", w[0]), ("vory", w[1]), ("vorz", w[2])):
+        ok, errs = fo.compare_key(fo.keyvalues(f, t, p), fo.read_key(os.path.join(d, f"{nm}_000500.ref")))
+        assert ok, (nm, errs)
+
+
+def test_write_integrals_insect_matches_oracle(tmp_path):
+    """cal_drag (drag.f90:17-454) per part + energies (integrals.f90:48-238)."""
+    ini = _insect_ini(tmp_path)
+    sim = fo.Simulation(ini)
+    sim.init_fields()
+    sim.uk = _perturbed_state(sim)
+    with flusi_amd.Flusi(ini, flusi_amd.FP64) as s:
+        s.init_fields()
+        s.set_uk(sim.uk)
+        o = s.write_integrals(0.25)
+    sim.write_integrals(0.25)
+    F = sim.last_integrals["force"]
+    T = sim.last_integrals["torque"]
+    assert np.allclose(o[0:3], F, rtol=1e-10, atol=1e-12)
+    assert np.allclose(o[3:6], T, rtol=1e-10, atol=1e-12)
+    fp = sim.last_integrals["force_part"]
+    tp = sim.last_integrals["torque_part"]
+    for c in (1, 2, 3):
+        assert np.allclose(o[32 + 3 * (c - 1):35 + 3 * (c - 1)], fp[c], rtol=1e-10, atol=1e-12)
+        assert np.allclose(o[44 + 3 * (c - 1):47 + 3 * (c - 1)], tp[c], rtol=1e-10, atol=1e-12)
+    e = sim.series["energy.t"][-1]
+    assert np.allclose(o[10:26], e[1:17], rtol=1e-10, atol=1e-14)
+    assert np.allclose(o[6:10], e[17:21], rtol=1e-10, atol=1e-12)
+    kin = sim.series["forces.t"][-1]
+    assert o[59] == pytest.approx(kin[13], rel=1e-9, abs=1e-12)
+
+
+def test_mixed_precision_force_error_sphere(tmp_path):
+    """FP32-mixed against FP64 on the penalised sphere with identical fixed
+    steps: forces agree to the FP32 FFT round-off level."""
+    ini = rewrite_ini(os.path.join(GOLDEN, "sphere_sponge_RK2", "sphere_sponge_RK2.ini"), str(tmp_path),
+                      Time__dt_fixed="1.0e-2", Saving__tintegral=100.0, Saving__itdrag=100000, Saving__tsave=100.0)
+    res = {}
+    for prec in (flusi_amd.FP64, flusi_amd.FP32_MIXED):
+        with flusi_amd.Flusi(ini, prec) as s:
+            s.init_fields()
+            s.advance(20)
+            t = s.get_time()[0]
+            res[prec] = s.write_integrals(t)
+    f64, f32 = res[flusi_amd.FP64][0:3], res[flusi_amd.FP32_MIXED][0:3]
+    rel = abs(f32[0] - f64[0]) / abs(f64[0])
+    print("sphere drag rel err FP32-mixed vs FP64:", rel)
+    assert rel < 1e-4

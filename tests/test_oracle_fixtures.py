@@ -1,0 +1,80 @@
+"""The oracle against the reference's own regression fixtures (golden data
+from tests/tests_data.tar.gz of the reference, copied under tests/golden/),
+compared with the reference's tolerance rules (postprocessing/compare_key.f90,
+compare_timeseries.f90).  CPU only."""
+import glob
+import os
+
+import pytest
+
+from oracle import flusi_oracle as fo
+
+
+def _ref_keys(d):
+    """6-value key files of the fixture, indexed by (prefix, stored time)."""
+    out = {}
+    for f in sorted(glob.glob(os.path.join(d, "*.ref"))):
+        vals = open(f).read().split()
+        if len(vals) < 6:
+            continue
+        pre = os.path.basename(f).split("_")[0]
+        out[(pre, round(float(vals[0]), 6))] = fo.read_key(f)
+    return out
+
+
+def _check_run(d, ini, series):
+    sim = fo.Simulation(os.path.join(d, ini), workers=4).run(save_keys=True)
+    mine = {(k.split("_")[0], round(v[0], 6)): v for k, v in sim.keys.items()}
+    refs = _ref_keys(d)
+    assert refs
+    for key, ref in refs.items():
+        assert key in mine, key
+        ok, errs = fo.compare_key(mine[key], ref)
+        assert ok, (key, errs)
+    for ts in series:
+        ref = fo.read_timeseries(os.path.join(d, ts))
+        got = sim.series[ts]
+        n = min(len(ref), len(got))
+        assert n >= len(ref) - 1
+        ncol = min(len(ref[0]), len(got[0]))
+        ok, worst = fo.compare_timeseries([r[:ncol] for r in got[:n]], [r[:ncol] for r in ref[:n]])
+        assert ok, (ts, worst)
+    return sim
+
+
+def test_insects_mask_jerry(golden):
+    """Jerry body + TwoEllipses wings, Drosophila_hovering_fry / flapper_dickinson
+    kinematics, dry run at t=0, 0.5, 1.0 (reference tests/insects_mask_jerry.sh)."""
+    d = os.path.join(golden, "insects_mask_jerry")
+    sim = fo.Simulation(os.path.join(d, "jerry.ini"))
+    sim.insect = fo.Insect(sim.ini, sim.p, d)
+    rows = []
+    for t in (0.0, 0.5, 1.0):
+        sim.create_mask(t)
+        rows.append(sim.insect.kinematics_row(t))
+        tag = "%06d" % int(t * 1000)
+        for nm, f in (("mask", sim.mask), ("usx", sim.us[0]), ("usy", sim.us[1]), ("usz", sim.us[2])):
+            ok, errs = fo.compare_key(fo.keyvalues(f, t, sim.p), fo.read_key(os.path.join(d, f"{nm}_{tag}.ref")))
+            assert ok, (nm, t, errs)
+    ok, worst = fo.compare_timeseries(rows, fo.read_timeseries(os.path.join(d, "kinematics.dry-run.t")))
+    assert ok, worst
+
+
+def test_vortex_ring1_AB2(golden):
+    _check_run(os.path.join(golden, "vortex_ring1_AB2"), "vortexring1_AB2.ini", ["dt.t", "energy.t", "meanflow.t"])
+
+
+def test_vortex_ring2_RK2(golden):
+    _check_run(os.path.join(golden, "vortex_ring2_RK2"), "vortexring2_RK2.ini", ["dt.t", "energy.t", "meanflow.t"])
+
+
+@pytest.mark.slow
+def test_sphere_sponge_RK2(golden):
+    """48^3 RK2 + cavity vorticity sponge, forces.t compared (sphere_sponge_RK2.sh:75-88)."""
+    _check_run(os.path.join(golden, "sphere_sponge_RK2"), "sphere_sponge_RK2.ini",
+               ["forces.t", "dt.t", "energy.t", "meanflow.t"])
+
+
+@pytest.mark.slow
+def test_sphere_AB2(golden):
+    _check_run(os.path.join(golden, "sphere"), "testing_sphere.ini", ["forces.t", "dt.t", "energy.t"])
