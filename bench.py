@@ -31,7 +31,7 @@ def make_ini(tmpdir, n, nt):
     txt = open(src).read()
     for k in ("nx", "ny", "nz"):
         txt = txt.replace(f"{k}=512;", f"{k}={n};")
-    txt = txt.replace("nt=999;", f"nt={nt};")
+    txt = txt.replace("nt=999;", f"nt={nt};").replace("itdrag=10;", "itdrag=99999;")
     path = os.path.join(tmpdir, "bench_insect.ini")
     with open(path, "w") as fh:
         fh.write(txt)
@@ -104,7 +104,18 @@ def main():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (counter calibration for HBM bytes)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": a.numel() * 4, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)

@@ -1,10 +1,12 @@
 // Device-side building blocks shared by every transform pass: complex helpers,
-// radix-2/3/4/8 butterflies and an in-LDS Stockham FFT over a tile of lines.
+// radix-2/3/4/8 butterflies, radix plans and Stockham stages over lines held
+// in LDS.
 //
-// A tile holds NL lines of length L in LDS, line i starting at i*LS.  All
-// threads of the workgroup cooperate: each Stockham stage loads the radix-r
-// butterflies a thread owns into registers, synchronises, and stores them in
-// auto-sorted order, so the transform runs in place in one LDS buffer.
+// LDS addressing is padded: element a of a line sits at a + (a >> PS) (one
+// spare slot per 16 complex<float> / 8 complex<double>, i.e. per bank row),
+// and lines are LSP slots apart with LSP = 1 modulo a bank row, so that both
+// the stride-R writes of early Stockham stages and the line-strided accesses
+// of the transposing passes avoid bank conflicts.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,29 +35,35 @@ template <int S, typename C> __device__ __forceinline__ C mul_si(C a) {
 }
 template <typename C, typename R> __device__ __forceinline__ C scl(C a, R s) { C r; r.x = a.x * s; r.y = a.y * s; return r; }
 
+// ---------------------------------------------------------------- LDS padding
+template <typename T> struct Pad { static constexpr int PS = (sizeof(T) == 4) ? 4 : 3; };
+template <typename T> __host__ __device__ constexpr int lp(int a) { return a + (a >> Pad<T>::PS); }
+// slots of one line holding n elements, rounded up to 1 modulo a bank row
+template <typename T> __host__ __device__ constexpr int line_stride(int n) {
+  return (lp<T>(n - 1) + 1) + ((((1 - (lp<T>(n - 1) + 1)) % (1 << Pad<T>::PS)) + (1 << Pad<T>::PS)) % (1 << Pad<T>::PS));
+}
+
 // ---------------------------------------------------------------- radix plans
-// L = product of radices (3 first, then 8s, then one 4 or 2).  Supported
-// lengths are 2^a * 3^b with b <= 1 and L >= 2.
-__host__ __device__ constexpr int plan_count(int L) {
-  int n = 0;
-  if (L % 3 == 0) { L /= 3; ++n; }
-  while (L % 8 == 0 && L > 1) { L /= 8; ++n; }
-  if (L == 4 || L == 2) ++n;
-  return n;
+// PLAN_TILE: 3 first, then 8s, then one 4 or 2 (y/z passes, L >= 16).
+// PLAN_XF:   4 first, then 3, then 8s, then one 4 or 2 (x pass forward).
+// PLAN_XI:   PLAN_XF reversed (x pass inverse), so that the last inverse
+//            stage and the first forward stage share their index sets.
+enum { PLAN_TILE = 0, PLAN_XF = 1, PLAN_XI = 2 };
+
+__host__ __device__ constexpr int plan_list(int kind, int L, int s, bool count) {
+  int n = 0, m = L;
+  if (kind == PLAN_XF && m % 4 == 0 && m > 4) { if (!count && n == s) return 4; m /= 4; ++n; }
+  if (m % 3 == 0) { if (!count && n == s) return 3; m /= 3; ++n; }
+  while (m % 8 == 0 && m > 1) { if (!count && n == s) return 8; m /= 8; ++n; }
+  if (m == 4 || m == 2) { if (!count && n == s) return m; ++n; m = 1; }
+  return count ? (m == 1 ? n : -1) : 1;
 }
-__host__ __device__ constexpr int plan_radix(int L, int s) {
-  int n = 0;
-  if (L % 3 == 0) { if (n == s) return 3; L /= 3; ++n; }
-  while (L % 8 == 0 && L > 1) { if (n == s) return 8; L /= 8; ++n; }
-  if (L == 4) return 4;
-  if (L == 2) return 2;
-  return 1;
+__host__ __device__ constexpr int plan_count_k(int kind, int L) {
+  return plan_list(kind == PLAN_XI ? PLAN_XF : kind, L, 0, true);
 }
-__host__ __device__ constexpr bool plan_ok(int L) {
-  int m = L;
-  if (m % 3 == 0) m /= 3;
-  while (m % 2 == 0) m /= 2;
-  return m == 1 && L >= 2;
+__host__ __device__ constexpr int plan_radix_k(int kind, int L, int s) {
+  return kind == PLAN_XI ? plan_list(PLAN_XF, L, plan_count_k(PLAN_XF, L) - 1 - s, false)
+                         : plan_list(kind, L, s, false);
 }
 
 // ---------------------------------------------------------------- butterflies
@@ -101,35 +109,42 @@ template <int S, typename T> struct Bfly<8, S, T> {
   }
 };
 
-// ------------------------------------------------------------ Stockham stage
-// One stage of radix R over NL lines of length L held in LDS.  Ns is the
-// length of the already-transformed sub-sequences.  tw[m] = exp(-2 pi i m/L);
-// the inverse (S=+1) uses its conjugate.
-template <typename T, int L, int NL, int NT, int R, int S, int Ns>
-__device__ __forceinline__ void stockham_stage(cplx<T>* buf, int LS, const cplx<T>* __restrict__ tw) {
-  constexpr int NB = NL * (L / R);                 // butterflies in the tile
-  constexpr int PER = (NB + NT - 1) / NT;          // per thread (upper bound)
+// Stockham stage of radix R: butterfly j (0 <= j < L/R) reads positions
+// j + q L/R, multiplies input q by W_{Ns R}^{q k} (k = j mod Ns), applies the
+// radix-R DFT and writes positions (j/Ns) Ns R + k + q Ns.  tw[m] = exp(-2 pi
+// i m / L); the inverse (S = +1) uses its conjugate.
+template <typename T, int L, int R, int S, int Ns>
+__device__ __forceinline__ void twiddle_bfly(cplx<T>* v, int j, const cplx<T>* __restrict__ tw) {
+  if (Ns > 1) {
+    const int step = (j % Ns) * (L / (Ns * R));
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      const cplx<T> w = tw[q * step];
+      v[q] = (S < 0) ? cmul(v[q], w) : conjmul(v[q], w);
+    }
+  }
+  Bfly<R, S, T>::run(v);
+}
+template <int R, int Ns> __device__ __forceinline__ int stage_out(int j, int q) {
+  return (j / Ns) * Ns * R + (j % Ns) + q * Ns;
+}
+
+// One Stockham stage over NL lines (line i at i*LSP) held in LDS.
+template <typename T, int L, int NL, int NT, int R, int S, int Ns, int LSP>
+__device__ __forceinline__ void lds_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+  constexpr int NB = NL * (L / R);
+  constexpr int PER = (NB + NT - 1) / NT;
   cplx<T> v[PER][R];
   const int tid = threadIdx.x;
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
     if (NB % NT == 0 || b < NB) {
-      const int line = b / (L / R);
-      const int j = b - line * (L / R);
-      const int k = j % Ns;
-      cplx<T>* src = buf + line * LS + j;
+      const int line = b / (L / R), j = b - line * (L / R);
+      const cplx<T>* src = buf + line * LSP;
 #pragma unroll
-      for (int q = 0; q < R; ++q) v[p][q] = src[q * (L / R)];
-      if (Ns > 1) {
-        const int step = k * (L / (Ns * R));
-#pragma unroll
-        for (int q = 1; q < R; ++q) {
-          cplx<T> w = tw[q * step];
-          v[p][q] = (S < 0) ? cmul(v[p][q], w) : conjmul(v[p][q], w);
-        }
-      }
-      Bfly<R, S, T>::run(v[p]);
+      for (int q = 0; q < R; ++q) v[p][q] = src[lp<T>(j + q * (L / R))];
+      twiddle_bfly<T, L, R, S, Ns>(v[p], j, tw);
     }
   }
   __syncthreads();
@@ -137,39 +152,37 @@ __device__ __forceinline__ void stockham_stage(cplx<T>* buf, int LS, const cplx<
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
     if (NB % NT == 0 || b < NB) {
-      const int line = b / (L / R);
-      const int j = b - line * (L / R);
-      const int k = j % Ns;
-      cplx<T>* dst = buf + line * LS + (j / Ns) * Ns * R + k;
+      const int line = b / (L / R), j = b - line * (L / R);
+      cplx<T>* dst = buf + line * LSP;
 #pragma unroll
-      for (int q = 0; q < R; ++q) dst[q * Ns] = v[p][q];
+      for (int q = 0; q < R; ++q) dst[lp<T>(stage_out<R, Ns>(j, q))] = v[p][q];
     }
   }
   __syncthreads();
 }
 
-template <typename T, int L, int NL, int NT, int S, int STAGE, int NS, bool DONE = (STAGE >= plan_count(L))>
-struct StockhamRun;
-template <typename T, int L, int NL, int NT, int S, int STAGE, int NS>
-struct StockhamRun<T, L, NL, NT, S, STAGE, NS, false> {
-  static __device__ __forceinline__ void run(cplx<T>* buf, int LS, const cplx<T>* tw) {
-    constexpr int R = plan_radix(L, STAGE);
-    stockham_stage<T, L, NL, NT, R, S, NS>(buf, LS, tw);
-    StockhamRun<T, L, NL, NT, S, STAGE + 1, NS * R>::run(buf, LS, tw);
+// Stages [STAGE, LAST) of plan KIND; NS is the sub-transform length entering STAGE.
+template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE, int LAST, int NS,
+          bool DONE = (STAGE >= LAST)>
+struct Stages;
+template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE, int LAST, int NS>
+struct Stages<T, L, NL, NT, KIND, S, LSP, STAGE, LAST, NS, false> {
+  static __device__ __forceinline__ void run(cplx<T>* buf, const cplx<T>* tw) {
+    constexpr int R = plan_radix_k(KIND, L, STAGE);
+    lds_stage<T, L, NL, NT, R, S, NS, LSP>(buf, tw);
+    Stages<T, L, NL, NT, KIND, S, LSP, STAGE + 1, LAST, NS * R>::run(buf, tw);
   }
 };
-template <typename T, int L, int NL, int NT, int S, int STAGE, int NS>
-struct StockhamRun<T, L, NL, NT, S, STAGE, NS, true> {
-  static __device__ __forceinline__ void run(cplx<T>*, int, const cplx<T>*) {}
+template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE, int LAST, int NS>
+struct Stages<T, L, NL, NT, KIND, S, LSP, STAGE, LAST, NS, true> {
+  static __device__ __forceinline__ void run(cplx<T>*, const cplx<T>*) {}
 };
 
-// Full in-place FFT of NL lines in LDS (caller synchronises before the call;
-// the call ends with a barrier).  S = -1 forward (exp(-i...)), S = +1 inverse;
-// both unnormalised.
-template <typename T, int L, int NL, int NT, int S>
-__device__ __forceinline__ void block_fft(cplx<T>* buf, int LS, const cplx<T>* __restrict__ tw) {
-  static_assert(plan_ok(L), "unsupported FFT length");
-  StockhamRun<T, L, NL, NT, S, 0, 1>::run(buf, LS, tw);
+// product of the radices of stages [0, s) of a plan
+__host__ __device__ constexpr int plan_prefix(int kind, int L, int s) {
+  int p = 1;
+  for (int i = 0; i < s; ++i) p *= plan_radix_k(kind, L, i);
+  return p;
 }
 
 }  // namespace flusi

@@ -40,6 +40,8 @@ struct InsectArgs {
   int color_w[2];           // 3 (right), 2 (left)
   double vb[3];             // M_body * v_body
   double rot_body_b[3];
+  int use_box;              // 1: skip points outside the index boxes below
+  int box_lo[3][3], box_hi[3][3];  // [body, right wing, left wing][axis]
 };
 
 // drag.f90:17-454 + integrals.f90:48-238 reductions over physical u, w, div

@@ -90,9 +90,10 @@ void launch_cavity(const GridGeom& g, const CavityArgs& a, void* mask, void* us,
 // stage II body-motion transform (wings_geometry.f90:71-108).
 template <typename T>
 __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* color) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.npts) return;
-  const int ix = p % g.nx, iy = (p / g.nx) % g.ny, iz = p / ((long)g.nx * g.ny);
+  // grid: x in blocks of blockDim.x, y = blockIdx.y, z = blockIdx.z
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
+  if (ix >= g.nx) return;
+  const long p = ((long)iz * g.ny + iy) * g.nx + ix;
   const uint8_t c_old = color[p];
   const bool old_wing = (c_old == 2 || c_old == 3);
   double m;
@@ -102,13 +103,27 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
   else if (old_wing) { m = 0.0; c = 0; mask_loaded = true; }
   else { m = 0.0; c = c_old; }
   double u_s[3] = {0.0, 0.0, 0.0};
+  bool changed = a.full_reset || old_wing;
+  auto in_box = [&](int k) {
+    return !a.use_box || (ix >= a.box_lo[k][0] && ix <= a.box_hi[k][0] && iy >= a.box_lo[k][1] &&
+                          iy <= a.box_hi[k][1] && iz >= a.box_lo[k][2] && iz <= a.box_hi[k][2]);
+  };
+  const bool any = (a.draw_body && (in_box(0) || a.nspheres > 0)) || (a.draw_wing[0] && in_box(1)) ||
+                   (a.draw_wing[1] && in_box(2));
+  if (!any) {
+    // nothing can be drawn here: only the deletion of old wing points remains
+    if (changed) {
+      mask[p] = T(0); color[p] = 0;
+      us[p] = T(0); us[g.npts + p] = T(0); us[2 * g.npts + p] = T(0);
+    }
+    return;
+  }
   double xg[3] = {ix * g.dx - a.xc[0], iy * g.dy - a.xc[1], iz * g.dz - a.xc[2]};
   periodize_d(xg, g);
   double xb[3];
   matvec(a.Mb, xg, xb);
-  bool changed = a.full_reset || old_wing;
 
-  if (a.draw_body) {
+  if (a.draw_body && in_box(0)) {
     const double ab = a.L_body / 2.0, b = a.b_body;
     if (fabs(xb[1]) <= b + a.safety && fabs(xb[2]) <= b + a.safety && fabs(xb[0]) < a.L_body / 2 + a.safety) {
       const double rx = xb[0] / ab;
@@ -122,6 +137,8 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
         }
       }
     }
+  }
+  if (a.draw_body) {
     for (int sidx = 0; sidx < a.nspheres; ++sidx) {
       if (ix < a.sph_lo[sidx][0] || ix > a.sph_hi[sidx][0] || iy < a.sph_lo[sidx][1] || iy > a.sph_hi[sidx][1] ||
           iz < a.sph_lo[sidx][2] || iz > a.sph_hi[sidx][2]) continue;
@@ -137,7 +154,7 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
   }
 
   for (int w = 0; w < 2; ++w) {
-    if (!a.draw_wing[w]) continue;
+    if (!a.draw_wing[w] || !in_box(1 + w)) continue;
     const double d[3] = {xb[0] - a.piv[w][0], xb[1] - a.piv[w][1], xb[2] - a.piv[w][2]};
     double xw[3];
     matvec(a.Mw[w], d, xw);
@@ -182,7 +199,7 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
 template <typename T>
 void launch_insect(const GridGeom& g, const InsectArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s) {
   const int nt = 256;
-  hipLaunchKernelGGL(insect_k<T>, dim3((g.npts + nt - 1) / nt), dim3(nt), 0, s, g, a, (T*)mask, (T*)us, color);
+  hipLaunchKernelGGL(insect_k<T>, dim3((g.nx + nt - 1) / nt, g.ny, g.nz), dim3(nt), 0, s, g, a, (T*)mask, (T*)us, color);
 }
 
 // ------------------------------------------------------------- vortex ring
@@ -358,46 +375,71 @@ void launch_diag(const GridGeom& g, const DiagArgs& a, hipStream_t s) {
 
 // ------------------------------------------------------------ row reducer
 // Combines per-block partial rows in a fixed order (bitwise reproducible):
-// Neumaier-compensated sums, or max/min.
-__global__ void reduce_rows_k(const double* partial, int nrows, int nv, const int* kinds, double* out) {
-  __shared__ double sv[1024];
-  __shared__ double sc[1024];
-  const int tid = threadIdx.x;
-  for (int v = 0; v < nv; ++v) {
-    const int kind = kinds[v];
-    double s = (kind == 1) ? -1.0e308 : (kind == 2 ? 1.0e308 : 0.0), comp = 0.0;
-    for (int r = tid; r < nrows; r += blockDim.x) {
-      const double x = partial[(long)r * nv + v];
-      if (kind == 1) s = fmax(s, x);
-      else if (kind == 2) s = fmin(s, x);
+// Neumaier-compensated sums, or max/min.  Phase 1: a grid of (chunk, value)
+// blocks, each a fixed chunk of rows; phase 2: one thread per value walks the
+// chunk results in order.
+static constexpr int RED_CHUNKS = 64;
+static constexpr int RED_NT = 256;
+
+__device__ __forceinline__ void neu_add(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+
+__global__ __launch_bounds__(RED_NT) void reduce_rows_p1(const double* partial, int nrows, int nv, const int* kinds, double* mid) {
+  __shared__ double sv[RED_NT];
+  __shared__ double sc[RED_NT];
+  const int v = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+  const int kind = kinds[v];
+  const int chunk = (nrows + RED_CHUNKS - 1) / RED_CHUNKS;
+  const int r0 = b * chunk, r1 = min(nrows, r0 + chunk);
+  double s = (kind == 1) ? -1.0e308 : (kind == 2 ? 1.0e308 : 0.0), comp = 0.0;
+  for (int r = r0 + tid; r < r1; r += RED_NT) {
+    const double x = partial[(long)r * nv + v];
+    if (kind == 1) s = fmax(s, x);
+    else if (kind == 2) s = fmin(s, x);
+    else neu_add(s, comp, x);
+  }
+  sv[tid] = s; sc[tid] = comp;
+  __syncthreads();
+  for (int w = RED_NT / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      if (kind == 1) sv[tid] = fmax(sv[tid], sv[tid + w]);
+      else if (kind == 2) sv[tid] = fmin(sv[tid], sv[tid + w]);
       else {
-        const double t = s + x;
-        comp += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
-        s = t;
+        double a = sv[tid], c = sc[tid] + sc[tid + w];
+        neu_add(a, c, sv[tid + w]);
+        sv[tid] = a; sc[tid] = c;
       }
     }
-    sv[tid] = s; sc[tid] = comp;
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if (tid < w) {
-        if (kind == 1) sv[tid] = fmax(sv[tid], sv[tid + w]);
-        else if (kind == 2) sv[tid] = fmin(sv[tid], sv[tid + w]);
-        else {
-          const double a = sv[tid], b = sv[tid + w];
-          const double t = a + b;
-          const double e = (fabs(a) >= fabs(b)) ? (a - t) + b : (b - t) + a;
-          sv[tid] = t; sc[tid] = sc[tid] + sc[tid + w] + e;
-        }
-      }
-      __syncthreads();
-    }
-    if (tid == 0) out[v] = (kind == 0) ? sv[0] + sc[0] : sv[0];
-    __syncthreads();
+  }
+  if (tid == 0) {
+    mid[(long)v * RED_CHUNKS * 2 + 2 * b] = sv[0];
+    mid[(long)v * RED_CHUNKS * 2 + 2 * b + 1] = sc[0];
   }
 }
 
-void launch_reduce_rows(const double* partial, int nrows, int nv, const int* kinds_dev, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_rows_k, dim3(1), dim3(1024), 0, s, partial, nrows, nv, kinds_dev, out);
+__global__ void reduce_rows_p2(const double* mid, int nv, const int* kinds, double* out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int kind = kinds[v];
+  double s = mid[(long)v * RED_CHUNKS * 2], c = mid[(long)v * RED_CHUNKS * 2 + 1];
+  for (int b = 1; b < RED_CHUNKS; ++b) {
+    const double x = mid[(long)v * RED_CHUNKS * 2 + 2 * b];
+    if (kind == 1) s = fmax(s, x);
+    else if (kind == 2) s = fmin(s, x);
+    else { c += mid[(long)v * RED_CHUNKS * 2 + 2 * b + 1]; neu_add(s, c, x); }
+  }
+  out[v] = (kind == 0) ? s + c : s;
+}
+
+size_t reduce_rows_scratch(int nv) { return (size_t)nv * RED_CHUNKS * 2; }
+
+void launch_reduce_rows(const double* partial, int nrows, int nv, const int* kinds_dev, double* mid, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_rows_p1, dim3(RED_CHUNKS, nv), dim3(RED_NT), 0, s, partial, nrows, nv, kinds_dev, mid);
+  hipLaunchKernelGGL(reduce_rows_p2, dim3((nv + 63) / 64), dim3(64), 0, s, mid, nv, kinds_dev, out);
 }
 
 // ------------------------------------------------------------ conversions

@@ -58,6 +58,7 @@ struct XPassArgs {
   double sponge_mask_thr;
   int nz;
   double* partial;     // per block: umax^2, sum chi(u-us) x,y,z (4 doubles)
+  int want_stats;      // write the per-block statistics (adaptive dt / dynamic mean flow)
 };
 
 // z-forward pass with the fused spectral epilogue (projection, explicit
@@ -113,10 +114,12 @@ template <typename T> void launch_x_pass(const XPassArgs& a, int mode, hipStream
 template <typename T> void launch_zstep(const ZStepArgs& a, hipStream_t s);
 int x_pass_blocks(int nx, int nlines, bool fp64);
 int zstep_blocks(int nz, int ny, int nxh, bool fp64);
-bool fft_length_supported(int n);
+bool fft_length_supported(int n);   // y/z line lengths
+bool x_length_supported(int m);     // nx/2 of the real x axis
 
 // reductions (fields.hip)
 // out[v] = combine over rows of partial[row*nv + v]; kind[v]: 0 = Neumaier sum, 1 = max, 2 = min
-void launch_reduce_rows(const double* partial, int nrows, int nv, const int* kinds_host, double* out, hipStream_t s);
+void launch_reduce_rows(const double* partial, int nrows, int nv, const int* kinds_dev, double* mid, double* out, hipStream_t s);
+size_t reduce_rows_scratch(int nv);  // doubles of `mid` needed for nv values
 
 }  // namespace flusi

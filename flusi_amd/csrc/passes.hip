@@ -24,8 +24,9 @@ namespace flusi {
 template <typename T> struct Tile { static constexpr int TL = 16; };
 template <> struct Tile<double> { static constexpr int TL = 8; };
 
+// threads of a tile block: one per radix-8 butterfly of the tile, 64..1024
 __host__ __device__ constexpr int pick_nt(int n) {
-  return n % 512 == 0 ? 512 : (n % 256 == 0 ? 256 : (n % 128 == 0 ? 128 : 64));
+  return (n / 8) >= 1024 ? 1024 : ((n / 8) <= 64 ? 64 : ((n / 8 + 63) / 64) * 64);
 }
 
 template <typename C> __device__ __forceinline__ C conj_(C a) { a.y = -a.y; return a; }
@@ -52,11 +53,21 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // ======================================================== generic tile pass
+// One workgroup transforms TL lines of length L for each job: the first
+// Stockham stage is computed straight from the global loads, the middle
+// stages in LDS and the last stage straight into the global stores.  The
+// thread <-> butterfly mapping of the first (last) stage follows the input
+// (output) layout: consecutive threads walk the contiguous axis, or the TL
+// lines of a 128-byte segment.
 template <typename T, int L, int TL, int NT, int S, bool CONTIG_IN>
 __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
-  constexpr int LS = L + 1;
-  constexpr int EPT = TL * L / NT;
-  static_assert(TL * L % NT == 0, "tile must divide the block");
+  constexpr int NSTG = plan_count_k(PLAN_TILE, L);
+  static_assert(NSTG >= 2, "tile pass needs at least two stages");
+  constexpr int R0 = plan_radix_k(PLAN_TILE, L, 0);
+  constexpr int RL = plan_radix_k(PLAN_TILE, L, NSTG - 1);
+  constexpr int LSP = line_stride<T>(L);
+  constexpr int NB0 = TL * (L / R0), PER0 = (NB0 + NT - 1) / NT;
+  constexpr int NBL = TL * (L / RL), PERL = (NBL + NT - 1) / NT;
   extern __shared__ char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   const int tid = threadIdx.x;
@@ -64,38 +75,68 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
   const long c = blockIdx.y;
   const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw);
   const T sc = (T)a.scale;
-  for (int j = 0; j < a.njobs; ++j) {
-    const cplx<T>* in0 = reinterpret_cast<const cplx<T>*>(a.jobs[j].in0);
-    const cplx<T>* in1 = reinterpret_cast<const cplx<T>*>(a.jobs[j].in1);
-    cplx<T>* out = reinterpret_cast<cplx<T>*>(a.jobs[j].out);
-    const int op = a.jobs[j].op;
+  for (int jb = 0; jb < a.njobs; ++jb) {
+    const cplx<T>* in0 = reinterpret_cast<const cplx<T>*>(a.jobs[jb].in0);
+    const cplx<T>* in1 = reinterpret_cast<const cplx<T>*>(a.jobs[jb].in1);
+    cplx<T>* out = reinterpret_cast<cplx<T>*>(a.jobs[jb].out);
+    const int op = a.jobs[jb].op;
+    cplx<T> v[PER0][R0];
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int e = tid + i * NT;
-      int aa, bb;
-      if (CONTIG_IN) { aa = e % L; bb = e / L; } else { bb = e % TL; aa = e / TL; }
-      cplx<T> v = mk<T>(T(0), T(0));
-      if (b0 + bb < a.nb) {
-        const long idx = c * a.sin.c + (long)(b0 + bb) * a.sin.b + (long)aa * a.sin.a;
-        v = in0[idx];
-        if (op != OP_COPY) {
-          v = ik_<T>((T)a.kline[aa], v);
-          if (op == OP_DERIV_MINUS) v = sub(v, in1[idx]);
+    for (int p = 0; p < PER0; ++p) {
+      const int beta = tid + p * NT;
+      if (NB0 % NT == 0 || beta < NB0) {
+        int bb, j;
+        if (CONTIG_IN) { bb = beta / (L / R0); j = beta - bb * (L / R0); }
+        else { bb = beta % TL; j = beta / TL; }
+        const bool ok = b0 + bb < a.nb;
+        const long base = c * a.sin.c + (long)(b0 + bb) * a.sin.b;
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+          const int aa = j + q * (L / R0);
+          cplx<T> x = mk<T>(T(0), T(0));
+          if (ok) {
+            const long idx = base + (long)aa * a.sin.a;
+            x = in0[idx];
+            if (op != OP_COPY) {
+              x = ik_<T>((T)a.kline[aa], x);
+              if (op == OP_DERIV_MINUS) x = sub(x, in1[idx]);
+            }
+          }
+          v[p][q] = x;
         }
+        Bfly<R0, S, T>::run(v[p]);
       }
-      buf[bb * LS + aa] = v;
+    }
+#pragma unroll
+    for (int p = 0; p < PER0; ++p) {
+      const int beta = tid + p * NT;
+      if (NB0 % NT == 0 || beta < NB0) {
+        int bb, j;
+        if (CONTIG_IN) { bb = beta / (L / R0); j = beta - bb * (L / R0); }
+        else { bb = beta % TL; j = beta / TL; }
+#pragma unroll
+        for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
+      }
     }
     __syncthreads();
-    block_fft<T, L, TL, NT, S>(buf, LS, tw);
+    Stages<T, L, TL, NT, PLAN_TILE, S, LSP, 1, NSTG - 1, R0>::run(buf, tw);
+    constexpr int NSL = L / RL;
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int e = tid + i * NT;
-      int aa, bb;
-      if (CONTIG_IN) { bb = e % TL; aa = e / TL; } else { aa = e % L; bb = e / L; }
-      if (b0 + bb < a.nb) {
-        const long idx = c * a.sout.c + (long)(b0 + bb) * a.sout.b + (long)aa * a.sout.a;
-        const cplx<T> v = buf[bb * LS + aa];
-        out[idx] = mk<T>(v.x * sc, v.y * sc);
+    for (int p = 0; p < PERL; ++p) {
+      const int beta = tid + p * NT;
+      if (NBL % NT == 0 || beta < NBL) {
+        int bb, j;
+        if (!CONTIG_IN) { bb = beta / (L / RL); j = beta - bb * (L / RL); }
+        else { bb = beta % TL; j = beta / TL; }
+        cplx<T> w[RL];
+#pragma unroll
+        for (int q = 0; q < RL; ++q) w[q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
+        twiddle_bfly<T, L, RL, S, NSL>(w, j, tw);
+        if (b0 + bb < a.nb) {
+          const long base = c * a.sout.c + (long)(b0 + bb) * a.sout.b;
+#pragma unroll
+          for (int q = 0; q < RL; ++q) out[base + (long)(j + q * (L / RL)) * a.sout.a] = mk<T>(w[q].x * sc, w[q].y * sc);
+        }
       }
     }
     __syncthreads();
@@ -106,7 +147,7 @@ template <typename T, int L, int S, bool CI>
 static void run_tile(const TilePassArgs& a, hipStream_t s) {
   constexpr int TL = Tile<T>::TL;
   constexpr int NT = pick_nt(TL * L);
-  const size_t smem = sizeof(cplx<T>) * TL * (L + 1);
+  const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
   static bool init = false;
   if (!init) { set_smem(tile_pass_k<T, L, TL, NT, S, CI>, smem); init = true; }
   dim3 grid((a.nb + TL - 1) / TL, a.nc);
@@ -119,9 +160,16 @@ static void run_tile_dir(const TilePassArgs& a, int dir, bool ci, hipStream_t s)
   else { if (ci) run_tile<T, L, -1, true>(a, s); else run_tile<T, L, -1, false>(a, s); }
 }
 
+bool x_length_supported(int m) {
+  switch (m) {
+    case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256: return true;
+    default: return false;
+  }
+}
+
 bool fft_length_supported(int n) {
   switch (n) {
-    case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128:
+    case 16: case 24: case 32: case 48: case 64: case 96: case 128:
     case 192: case 256: case 384: case 512: return true;
     default: return false;
   }
@@ -130,7 +178,6 @@ bool fft_length_supported(int n) {
 template <typename T>
 void launch_tile_pass(const TilePassArgs& a, int L, int dir, bool ci, hipStream_t s) {
   switch (L) {
-    case 8: run_tile_dir<T, 8>(a, dir, ci, s); break;
     case 16: run_tile_dir<T, 16>(a, dir, ci, s); break;
     case 24: run_tile_dir<T, 24>(a, dir, ci, s); break;
     case 32: run_tile_dir<T, 32>(a, dir, ci, s); break;
@@ -152,14 +199,28 @@ template void launch_tile_pass<double>(const TilePassArgs&, int, int, bool, hipS
 template <typename T> struct XTile { static constexpr int W = 1024; };
 template <> struct XTile<double> { static constexpr int W = 512; };
 __host__ __device__ constexpr int xtlx(int M, int W) { return (W / M) < 1 ? 1 : ((W / M) > 16 ? 16 : (W / M)); }
+// threads of an x-pass block: one per butterfly of the fused stage, 64..256
+__host__ __device__ constexpr int x_threads(int nb) { return nb >= 256 ? 256 : (nb <= 64 ? 64 : ((nb + 63) / 64) * 64); }
 
 // z[m] = x[2m] + i x[2m+1] is transformed with a half-length complex FFT;
 // the split/merge steps below follow from the even/odd decomposition of the
 // length-nx DFT.  The imaginary parts of the kx = 0 and kx = nx/2 entries are
 // ignored on input, as a C2R transform does.
+//
+// Structure of the RHS mode (per workgroup: TLX x-lines, all fields):
+//   A  spectral lines -> LDS
+//   B  form u, w spectra (w_y, w_z need i kx), pack, inverse stage 0 (regs)
+//   C  inverse middle stages (LDS)
+//   D  inverse last stage -> u x w - chi (u - us) [+ sponge] -> forward
+//      stage 0, all in registers (PLAN_XI ends with the radix PLAN_XF starts with)
+//   E  forward remaining stages (LDS), split into nx/2+1 entries, store
 template <typename T, int M, int TLX, int NT, int MODE, int NFI, int NFO, bool SPONGE>
 __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
-  constexpr int LSX = (M + 1) | 1;
+  constexpr int NST = plan_count_k(PLAN_XF, M);
+  static_assert(NST >= 2, "x pass needs at least two stages");
+  constexpr int RI0 = plan_radix_k(PLAN_XI, M, 0);   // first inverse radix
+  constexpr int RM = plan_radix_k(PLAN_XF, M, 0);    // last inverse = first forward radix
+  constexpr int LSP = line_stride<T>(M + 1);
   constexpr int NP = M / 2 + 1;
   constexpr int nxh = M + 1;
   extern __shared__ char smem_raw[];
@@ -169,8 +230,10 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
   const int l0 = blockIdx.x * TLX;
   const cplx<T>* twM = reinterpret_cast<const cplx<T>*>(a.twM);
   const cplx<T>* twN = reinterpret_cast<const cplx<T>*>(a.twN);
+  auto L_ = [&](int f, int line) { return buf + (f * TLX + line) * LSP; };
 
   if (MODE != XM_FWD) {
+    // ---- A
     const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in);
 #pragma unroll
     for (int f = 0; f < NFI; ++f)
@@ -179,156 +242,226 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
         const int gl = l0 + line;
         cplx<T> v = mk<T>(T(0), T(0));
         if (gl < a.nlines) v = in[f * a.fstride_in + (long)gl * nxh + k];
-        buf[(f * TLX + line) * LSX + k] = v;
+        L_(f, line)[lp<T>(k)] = v;
       }
     __syncthreads();
-    for (int e = tid; e < TLX * NP; e += NT) {
-      const int line = e / NP, k = e - line * NP, k2 = M - k;
-      cplx<T> xa[NFI], xb[NFI];
+    // ---- B: butterfly (f, line, j) of inverse stage 0 reads Z at j + q M/RI0
+    constexpr int NB = NFI * TLX * (M / RI0), PER = (NB + NT - 1) / NT;
+    cplx<T> v[PER][RI0];
 #pragma unroll
-      for (int f = 0; f < NFI; ++f) {
-        xa[f] = buf[(f * TLX + line) * LSX + k];
-        xb[f] = buf[(f * TLX + line) * LSX + k2];
-      }
-      if (MODE == XM_RHS || MODE == XM_INV_UW) {
-        // fields: 0 Vx, 1 Vy, 2 Vz, 3 Wx, 4 A = dz u_x, 5 B = dy u_x
-        const T ka = (T)a.kx[k], kb = (T)a.kx[k2];
-        const cplx<T> wya = sub(xa[4], ik_<T>(ka, xa[2])), wza = sub(ik_<T>(ka, xa[1]), xa[5]);
-        const cplx<T> wyb = sub(xb[4], ik_<T>(kb, xb[2])), wzb = sub(ik_<T>(kb, xb[1]), xb[5]);
-        xa[4] = wya; xa[5] = wza; xb[4] = wyb; xb[5] = wzb;
-      }
-      if (k == 0) {
+    for (int p = 0; p < PER; ++p) {
+      const int beta = tid + p * NT;
+      if (NB % NT == 0 || beta < NB) {
+        const int f = beta / (TLX * (M / RI0));
+        const int rem = beta - f * (TLX * (M / RI0));
+        const int line = rem / (M / RI0), j = rem - line * (M / RI0);
 #pragma unroll
-        for (int f = 0; f < NFI; ++f) { xa[f].y = T(0); xb[f].y = T(0); }
-      }
-      const cplx<T> ea = conj_(twN[k]), eb = conj_(twN[k2]);
-#pragma unroll
-      for (int f = 0; f < NFI; ++f) {
-        const cplx<T> s1 = add(xa[f], conj_(xb[f])), d1 = sub(xa[f], conj_(xb[f]));
-        const cplx<T> za = add(s1, ik_<T>(T(1), cmul(ea, d1)));
-        buf[(f * TLX + line) * LSX + k] = za;
-        if (k2 < M && k2 != k) {
-          const cplx<T> s2 = add(xb[f], conj_(xa[f])), d2 = sub(xb[f], conj_(xa[f]));
-          buf[(f * TLX + line) * LSX + k2] = add(s2, ik_<T>(T(1), cmul(eb, d2)));
+        for (int q = 0; q < RI0; ++q) {
+          const int k = j + q * (M / RI0), k2 = M - k;
+          cplx<T> xa, xb;
+          if ((MODE == XM_RHS || MODE == XM_INV_UW) && f == 4) {
+            // w_y = dz u_x - dx u_z = A - i kx V_z
+            xa = sub(L_(4, line)[lp<T>(k)], ik_<T>((T)a.kx[k], L_(2, line)[lp<T>(k)]));
+            xb = sub(L_(4, line)[lp<T>(k2)], ik_<T>((T)a.kx[k2], L_(2, line)[lp<T>(k2)]));
+          } else if ((MODE == XM_RHS || MODE == XM_INV_UW) && f == 5) {
+            // w_z = dx u_y - dy u_x = i kx V_y - B
+            xa = sub(ik_<T>((T)a.kx[k], L_(1, line)[lp<T>(k)]), L_(5, line)[lp<T>(k)]);
+            xb = sub(ik_<T>((T)a.kx[k2], L_(1, line)[lp<T>(k2)]), L_(5, line)[lp<T>(k2)]);
+          } else {
+            xa = L_(f, line)[lp<T>(k)];
+            xb = L_(f, line)[lp<T>(k2)];
+          }
+          if (k == 0) { xa.y = T(0); xb.y = T(0); }
+          const cplx<T> ea = conj_(twN[k]);
+          const cplx<T> s1 = add(xa, conj_(xb)), d1 = sub(xa, conj_(xb));
+          v[p][q] = add(s1, ik_<T>(T(1), cmul(ea, d1)));
         }
+        Bfly<RI0, +1, T>::run(v[p]);
       }
     }
     __syncthreads();
-    block_fft<T, M, NFI * TLX, NT, +1>(buf, LSX, twM);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int beta = tid + p * NT;
+      if (NB % NT == 0 || beta < NB) {
+        const int f = beta / (TLX * (M / RI0));
+        const int rem = beta - f * (TLX * (M / RI0));
+        const int line = rem / (M / RI0), j = rem - line * (M / RI0);
+#pragma unroll
+        for (int q = 0; q < RI0; ++q) L_(f, line)[lp<T>(j * RI0 + q)] = v[p][q];
+      }
+    }
+    __syncthreads();
+    // ---- C
+    Stages<T, M, NFI * TLX, NT, PLAN_XI, +1, LSP, 1, NST - 1, RI0>::run(buf, twM);
   }
 
+  constexpr int NSD = M / RM;  // sub-transform length entering the last inverse stage
+
   if (MODE == XM_INV_UW || MODE == XM_INV) {
+    // inverse last stage straight to the physical field
     cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out);
+    constexpr int NB = NFI * TLX * (M / RM), PER = (NB + NT - 1) / NT;
 #pragma unroll
-    for (int f = 0; f < NFI; ++f)
-      for (int e = tid; e < TLX * M; e += NT) {
-        const int line = e / M, m = e - line * M;
+    for (int p = 0; p < PER; ++p) {
+      const int beta = tid + p * NT;
+      if (NB % NT == 0 || beta < NB) {
+        const int f = beta / (TLX * (M / RM));
+        const int rem = beta - f * (TLX * (M / RM));
+        const int line = rem / (M / RM), j = rem - line * (M / RM);
+        cplx<T> w[RM];
+#pragma unroll
+        for (int q = 0; q < RM; ++q) w[q] = L_(f, line)[lp<T>(j + q * (M / RM))];
+        twiddle_bfly<T, M, RM, +1, NSD>(w, j, twM);
         const int gl = l0 + line;
-        if (gl < a.nlines)
-          out[(f * a.fstride_out + (long)gl * a.nx) / 2 + m] = buf[(f * TLX + line) * LSX + m];
+        if (gl < a.nlines) {
+#pragma unroll
+          for (int q = 0; q < RM; ++q) out[(f * a.fstride_out + (long)gl * a.nx) / 2 + j + q * (M / RM)] = w[q];
+        }
       }
+    }
     return;
   }
 
   if (MODE == XM_FWD) {
+    // physical pairs straight into forward stage 0
     const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in);
+    constexpr int NB = NFI * TLX * (M / RM), PER = (NB + NT - 1) / NT;
+    cplx<T> v[PER][RM];
 #pragma unroll
-    for (int f = 0; f < NFI; ++f)
-      for (int e = tid; e < TLX * M; e += NT) {
-        const int line = e / M, m = e - line * M;
+    for (int p = 0; p < PER; ++p) {
+      const int beta = tid + p * NT;
+      if (NB % NT == 0 || beta < NB) {
+        const int f = beta / (TLX * (M / RM));
+        const int rem = beta - f * (TLX * (M / RM));
+        const int line = rem / (M / RM), j = rem - line * (M / RM);
         const int gl = l0 + line;
-        cplx<T> v = mk<T>(T(0), T(0));
-        if (gl < a.nlines) v = in[(f * a.fstride_in + (long)gl * a.nx) / 2 + m];
-        buf[(f * TLX + line) * LSX + m] = v;
+#pragma unroll
+        for (int q = 0; q < RM; ++q)
+          v[p][q] = (gl < a.nlines) ? in[(f * a.fstride_in + (long)gl * a.nx) / 2 + j + q * (M / RM)] : mk<T>(T(0), T(0));
+        Bfly<RM, -1, T>::run(v[p]);
+#pragma unroll
+        for (int q = 0; q < RM; ++q) L_(f, line)[lp<T>(j * RM + q)] = v[p][q];
       }
+    }
     __syncthreads();
   }
 
   if (MODE == XM_RHS) {
+    // ---- D
     const T* mask = reinterpret_cast<const T*>(a.mask);
     const T* us = reinterpret_cast<const T*>(a.us);
     double umax2 = 0.0, fs[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
-    for (int e = tid; e < TLX * M; e += NT) {
-      const int line = e / M, m = e - line * M;
-      const int gl = l0 + line;
-      if (gl >= a.nlines) continue;
-      cplx<T> z[6];
+    constexpr int NB = TLX * (M / RM), PER = (NB + NT - 1) / NT;
+    cplx<T> nout[PER][NFO][RM];
 #pragma unroll
-      for (int f = 0; f < 6; ++f) z[f] = buf[(f * TLX + line) * LSX + m];
-      const long p0 = (long)gl * a.nx + 2 * m;
-      const int iy = gl % a.ny, iz = gl / a.ny;
-      double nout[3][2], sout[3][2];
+    for (int p = 0; p < PER; ++p) {
+      const int beta = tid + p * NT;
+      if (NB % NT == 0 || beta < NB) {
+        const int line = beta / (M / RM), j = beta - line * (M / RM);
+        const int gl = l0 + line;
+        cplx<T> z[6][RM];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const double ux = h ? z[0].y : z[0].x, uy = h ? z[1].y : z[1].x, uz = h ? z[2].y : z[2].x;
-        const double wx = h ? z[3].y : z[3].x, wy = h ? z[4].y : z[4].x, wz = h ? z[5].y : z[5].x;
-        const double mk_ = (double)mask[p0 + h];
-        double dux = ux, duy = uy, duz = uz;
-        if (mk_ != 0.0) {
-          dux -= (double)us[p0 + h];
-          duy -= (double)us[a.npts + p0 + h];
-          duz -= (double)us[2 * a.npts + p0 + h];
+        for (int f = 0; f < 6; ++f) {
+#pragma unroll
+          for (int q = 0; q < RM; ++q) z[f][q] = L_(f, line)[lp<T>(j + q * (M / RM))];
+          twiddle_bfly<T, M, RM, +1, NSD>(z[f], j, twM);
         }
-        const double chi = mk_ * a.eps_inv;
-        const double px = chi * dux, py = chi * duy, pz = chi * duz;
-        // Neumaier-compensated running sums of the penalisation term
-        const double pv[3] = {px, py, pz};
+        const bool live = gl < a.nlines;
+        const int iy = live ? gl % a.ny : 0, iz = live ? gl / a.ny : 0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double t = fs[c] + pv[c];
-          fc[c] += (fabs(fs[c]) >= fabs(pv[c])) ? (fs[c] - t) + pv[c] : (pv[c] - t) + fs[c];
-          fs[c] = t;
-        }
-        umax2 = fmax(umax2, ux * ux + uy * uy + uz * uz);
-        nout[0][h] = uy * wz - uz * wy - px;
-        nout[1][h] = uz * wx - ux * wz - py;
-        nout[2][h] = ux * wy - uy * wx - pz;
-        if (SPONGE) {
-          const int ix = 2 * m + h;
-          bool in_sp = false;
-          const int th = a.sponge_thickness;
-          if (a.sponge == 1) {
-            if (a.nx > 4 && (ix <= th - 1 || ix >= a.nx - th)) in_sp = true;
-            if (iy <= th - 1 || iy >= a.ny - th) in_sp = true;
-            if (iz <= th - 1 || iz >= a.nz - th) in_sp = true;
-          } else if (a.sponge == 2) {
-            in_sp = ix >= a.nx - th;
+        for (int q = 0; q < RM; ++q) {
+          const int pos = j + q * (M / RM);
+          double nv[3][2], sv[3][2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const double ux = h ? z[0][q].y : z[0][q].x, uy = h ? z[1][q].y : z[1][q].x, uz = h ? z[2][q].y : z[2][q].x;
+            const double wx = h ? z[3][q].y : z[3][q].x, wy = h ? z[4][q].y : z[4][q].x, wz = h ? z[5][q].y : z[5][q].x;
+            const long pt = (long)gl * a.nx + 2 * pos + h;
+            const double mk_ = live ? (double)mask[pt] : 0.0;
+            double dux = ux, duy = uy, duz = uz;
+            if (mk_ != 0.0) {
+              dux -= (double)us[pt];
+              duy -= (double)us[a.npts + pt];
+              duz -= (double)us[2 * a.npts + pt];
+            }
+            const double chi = mk_ * a.eps_inv;
+            const double pv[3] = {chi * dux, chi * duy, chi * duz};
+            if (live) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                const double t = fs[c] + pv[c];
+                fc[c] += (fabs(fs[c]) >= fabs(pv[c])) ? (fs[c] - t) + pv[c] : (pv[c] - t) + fs[c];
+                fs[c] = t;
+              }
+              umax2 = fmax(umax2, ux * ux + uy * uy + uz * uz);
+            }
+            nv[0][h] = uy * wz - uz * wy - pv[0];
+            nv[1][h] = uz * wx - ux * wz - pv[1];
+            nv[2][h] = ux * wy - uy * wx - pv[2];
+            if constexpr (SPONGE) {
+              const int ix = 2 * pos + h, th = a.sponge_thickness;
+              bool in_sp = false;
+              if (a.sponge == 1) {
+                if (a.nx > 4 && (ix <= th - 1 || ix >= a.nx - th)) in_sp = true;
+                if (iy <= th - 1 || iy >= a.ny - th) in_sp = true;
+                if (iz <= th - 1 || iz >= a.nz - th) in_sp = true;
+              } else if (a.sponge == 2) {
+                in_sp = ix >= a.nx - th;
+              }
+              in_sp = in_sp && (mk_ < a.sponge_mask_thr);
+              sv[0][h] = in_sp ? -wx * a.sponge_eps_inv : 0.0;
+              sv[1][h] = in_sp ? -wy * a.sponge_eps_inv : 0.0;
+              sv[2][h] = in_sp ? -wz * a.sponge_eps_inv : 0.0;
+            }
           }
-          in_sp = in_sp && (mk_ < a.sponge_mask_thr);
-          sout[0][h] = in_sp ? -wx * a.sponge_eps_inv : 0.0;
-          sout[1][h] = in_sp ? -wy * a.sponge_eps_inv : 0.0;
-          sout[2][h] = in_sp ? -wz * a.sponge_eps_inv : 0.0;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            nout[p][c][q] = mk<T>((T)nv[c][0], (T)nv[c][1]);
+            if constexpr (SPONGE) nout[p][3 + c][q] = mk<T>((T)sv[c][0], (T)sv[c][1]);
+          }
         }
+#pragma unroll
+        for (int c = 0; c < NFO; ++c) Bfly<RM, -1, T>::run(nout[p][c]);
       }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        buf[(c * TLX + line) * LSX + m] = mk<T>((T)nout[c][0], (T)nout[c][1]);
-        if (SPONGE) buf[((3 + c) * TLX + line) * LSX + m] = mk<T>((T)sout[c][0], (T)sout[c][1]);
-      }
-    }
-    // block reduction of the per-thread statistics
-    double v4[4] = {umax2, fs[0] + fc[0], fs[1] + fc[1], fs[2] + fc[2]};
-    v4[0] = wave_max(v4[0]);
-#pragma unroll
-    for (int c = 1; c < 4; ++c) v4[c] = wave_sum(v4[c]);
-    const int wv = tid >> 6;
-    if ((tid & 63) == 0) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) red[wv][c] = v4[c];
     }
     __syncthreads();
-    if (tid == 0) {
-      double r[4] = {red[0][0], red[0][1], red[0][2], red[0][3]};
-      for (int w = 1; w < NT / 64; ++w) {
-        r[0] = fmax(r[0], red[w][0]);
-        for (int c = 1; c < 4; ++c) r[c] += red[w][c];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int beta = tid + p * NT;
+      if (NB % NT == 0 || beta < NB) {
+        const int line = beta / (M / RM), j = beta - line * (M / RM);
+#pragma unroll
+        for (int c = 0; c < NFO; ++c)
+#pragma unroll
+          for (int q = 0; q < RM; ++q) L_(c, line)[lp<T>(j * RM + q)] = nout[p][c][q];
       }
-      for (int c = 0; c < 4; ++c) a.partial[(long)blockIdx.x * 4 + c] = r[c];
+    }
+    if (a.want_stats) {
+      double v4[4] = {umax2, fs[0] + fc[0], fs[1] + fc[1], fs[2] + fc[2]};
+      v4[0] = wave_max(v4[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) v4[c] = wave_sum(v4[c]);
+      if ((tid & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red[tid >> 6][c] = v4[c];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double r[4] = {red[0][0], red[0][1], red[0][2], red[0][3]};
+        for (int w = 1; w < NT / 64; ++w) {
+          r[0] = fmax(r[0], red[w][0]);
+          for (int c = 1; c < 4; ++c) r[c] += red[w][c];
+        }
+        for (int c = 0; c < 4; ++c) a.partial[(long)blockIdx.x * 4 + c] = r[c];
+      }
+    } else {
+      __syncthreads();
     }
   }
 
-  // forward half-length FFT and split into the nx/2+1 spectral entries
-  block_fft<T, M, NFO * TLX, NT, -1>(buf, LSX, twM);
+  // ---- E: remaining forward stages, then split into the nx/2+1 entries
+  Stages<T, M, NFO * TLX, NT, PLAN_XF, -1, LSP, 1, NST, RM>::run(buf, twM);
   cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out);
   const T half = T(0.5);
   for (int e = tid; e < TLX * NP; e += NT) {
@@ -338,12 +471,11 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
     const cplx<T> wa = twN[k], wb = twN[k2];
 #pragma unroll
     for (int f = 0; f < NFO; ++f) {
-      const cplx<T> zk = buf[(f * TLX + line) * LSX + k];
-      const cplx<T> zm = buf[(f * TLX + line) * LSX + (k2 == M ? 0 : k2)];
+      const cplx<T> zk = L_(f, line)[lp<T>(k)];
+      const cplx<T> zm = L_(f, line)[lp<T>(k2 == M ? 0 : k2)];
       const cplx<T> s1 = add(zk, conj_(zm)), d1 = sub(zk, conj_(zm));
-      const cplx<T> xk = scl(sub(s1, ik_<T>(T(1), cmul(wa, d1))), half);
       cplx<T>* o = out + f * a.fstride_out + (long)gl * nxh;
-      o[k] = xk;
+      o[k] = scl(sub(s1, ik_<T>(T(1), cmul(wa, d1))), half);
       if (k2 != k) {
         const cplx<T> s2 = add(zm, conj_(zk)), d2 = sub(zm, conj_(zk));
         o[k2] = scl(sub(s2, ik_<T>(T(1), cmul(wb, d2))), half);
@@ -361,9 +493,9 @@ int x_pass_blocks(int nx, int nlines, bool fp64) {
 template <typename T, int M, int MODE, int NFI, int NFO, bool SP>
 static void run_x(const XPassArgs& a, hipStream_t s) {
   constexpr int TLX = xtlx(M, XTile<T>::W);
-  constexpr int NT = 256;
+  constexpr int NT = x_threads(TLX * (M / plan_radix_k(PLAN_XF, M, 0)));
   constexpr int NF = NFI > NFO ? NFI : NFO;
-  const size_t smem = sizeof(cplx<T>) * (size_t)NF * TLX * ((M + 1) | 1);
+  const size_t smem = sizeof(cplx<T>) * (size_t)NF * TLX * line_stride<T>(M + 1);
   static bool init = false;
   if (!init) { set_smem(x_pass_k<T, M, TLX, NT, MODE, NFI, NFO, SP>, smem); init = true; }
   const int blocks = (a.nlines + TLX - 1) / TLX;
@@ -419,8 +551,13 @@ __device__ __forceinline__ double vis_factor(double dtv, double nu, double k2, b
 
 template <typename T, int L, int TL, int NT, bool SPONGE>
 __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
-  constexpr int LS = L + 1;
-  constexpr int EPT = TL * L / NT;
+  constexpr int NSTG = plan_count_k(PLAN_TILE, L);
+  constexpr int R0 = plan_radix_k(PLAN_TILE, L, 0);
+  constexpr int RL = plan_radix_k(PLAN_TILE, L, NSTG - 1);
+  constexpr int NSL = L / RL;
+  constexpr int LSP = line_stride<T>(L);
+  constexpr int NB0 = TL * (L / R0), PER0 = (NB0 + NT - 1) / NT;
+  constexpr int NBL = TL * (L / RL), PERL = (NBL + NT - 1) / NT;
   constexpr int NF = SPONGE ? 6 : 3;
   extern __shared__ char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -430,24 +567,36 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
   const int kx = blockIdx.y;
   const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw);
   const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
-  cplx<T> r[NF][EPT];
+  // r[f][p][q]: field f at (ky = b0 + bb, kz = j + q L/RL) of last-stage butterfly p
+  cplx<T> r[NF][PERL][RL];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
+    cplx<T> v[PER0][R0];
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int e = tid + i * NT;
-      const int bb = e % TL, aa = e / TL;
-      cplx<T> v = mk<T>(T(0), T(0));
-      if (b0 + bb < a.ny) v = nq[f * a.fs + ((long)kx * a.nz + aa) * a.ny + b0 + bb];
-      buf[bb * LS + aa] = v;
+    for (int p = 0; p < PER0; ++p) {
+      const int beta = tid + p * NT;
+      if (NB0 % NT == 0 || beta < NB0) {
+        const int bb = beta % TL, j = beta / TL;
+        const bool ok = b0 + bb < a.ny;
+#pragma unroll
+        for (int q = 0; q < R0; ++q)
+          v[p][q] = ok ? nq[f * a.fs + ((long)kx * a.nz + j + q * (L / R0)) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
+        Bfly<R0, -1, T>::run(v[p]);
+#pragma unroll
+        for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
+      }
     }
     __syncthreads();
-    block_fft<T, L, TL, NT, -1>(buf, LS, tw);
+    Stages<T, L, TL, NT, PLAN_TILE, -1, LSP, 1, NSTG - 1, R0>::run(buf, tw);
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int e = tid + i * NT;
-      const int aa = e % L, bb = e / L;
-      r[f][i] = buf[bb * LS + aa];
+    for (int p = 0; p < PERL; ++p) {
+      const int beta = tid + p * NT;
+      if (NBL % NT == 0 || beta < NBL) {
+        const int bb = beta / (L / RL), j = beta - bb * (L / RL);
+#pragma unroll
+        for (int q = 0; q < RL; ++q) r[f][p][q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
+        twiddle_bfly<T, L, RL, -1, NSL>(r[f][p], j, tw);
+      }
     }
     __syncthreads();
   }
@@ -460,14 +609,18 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
   cplx<T>* UST = reinterpret_cast<cplx<T>*>(a.ustage);
   const cplx<T>* UD = reinterpret_cast<const cplx<T>*>(a.udiff);
   cplx<T>* ACC = reinterpret_cast<cplx<T>*>(a.acc);
-  cplx<T>* R0 = reinterpret_cast<cplx<T>*>(a.r0);
-  const cplx<T>* R1 = reinterpret_cast<const cplx<T>*>(a.r1);
+  cplx<T>* RG0 = reinterpret_cast<cplx<T>*>(a.r0);
+  const cplx<T>* RG1 = reinterpret_cast<const cplx<T>*>(a.r1);
   cplx<T>* RO = reinterpret_cast<cplx<T>*>(a.rhs_out);
   const bool final_stage = a.mode == SM_EULER || a.mode == SM_AB2 || a.mode == SM_RK2_2 || a.mode == SM_RK4_4;
 #pragma unroll
-  for (int i = 0; i < EPT; ++i) {
-    const int e = tid + i * NT;
-    const int kzi = e % L, bb = e / L;
+  for (int p = 0; p < PERL; ++p)
+#pragma unroll
+  for (int q = 0; q < RL; ++q) {
+    const int beta = tid + p * NT;
+    if (!(NBL % NT == 0 || beta < NBL)) continue;
+    const int bb = beta / (L / RL), j = beta - bb * (L / RL);
+    const int kzi = j + q * (L / RL);
     const int ky = b0 + bb;
     if (ky >= a.ny) continue;
     const long si = ((long)kx * a.ny + ky) * a.nz + kzi;
@@ -476,13 +629,13 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
     const bool alias = ((kxt2 + a.kyt2[ky]) + a.kzt2[kzi]) >= 1.0;
     double nr[3], ni[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { nr[c] = (double)r[c][i].x * a.scale; ni[c] = (double)r[c][i].y * a.scale; }
-    if (SPONGE) {
+    for (int c = 0; c < 3; ++c) { nr[c] = (double)r[c][p][q].x * a.scale; ni[c] = (double)r[c][p][q].y * a.scale; }
+    if constexpr (SPONGE) {
       // sponge.f90:22-83: i k x (w/k^2), zero at k = 0
       if (k2 != 0.0) {
         double sr[3], sm[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) { sr[c] = (double)r[3 + c][i].x * a.scale / k2; sm[c] = (double)r[3 + c][i].y * a.scale / k2; }
+        for (int c = 0; c < 3; ++c) { sr[c] = (double)r[3 + c][p][q].x * a.scale / k2; sm[c] = (double)r[3 + c][p][q].y * a.scale / k2; }
         const double cr0 = kyv * sr[2] - kzv * sr[1], ci0 = kyv * sm[2] - kzv * sm[1];
         const double cr1 = kzv * sr[0] - kxv * sr[2], ci1 = kzv * sm[0] - kxv * sm[2];
         const double cr2 = kxv * sr[1] - kyv * sr[0], ci2 = kxv * sm[1] - kyv * sm[0];
@@ -528,7 +681,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
           unr[c] = ((double)u0.x + a.dt * nr[c]) * E;
           uni[c] = ((double)u0.y + a.dt * ni[c]) * E;
           U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-          R0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
+          RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
         }
         break;
       }
@@ -538,11 +691,11 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const cplx<T> u0 = U[c * a.fs + si];
-          const cplx<T> n1 = R1[c * a.fs + si];
+          const cplx<T> n1 = RG1[c * a.fs + si];
           unr[c] = ((double)u0.x + a.b10 * nr[c] + a.b11 * (double)n1.x) * E;
           uni[c] = ((double)u0.y + a.b10 * ni[c] + a.b11 * (double)n1.y) * E;
           U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-          R0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
+          RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
         }
         break;
       }
@@ -554,7 +707,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
         for (int c = 0; c < 3; ++c) {
           const double n0r = nr[c] * Eo, n0i = ni[c] * Eo;
           const cplx<T> u0 = U[c * a.fs + si];
-          R0[c * a.fs + si] = mk<T>((T)n0r, (T)n0i);
+          RG0[c * a.fs + si] = mk<T>((T)n0r, (T)n0i);
           U[c * a.fs + si] = mk<T>((T)((double)u0.x * En + a.dt * n0r), (T)((double)u0.y * En + a.dt * n0i));
         }
         break;
@@ -564,7 +717,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const cplx<T> u0 = U[c * a.fs + si];
-          const cplx<T> n0 = R0[c * a.fs + si];
+          const cplx<T> n0 = RG0[c * a.fs + si];
           unr[c] = (double)u0.x + 0.5 * a.dt * (-(double)n0.x + nr[c]);
           uni[c] = (double)u0.y + 0.5 * a.dt * (-(double)n0.y + ni[c]);
         }
@@ -640,7 +793,7 @@ template <typename T, int L, bool SP>
 static void run_z(const ZStepArgs& a, hipStream_t s) {
   constexpr int TL = Tile<T>::TL;
   constexpr int NT = pick_nt(TL * L);
-  const size_t smem = sizeof(cplx<T>) * TL * (L + 1);
+  const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
   static bool init = false;
   if (!init) { set_smem(zstep_k<T, L, TL, NT, SP>, smem); init = true; }
   dim3 grid((a.ny + TL - 1) / TL, a.nxh);
@@ -654,7 +807,6 @@ static void run_z_sp(const ZStepArgs& a, hipStream_t s) {
 template <typename T>
 void launch_zstep(const ZStepArgs& a, hipStream_t s) {
   switch (a.nz) {
-    case 8: run_z_sp<T, 8>(a, s); break;
     case 16: run_z_sp<T, 16>(a, s); break;
     case 24: run_z_sp<T, 24>(a, s); break;
     case 32: run_z_sp<T, 32>(a, s); break;

@@ -139,6 +139,7 @@ class Solver : public SolverBase {
   double* zpart = nullptr; int nzblocks = 0;
   double* dpart = nullptr; int ndblocks = 0;
   double* red = nullptr;    // small results
+  double* rmid = nullptr;   // reduction scratch
   int* kinds = nullptr;     // reduction kinds: [0..3] x pass, [4] energy, [8..8+DIAG_NV) diag
   double* stage = nullptr; size_t stage_n = 0;
   std::vector<double> kx_h, ky_h, kz_h, kxt2_h, kyt2_h, kzt2_h;
@@ -151,7 +152,7 @@ class Solver : public SolverBase {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto& e : ev) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
     void* ptrs[] = {uk, reg[0], reg[1], Q, P, mask, us, color, phys, tab, tw_z, tw_y, tw_M, tw_N,
-                    xpart, zpart, dpart, red, kinds, stage};
+                    xpart, zpart, dpart, red, rmid, kinds, stage};
     for (void* q : ptrs) if (q) (void)hipFree(q);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -166,14 +167,13 @@ class Solver : public SolverBase {
   void need_fft() const {
     if (!fft_ok)
       throw FlusiError(900003, "grid " + std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz) +
-                                   " has no FFT plan (nx/2, ny, nz must each be 2^a or 3*2^a between 8 and 512, "
-                                   "nx <= 512); only mask generation is available");
+                                   " has no FFT plan (nx/2 in 8..256 and ny, nz in 16..512, each 2^a or 3*2^a); "
+                                   "only mask generation is available");
   }
 
   void setup() {
     nx = p.nx; ny = p.ny; nz = p.nz; nxh = nx / 2 + 1;
-    fft_ok = nx % 2 == 0 && nx / 2 >= 8 && nx / 2 <= 256 && fft_length_supported(nx / 2) &&
-             fft_length_supported(ny) && fft_length_supported(nz);
+    fft_ok = nx % 2 == 0 && x_length_supported(nx / 2) && fft_length_supported(ny) && fft_length_supported(nz);
     npts = (long)nx * ny * nz;
     nc = (long)nxh * ny * nz;
     geom = GridGeom{nx, ny, nz, nxh, npts, nc, p.dx(), p.dy(), p.dz(), p.xl, p.yl, p.zl, p.periodic ? 1 : 0};
@@ -227,6 +227,7 @@ class Solver : public SolverBase {
       dpart = (double*)dalloc(sizeof(double) * DIAG_NV * ndblocks);
     }
     red = (double*)dalloc(sizeof(double) * 128);
+    rmid = (double*)dalloc(sizeof(double) * reduce_rows_scratch(DIAG_NV));
     std::vector<int> k(8 + DIAG_NV, 0);
     k[0] = 1;  // umax^2: max
     for (int v = 0; v < DIAG_NV; ++v) {
@@ -359,9 +360,11 @@ class Solver : public SolverBase {
     a.sponge_eps_inv = a.sponge ? 1.0 / p.eps_sponge : 0.0;
     a.sponge_mask_thr = (a.sponge == 2) ? (double)1.0e-12f : 1.0e-12;
     a.partial = xpart;
+    a.want_stats = (p.dt_fixed <= 0.0 || p.iMeanFlow[0] == "dynamic" || p.iMeanFlow[1] == "dynamic" ||
+                    p.iMeanFlow[2] == "dynamic") ? 1 : 0;
     timed(TM_X, [&] { launch_x_pass<T>(a, XM_RHS, stream); });
     kcheck("x pass");
-    launch_reduce_rows(xpart, nxblocks, 4, kinds, red, stream);
+    if (a.want_stats) launch_reduce_rows(xpart, nxblocks, 4, kinds, rmid, red, stream);
     kcheck("reduce x");
   }
 
@@ -436,7 +439,7 @@ class Solver : public SolverBase {
 
   void record_ekin() {
     if (out_dir.empty()) return;
-    launch_reduce_rows(zpart, nzblocks, 1, kinds + 4, red + 8, stream);
+    launch_reduce_rows(zpart, nzblocks, 1, kinds + 4, rmid, red + 8, stream);
     double e = 0.0;
     HIPCHECK(hipMemcpyAsync(&e, red + 8, sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
@@ -607,6 +610,32 @@ class Solver : public SolverBase {
     a.color_w[0] = 3; a.color_w[1] = 2;
     const Vec3 vb = mat_vec(s.Mb, s.vc);
     for (int i = 0; i < 3; ++i) { a.vb[i] = vb[i]; a.rot_body_b[i] = s.rot_body_b[i]; }
+    // index boxes enclosing every point a part's geometric tests can accept
+    // (corners of its local bounding box mapped to the global frame, one cell
+    // of margin); unused with periodic insects, whose parts may wrap
+    a.use_box = p.periodic ? 0 : 1;
+    const Mat3 MbT = transpose(s.Mb);
+    auto set_box = [&](int k, const Vec3& lo, const Vec3& hi, const Mat3* Mw, const Vec3& piv) {
+      double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+      for (int c = 0; c < 8; ++c) {
+        Vec3 v{(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+        if (Mw) { v = mat_vec(transpose(*Mw), v); for (int i = 0; i < 3; ++i) v[i] += piv[i]; }
+        v = mat_vec(MbT, v);
+        for (int i = 0; i < 3; ++i) { mn[i] = std::min(mn[i], v[i] + s.xc[i]); mx[i] = std::max(mx[i], v[i] + s.xc[i]); }
+      }
+      const double d[3] = {p.dx(), p.dy(), p.dz()};
+      const int nn[3] = {nx, ny, nz};
+      for (int i = 0; i < 3; ++i) {
+        a.box_lo[k][i] = std::max(0, (int)std::floor(mn[i] / d[i]) - 1);
+        a.box_hi[k][i] = std::min(nn[i] - 1, (int)std::ceil(mx[i] / d[i]) + 1);
+      }
+    };
+    const double sf = I.safety;
+    set_box(0, {-I.L_body / 2 - sf, -I.b_body - sf, -I.b_body - sf}, {I.L_body / 2 + sf, I.b_body + sf, I.b_body + sf},
+            nullptr, {0, 0, 0});
+    const Vec3 wlo{-I.b_bot - sf, -sf, -0.5 * I.WingThickness - sf}, whi{I.b_top + sf, I.L_span + sf, 0.5 * I.WingThickness + sf};
+    set_box(1, wlo, whi, &s.Mwr, I.x_pivot_r_b);
+    set_box(2, wlo, whi, &s.Mwl, I.x_pivot_l_b);
     return a;
   }
 
@@ -815,7 +844,7 @@ class Solver : public SolverBase {
     d.partial = dpart;
     timed(TM_DIAG, [&] { launch_diag<T>(geom, d, stream); });
     kcheck("diag");
-    launch_reduce_rows(dpart, ndblocks, DIAG_NV, kinds + 8, red + 16, stream);
+    launch_reduce_rows(dpart, ndblocks, DIAG_NV, kinds + 8, rmid, red + 16, stream);
     kcheck("reduce diag");
     std::vector<double> r(DIAG_NV);
     HIPCHECK(hipMemcpyAsync(r.data(), red + 16, sizeof(double) * DIAG_NV, hipMemcpyDeviceToHost, stream));
