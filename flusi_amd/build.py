@@ -52,7 +52,32 @@ def build(force: bool = True, jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+FLANG = os.environ.get("FLUSI_FC", "/opt/rocm/bin/amdflang")
+FORTRAN_BIN = os.path.join(HERE, "bin", "flusi_amd")
+
+
+def build_fortran() -> str:
+    """The iso_c_binding module and the Fortran host driver (needs amdflang);
+    returns the driver path, or "" when no Fortran compiler is present."""
+    if not os.path.exists(FLANG):
+        return ""
+    fdir = os.path.join(HERE, "fortran")
+    obj = os.path.join(BUILD, "flusi_hip_f.o")
+    cmds = [
+        [FLANG, "-O2", "-J", BUILD, "-c", os.path.join(fdir, "flusi_hip.f90"), "-o", obj],
+        [FLANG, "-O2", "-J", BUILD, os.path.join(fdir, "flusi_amd_main.f90"), obj, "-L", HERE, "-lflusi_hip",
+         "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_BIN],
+    ]
+    os.makedirs(os.path.dirname(FORTRAN_BIN), exist_ok=True)
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return FORTRAN_BIN
 
 
 if __name__ == "__main__":

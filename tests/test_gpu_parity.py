@@ -251,3 +251,26 @@ def test_mixed_precision_force_error_sphere(tmp_path):
     rel = abs(f32[0] - f64[0]) / abs(f64[0])
     print("sphere drag rel err FP32-mixed vs FP64:", rel)
     assert rel < 1e-4
+
+
+def test_fortran_host_runs_vortex_ring_fixture(tmp_path):
+    """The Fortran host (iso_c_binding module + driver) runs the
+    vortex_ring2_RK2 fixture to T=0.5; its dt.t / energy.t match the
+    reference's with compare_timeseries.f90's rule."""
+    import shutil
+    import subprocess
+    from flusi_amd import build as b
+    if not os.path.exists(b.FORTRAN_BIN):
+        pytest.skip("Fortran driver not built (no amdflang)")
+    d = os.path.join(GOLDEN, "vortex_ring2_RK2")
+    shutil.copy(os.path.join(d, "vortexring2_RK2.ini"), tmp_path)
+    r = subprocess.run([b.FORTRAN_BIN, "vortexring2_RK2.ini"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for ts in ("dt.t", "energy.t"):
+        ref = fo.read_timeseries(os.path.join(d, ts))
+        got = fo.read_timeseries(str(tmp_path / ts))
+        n = min(len(ref), len(got))
+        assert n >= len(ref) - 1
+        ok, worst = fo.compare_timeseries(got[:n], ref[:n])
+        assert ok, (ts, worst)

@@ -38,3 +38,19 @@ def test_library_exports_every_declared_symbol():
 def test_binding_fails_loudly_without_library(tmp_path):
     with pytest.raises(RuntimeError, match="missing"):
         flusi_amd.load_library(str(tmp_path / "nope.so"))
+
+
+def test_fortran_interface_module_links():
+    """The iso_c_binding module and the Fortran driver build against the
+    library (amdflang from ROCm); no GPU call is made."""
+    from flusi_amd import build as b
+    if not os.path.exists(b.FLANG):
+        pytest.skip("no Fortran compiler in this image")
+    if not os.path.exists(flusi_amd.LIB_PATH):
+        pytest.fail("libflusi_hip.so is not built")
+    os.makedirs(b.BUILD, exist_ok=True)
+    exe = b.build_fortran()
+    assert os.path.exists(exe)
+    import subprocess
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert "usage: flusi_amd" in r.stdout
