@@ -12,19 +12,22 @@ struct Strides { long a, b, c; };
 
 enum TileOp { OP_COPY = 0, OP_DERIV = 1, OP_DERIV_MINUS = 2 };
 
+// One job reads its operands once and writes one or two transforms of them.
 struct TileJob {
   const void* in0;   // complex field, layout given by Strides in
-  const void* in1;   // second operand of OP_DERIV_MINUS
+  const void* in1;   // second operand of OP_DERIV_MINUS (nullptr: not read)
   void* out;
   int op;            // out = in0 | i k_a in0 | i k_a in0 - in1
+  void* out2;        // second output of the same operands (nullptr: none)
+  int op2;
 };
 
 struct TilePassArgs {
-  TileJob jobs[8];
+  TileJob jobs[6];
   int njobs;
   int nb, nc;          // extents of the tile axis and the outer axis
   Strides sin, sout;
-  const double* kline; // wavenumbers along the FFT axis (OP_DERIV*)
+  double kscale;       // k_a = kscale * wrap(a): wavenumbers along the FFT axis
   double scale;        // applied to every stored value
   const void* tw;      // L twiddles exp(-2 pi i m / L), element type of the pass
 };
@@ -45,6 +48,7 @@ struct XPassArgs {
   int nlines;          // ny*nz
   int nx, ny, nxh;
   const double* kx;    // nxh wavenumbers
+  double kx_scale;     // kx = kx_scale * k
   const void* twM;     // M = nx/2 twiddles
   const void* twN;     // nxh twiddles exp(-2 pi i k / nx)
   // RHS extras
@@ -113,6 +117,10 @@ template <typename T> void launch_tile_pass(const TilePassArgs& a, int L, int di
 template <typename T> void launch_x_pass(const XPassArgs& a, int mode, hipStream_t s);
 template <typename T> void launch_zstep(const ZStepArgs& a, hipStream_t s);
 int x_pass_blocks(int nx, int nlines, bool fp64);
+// the right-hand-side x pass with one wavefront per line (xrhs_wave.hip)
+template <typename T> void launch_x_rhs_wave(const XPassArgs& a, hipStream_t s);
+bool x_rhs_wave_supported(int nx);
+int x_rhs_wave_blocks(int nx, int nlines);
 int zstep_blocks(int nz, int ny, int nxh, bool fp64);
 bool fft_length_supported(int n);   // y/z line lengths
 bool x_length_supported(int m);     // nx/2 of the real x axis

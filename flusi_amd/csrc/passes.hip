@@ -16,6 +16,7 @@
 // TL lines on the other, staging the tile through LDS.
 #include "fft_core.h"
 #include "flusi_kernels.h"
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -59,7 +60,12 @@ __device__ __forceinline__ double wave_max(double v) {
 // thread <-> butterfly mapping of the first (last) stage follows the input
 // (output) layout: consecutive threads walk the contiguous axis, or the TL
 // lines of a 128-byte segment.
-template <typename T, int L, int TL, int NT, int S, bool CONTIG_IN>
+//
+// A job's operands are read once into registers and give one or two
+// transforms (e.g. u and i k u).  The next job's loads are issued as soon as
+// the current operands have been consumed, so they are in flight while the
+// LDS stages of the current transform run.
+template <typename T, int L, int TL, int NT, int S, bool CONTIG_IN, bool PF>
 __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
   constexpr int NSTG = plan_count_k(PLAN_TILE, L);
   static_assert(NSTG >= 2, "tile pass needs at least two stages");
@@ -75,12 +81,11 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
   const long c = blockIdx.y;
   const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw);
   const T sc = (T)a.scale;
-  for (int jb = 0; jb < a.njobs; ++jb) {
-    const cplx<T>* in0 = reinterpret_cast<const cplx<T>*>(a.jobs[jb].in0);
-    const cplx<T>* in1 = reinterpret_cast<const cplx<T>*>(a.jobs[jb].in1);
-    cplx<T>* out = reinterpret_cast<cplx<T>*>(a.jobs[jb].out);
-    const int op = a.jobs[jb].op;
-    cplx<T> v[PER0][R0];
+
+  cplx<T> r0[PER0][R0], r1[PER0][R0];
+  // operand `which` (0: in0, prefetched; 1: in1, read when its job starts)
+  auto load = [&](int jb, int which) {
+    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(which ? a.jobs[jb].in1 : a.jobs[jb].in0);
 #pragma unroll
     for (int p = 0; p < PER0; ++p) {
       const int beta = tid + p * NT;
@@ -89,69 +94,109 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
         if (CONTIG_IN) { bb = beta / (L / R0); j = beta - bb * (L / R0); }
         else { bb = beta % TL; j = beta / TL; }
         const bool ok = b0 + bb < a.nb;
-        const long base = c * a.sin.c + (long)(b0 + bb) * a.sin.b;
+        // the contiguous side has unit stride (checked on the host)
+        const long base = CONTIG_IN ? c * a.sin.c + (long)(b0 + bb) * a.sin.b + j
+                                    : c * a.sin.c + (long)(b0 + bb) + (long)j * a.sin.a;
+        const long step = CONTIG_IN ? (long)(L / R0) : (long)(L / R0) * a.sin.a;
 #pragma unroll
         for (int q = 0; q < R0; ++q) {
-          const int aa = j + q * (L / R0);
-          cplx<T> x = mk<T>(T(0), T(0));
-          if (ok) {
-            const long idx = base + (long)aa * a.sin.a;
-            x = in0[idx];
-            if (op != OP_COPY) {
-              x = ik_<T>((T)a.kline[aa], x);
-              if (op == OP_DERIV_MINUS) x = sub(x, in1[idx]);
+          const cplx<T> x = ok ? in[base + q * step] : mk<T>(T(0), T(0));
+          if (which) r1[p][q] = x;
+          else r0[p][q] = x;
+        }
+      }
+    }
+  };
+
+  if (PF) load(0, 0);
+  for (int jb = 0; jb < a.njobs; ++jb) {
+    const int nout = a.jobs[jb].out2 ? 2 : 1;
+    if (!PF) load(jb, 0);
+    if (a.jobs[jb].in1) load(jb, 1);
+    for (int o = 0; o < nout; ++o) {
+      const int op = o ? a.jobs[jb].op2 : a.jobs[jb].op;
+      cplx<T>* out = reinterpret_cast<cplx<T>*>(o ? a.jobs[jb].out2 : a.jobs[jb].out);
+#pragma unroll
+      for (int p = 0; p < PER0; ++p) {
+        const int beta = tid + p * NT;
+        if (NB0 % NT == 0 || beta < NB0) {
+          int bb, j;
+          if (CONTIG_IN) { bb = beta / (L / R0); j = beta - bb * (L / R0); }
+          else { bb = beta % TL; j = beta / TL; }
+          cplx<T> v[R0];
+          if (op == OP_COPY) {
+#pragma unroll
+            for (int q = 0; q < R0; ++q) v[q] = r0[p][q];
+          } else {
+            // k_a = kscale * wrap(a), wrap(a) = a - L for a >= L/2; the empty
+            // asm keeps the wavenumbers from being hoisted out of the job loop
+            // (they would hold R0 registers for the whole kernel)
+            int jj = j;
+            asm volatile("" : "+v"(jj));
+#pragma unroll
+            for (int q = 0; q < R0; ++q) {
+              const int aa = jj + q * (L / R0);
+              const T k = (T)(a.kscale * double(aa < L / 2 ? aa : aa - L));
+              v[q] = ik_<T>(k, r0[p][q]);
+            }
+            if (op == OP_DERIV_MINUS) {
+#pragma unroll
+              for (int q = 0; q < R0; ++q) v[q] = sub(v[q], r1[p][q]);
             }
           }
-          v[p][q] = x;
-        }
-        Bfly<R0, S, T>::run(v[p]);
-      }
-    }
+          Bfly<R0, S, T>::run(v);
 #pragma unroll
-    for (int p = 0; p < PER0; ++p) {
-      const int beta = tid + p * NT;
-      if (NB0 % NT == 0 || beta < NB0) {
-        int bb, j;
-        if (CONTIG_IN) { bb = beta / (L / R0); j = beta - bb * (L / R0); }
-        else { bb = beta % TL; j = beta / TL; }
-#pragma unroll
-        for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
-      }
-    }
-    __syncthreads();
-    Stages<T, L, TL, NT, PLAN_TILE, S, LSP, 1, NSTG - 1, R0>::run(buf, tw);
-    constexpr int NSL = L / RL;
-#pragma unroll
-    for (int p = 0; p < PERL; ++p) {
-      const int beta = tid + p * NT;
-      if (NBL % NT == 0 || beta < NBL) {
-        int bb, j;
-        if (!CONTIG_IN) { bb = beta / (L / RL); j = beta - bb * (L / RL); }
-        else { bb = beta % TL; j = beta / TL; }
-        cplx<T> w[RL];
-#pragma unroll
-        for (int q = 0; q < RL; ++q) w[q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
-        twiddle_bfly<T, L, RL, S, NSL>(w, j, tw);
-        if (b0 + bb < a.nb) {
-          const long base = c * a.sout.c + (long)(b0 + bb) * a.sout.b;
-#pragma unroll
-          for (int q = 0; q < RL; ++q) out[base + (long)(j + q * (L / RL)) * a.sout.a] = mk<T>(w[q].x * sc, w[q].y * sc);
+          for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[q];
         }
       }
+      if (PF && o == nout - 1 && jb + 1 < a.njobs) load(jb + 1, 0);
+      __syncthreads();
+      Stages<T, L, TL, NT, PLAN_TILE, S, LSP, 1, NSTG - 1, R0>::run(buf, tw);
+      constexpr int NSL = L / RL;
+#pragma unroll
+      for (int p = 0; p < PERL; ++p) {
+        const int beta = tid + p * NT;
+        if (NBL % NT == 0 || beta < NBL) {
+          int bb, j;
+          if (!CONTIG_IN) { bb = beta / (L / RL); j = beta - bb * (L / RL); }
+          else { bb = beta % TL; j = beta / TL; }
+          cplx<T> w[RL];
+#pragma unroll
+          for (int q = 0; q < RL; ++q) w[q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
+          twiddle_bfly<T, L, RL, S, NSL>(w, j, tw);
+          if (b0 + bb < a.nb) {
+            const long base = !CONTIG_IN ? c * a.sout.c + (long)(b0 + bb) * a.sout.b + j
+                                         : c * a.sout.c + (long)(b0 + bb) + (long)j * a.sout.a;
+            const long step = !CONTIG_IN ? (long)(L / RL) : (long)(L / RL) * a.sout.a;
+#pragma unroll
+            for (int q = 0; q < RL; ++q) out[base + q * step] = mk<T>(w[q].x * sc, w[q].y * sc);
+          }
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
-template <typename T, int L, int S, bool CI>
-static void run_tile(const TilePassArgs& a, hipStream_t s) {
+static int tile_prefetch() {
+  static int v = [] { const char* e = getenv("FLUSI_TILE_PREFETCH"); return e ? atoi(e) : 0; }();
+  return v;
+}
+
+template <typename T, int L, int S, bool CI, bool PF>
+static void run_tile_pf(const TilePassArgs& a, hipStream_t s) {
   constexpr int TL = Tile<T>::TL;
   constexpr int NT = pick_nt(TL * L);
   const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
   static bool init = false;
-  if (!init) { set_smem(tile_pass_k<T, L, TL, NT, S, CI>, smem); init = true; }
+  if (!init) { set_smem(tile_pass_k<T, L, TL, NT, S, CI, PF>, smem); init = true; }
   dim3 grid((a.nb + TL - 1) / TL, a.nc);
-  hipLaunchKernelGGL((tile_pass_k<T, L, TL, NT, S, CI>), grid, dim3(NT), smem, s, a);
+  hipLaunchKernelGGL((tile_pass_k<T, L, TL, NT, S, CI, PF>), grid, dim3(NT), smem, s, a);
+}
+template <typename T, int L, int S, bool CI>
+static void run_tile(const TilePassArgs& a, hipStream_t s) {
+  if (tile_prefetch()) run_tile_pf<T, L, S, CI, true>(a, s);
+  else run_tile_pf<T, L, S, CI, false>(a, s);
 }
 
 template <typename T, int L>
@@ -177,6 +222,10 @@ bool fft_length_supported(int n) {
 
 template <typename T>
 void launch_tile_pass(const TilePassArgs& a, int L, int dir, bool ci, hipStream_t s) {
+  // contiguous input lines -> output tile axis contiguous, and vice versa
+  const bool unit = ci ? (a.sin.a == 1 && a.sout.b == 1) : (a.sin.b == 1 && a.sout.a == 1);
+  if (!unit) throw std::runtime_error("tile pass: the contiguous axes must have unit stride");
+  if (a.njobs < 1 || a.njobs > 6) throw std::runtime_error("tile pass: 1..6 jobs");
   switch (L) {
     case 16: run_tile_dir<T, 16>(a, dir, ci, s); break;
     case 24: run_tile_dir<T, 24>(a, dir, ci, s); break;

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -136,6 +137,7 @@ class Solver : public SolverBase {
   double *kx_d, *ky_d, *kz_d, *kxt2_d, *kyt2_d, *kzt2_d;
   void* tw_z = nullptr; void* tw_y = nullptr; void* tw_M = nullptr; void* tw_N = nullptr;
   double* xpart = nullptr; int nxblocks = 0;
+  bool xwave = false;       // right-hand-side x pass with one wavefront per line
   double* zpart = nullptr; int nzblocks = 0;
   double* dpart = nullptr; int ndblocks = 0;
   double* red = nullptr;    // small results
@@ -143,6 +145,7 @@ class Solver : public SolverBase {
   int* kinds = nullptr;     // reduction kinds: [0..3] x pass, [4] energy, [8..8+DIAG_NV) diag
   double* stage = nullptr; size_t stage_n = 0;
   std::vector<double> kx_h, ky_h, kz_h, kxt2_h, kyt2_h, kzt2_h;
+  double ksy = 0.0, ksz = 0.0;  // 2 pi / yl, 2 pi / zl
   GridGeom geom{};
   bool timing = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
@@ -198,6 +201,7 @@ class Solver : public SolverBase {
     kcheck("zero state");
     // wavenumbers (cof_p3dfft.f90:661-680) and dealias terms (vis.f90:23-53)
     const double sx = 2.0 * PI / p.xl, sy = 2.0 * PI / p.yl, sz = 2.0 * PI / p.zl;
+    ksy = sy; ksz = sz;
     const double kxt = (2.0 / 3.0) * double(nx / 2 - 1), kyt = (2.0 / 3.0) * double(ny / 2 - 1),
                  kzt = (2.0 / 3.0) * double(nz / 2 - 1);
     kx_h.resize(nxh); ky_h.resize(ny); kz_h.resize(nz);
@@ -219,7 +223,9 @@ class Solver : public SolverBase {
     kx_d = tab; ky_d = kx_d + nxh; kz_d = ky_d + ny; kxt2_d = kz_d + nz; kyt2_d = kxt2_d + nxh; kzt2_d = kyt2_d + ny;
     if (fft_ok) {
       tw_z = make_tw(nz, nz); tw_y = make_tw(ny, ny); tw_M = make_tw(nx / 2, nx / 2); tw_N = make_tw(nxh, nx);
-      nxblocks = x_pass_blocks(nx, ny * nz, FP64);
+      xwave = x_rhs_wave_supported(nx);
+      if (const char* e = getenv("FLUSI_XWAVE")) xwave = xwave && atoi(e) != 0;
+      nxblocks = xwave ? x_rhs_wave_blocks(nx, ny * nz) : x_pass_blocks(nx, ny * nz, FP64);
       xpart = (double*)dalloc(sizeof(double) * 4 * nxblocks);
       nzblocks = zstep_blocks(nz, ny, nxh, FP64);
       zpart = (double*)dalloc(sizeof(double) * nzblocks);
@@ -283,7 +289,7 @@ class Solver : public SolverBase {
     a.njobs = nj; a.nb = ny; a.nc = nxh;
     a.sin = Strides{1, nz, (long)ny * nz};
     a.sout = Strides{ny, 1, (long)nz * ny};
-    a.kline = kz_d; a.scale = 1.0; a.tw = tw_z;
+    a.kscale = ksz; a.scale = 1.0; a.tw = tw_z;
     timed(TM_ZINV, [&] { launch_tile_pass<T>(a, nz, +1, true, stream); });
     kcheck("zinv");
   }
@@ -293,34 +299,34 @@ class Solver : public SolverBase {
     a.njobs = nj; a.nb = nxh; a.nc = nz;
     a.sin = Strides{1, (long)nz * ny, ny};
     a.sout = Strides{nxh, 1, (long)ny * nxh};
-    a.kline = ky_d; a.scale = 1.0; a.tw = tw_y;
+    a.kscale = ksy; a.scale = 1.0; a.tw = tw_y;
     timed(TM_YINV, [&] { launch_tile_pass<T>(a, ny, +1, true, stream); });
     kcheck("yinv");
   }
   void yfwd(int nf) {  // P -> Q
     TilePassArgs a{};
-    for (int j = 0; j < nf; ++j) a.jobs[j] = TileJob{cf(P, j), nullptr, cf(Q, j), OP_COPY};
+    for (int j = 0; j < nf; ++j) a.jobs[j] = TileJob{cf(P, j), nullptr, cf(Q, j), OP_COPY, nullptr, 0};
     a.njobs = nf; a.nb = nxh; a.nc = nz;
     a.sin = Strides{nxh, 1, (long)ny * nxh};
     a.sout = Strides{1, (long)nz * ny, ny};
-    a.kline = ky_d; a.scale = 1.0; a.tw = tw_y;
+    a.kscale = ksy; a.scale = 1.0; a.tw = tw_y;
     timed(TM_YFWD, [&] { launch_tile_pass<T>(a, ny, -1, false, stream); });
     kcheck("yfwd");
   }
   void zfwd_plain(int nf, void* dst) {  // Q -> S with 1/N (cof_p3dfft.f90:52-56)
     TilePassArgs a{};
-    for (int j = 0; j < nf; ++j) a.jobs[j] = TileJob{cf(Q, j), nullptr, cf(dst, j), OP_COPY};
+    for (int j = 0; j < nf; ++j) a.jobs[j] = TileJob{cf(Q, j), nullptr, cf(dst, j), OP_COPY, nullptr, 0};
     a.njobs = nf; a.nb = ny; a.nc = nxh;
     a.sin = Strides{ny, 1, (long)nz * ny};
     a.sout = Strides{1, nz, (long)ny * nz};
-    a.kline = kz_d; a.scale = 1.0 / (double(nx) * double(ny) * double(nz)); a.tw = tw_z;
+    a.kscale = ksz; a.scale = 1.0 / (double(nx) * double(ny) * double(nz)); a.tw = tw_z;
     timed(TM_ZSTEP, [&] { launch_tile_pass<T>(a, nz, -1, false, stream); });
     kcheck("zfwd");
   }
   XPassArgs xargs() {
     XPassArgs a{};
     a.nlines = ny * nz; a.nx = nx; a.ny = ny; a.nz = nz; a.nxh = nxh;
-    a.kx = kx_d; a.twM = tw_M; a.twN = tw_N;
+    a.kx = kx_d; a.kx_scale = 2.0 * PI / p.xl; a.twM = tw_M; a.twN = tw_N;
     a.npts = npts;
     return a;
   }
@@ -329,14 +335,17 @@ class Solver : public SolverBase {
   // P: 0 V_x, 1 V_y, 2 V_z, 3 w_x, 4 dz u_x, 5 dy u_x
   void inv_uw(const void* u_in) {
     const C* u = reinterpret_cast<const C*>(u_in);
-    TileJob zj[5] = {{u, nullptr, cf(Q, 0), OP_COPY}, {u, nullptr, cf(Q, 3), OP_DERIV},
-                     {u + nc, nullptr, cf(Q, 1), OP_COPY}, {u + nc, nullptr, cf(Q, 4), OP_DERIV},
-                     {u + 2 * nc, nullptr, cf(Q, 2), OP_COPY}};
-    zinv(zj, 5);
-    TileJob yj[6] = {{cf(Q, 0), nullptr, cf(P, 0), OP_COPY}, {cf(Q, 0), nullptr, cf(P, 5), OP_DERIV},
-                     {cf(Q, 1), nullptr, cf(P, 1), OP_COPY}, {cf(Q, 2), nullptr, cf(P, 2), OP_COPY},
-                     {cf(Q, 2), cf(Q, 4), cf(P, 3), OP_DERIV_MINUS}, {cf(Q, 3), nullptr, cf(P, 4), OP_COPY}};
-    yinv(yj, 6);
+    // each operand is read once: u_x gives U_x and D_x, u_y gives U_y and D_y
+    TileJob zj[3] = {{u, nullptr, cf(Q, 0), OP_COPY, cf(Q, 3), OP_DERIV},
+                     {u + nc, nullptr, cf(Q, 1), OP_COPY, cf(Q, 4), OP_DERIV},
+                     {u + 2 * nc, nullptr, cf(Q, 2), OP_COPY, nullptr, 0}};
+    zinv(zj, 3);
+    // U_x gives V_x and dy u_x; (U_z, D_y) give V_z and w_x
+    TileJob yj[4] = {{cf(Q, 0), nullptr, cf(P, 0), OP_COPY, cf(P, 5), OP_DERIV},
+                     {cf(Q, 2), cf(Q, 4), cf(P, 2), OP_COPY, cf(P, 3), OP_DERIV_MINUS},
+                     {cf(Q, 1), nullptr, cf(P, 1), OP_COPY, nullptr, 0},
+                     {cf(Q, 3), nullptr, cf(P, 4), OP_COPY, nullptr, 0}};
+    yinv(yj, 4);
   }
 
   int sponge_code() const {
@@ -362,7 +371,10 @@ class Solver : public SolverBase {
     a.partial = xpart;
     a.want_stats = (p.dt_fixed <= 0.0 || p.iMeanFlow[0] == "dynamic" || p.iMeanFlow[1] == "dynamic" ||
                     p.iMeanFlow[2] == "dynamic") ? 1 : 0;
-    timed(TM_X, [&] { launch_x_pass<T>(a, XM_RHS, stream); });
+    timed(TM_X, [&] {
+      if (xwave) launch_x_rhs_wave<T>(a, stream);
+      else launch_x_pass<T>(a, XM_RHS, stream);
+    });
     kcheck("x pass");
     if (a.want_stats) launch_reduce_rows(xpart, nxblocks, 4, kinds, rmid, red, stream);
     kcheck("reduce x");
@@ -702,8 +714,8 @@ class Solver : public SolverBase {
     const C* s = reinterpret_cast<const C*>(src);
     TileJob zj[3], yj[3];
     for (int f = 0; f < nf; ++f) {
-      zj[f] = TileJob{s + (long)f * nc, nullptr, cf(Q, f), OP_COPY};
-      yj[f] = TileJob{cf(Q, f), nullptr, cf(P, f), OP_COPY};
+      zj[f] = TileJob{s + (long)f * nc, nullptr, cf(Q, f), OP_COPY, nullptr, 0};
+      yj[f] = TileJob{cf(Q, f), nullptr, cf(P, f), OP_COPY, nullptr, 0};
     }
     zinv(zj, nf);
     yinv(yj, nf);

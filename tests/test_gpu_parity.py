@@ -92,6 +92,30 @@ def test_sphere_cavity_sponge_cal_nlk(tmp_path, prec, tol):
         assert rel_l2(nlk, nlk_ref) < tol * 50
 
 
+@pytest.mark.parametrize("nx", [128, 256, 512])
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_x_pass_per_wave_grids(tmp_path, nx, prec, tol):
+    """The right-hand-side x pass that holds one x line per wavefront (two for
+    nx = 256, four for nx = 128): cal_nlk with penalisation and the cavity
+    sponge, and one CFL-adapted RK2 step (max |u| comes from that pass)."""
+    ini = rewrite_ini(os.path.join(GOLDEN, "sphere_sponge_RK2", "sphere_sponge_RK2.ini"), str(tmp_path),
+                      Resolution__nx=nx, Resolution__ny=16, Resolution__nz=16)
+    sim = fo.Simulation(ini)
+    sim.init_fields()
+    uk = _perturbed_state(sim)
+    sim.uk = uk
+    nlk_ref, _ = sim.cal_nlk(0.0, uk)
+    with flusi_amd.Flusi(ini, prec) as s:
+        s.init_fields()
+        s.set_uk(uk)
+        assert rel_l2(s.cal_nlk(0.0), nlk_ref) < tol * 50
+        sim.step()
+        s.step()
+        t, it, dt0, dt1 = s.get_time()
+        assert it == sim.it and abs(dt1 - sim.dt1) <= 1e-12 * sim.dt1
+        assert rel_l2(s.get_uk(), sim.uk) < tol * 50
+
+
 def _run_both(ini, prec, nsteps, perturb=False):
     sim = fo.Simulation(ini)
     sim.init_fields()
