@@ -54,6 +54,21 @@ def pass_bytes(n, s):
     }
 
 
+def measured_traffic(n, kernel):
+    """HBM bytes per launch of a pass from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the same counters
+    read for a device copy of known size); None when not measured for this
+    grid.  Lines of the file: `traffic <grid> <PASS> <bytes>`."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_insect_fp32.md")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = ln.split()
+        if len(f) == 4 and f[0] == "traffic" and f[1] == str(n) and f[2] == kernel:
+            return float(f[3])
+    return None
+
+
 def run_gpu(ini, prec, warmup, steps, timers):
     """Warm up, then time `steps` steps; flusi_timers synchronises the solver's
     stream on both sides of the timed region and collects the per-pass HIP
@@ -143,7 +158,7 @@ def main():
     avg_s = ms32[best] / launches / 1e3
     ach = pb[best] / avg_s
     out["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK, "traffic": None, "kernel": best,
+                       "frac": ach / HBM_PEAK, "traffic": measured_traffic(n, best), "kernel": best,
                        "algorithmic_bytes_per_launch": pb[best], "avg_launch_ms": avg_s * 1e3}
     out["pass_ms_per_step"] = {k: v / args.steps for k, v in ms32.items() if v > 0}
     out["step_roofline_frac"] = B_PT_RK4[4] * gps32 / HBM_PEAK

@@ -42,6 +42,10 @@ struct InsectArgs {
   double rot_body_b[3];
   int use_box;              // 1: skip points outside the index boxes below
   int box_lo[3][3], box_hi[3][3];  // [body, right wing, left wing][axis]
+  // points visited: [reg_lo, reg_hi] per axis.  Outside it nothing is drawn
+  // and no wing point of the previous call lies there (the whole grid on a
+  // full reset or for periodic insects).
+  int reg_lo[3], reg_hi[3];
 };
 
 // drag.f90:17-454 + integrals.f90:48-238 reductions over physical u, w, div

@@ -90,9 +90,10 @@ void launch_cavity(const GridGeom& g, const CavityArgs& a, void* mask, void* us,
 // stage II body-motion transform (wings_geometry.f90:71-108).
 template <typename T>
 __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* color) {
-  // grid: x in blocks of blockDim.x, y = blockIdx.y, z = blockIdx.z
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
-  if (ix >= g.nx) return;
+  // grid over the region: x in blocks of blockDim.x, y = blockIdx.y, z = blockIdx.z
+  const int ix = a.reg_lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = a.reg_lo[1] + blockIdx.y, iz = a.reg_lo[2] + blockIdx.z;
+  if (ix > a.reg_hi[0]) return;
   const long p = ((long)iz * g.ny + iy) * g.nx + ix;
   const uint8_t c_old = color[p];
   const bool old_wing = (c_old == 2 || c_old == 3);
@@ -199,7 +200,14 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
 template <typename T>
 void launch_insect(const GridGeom& g, const InsectArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s) {
   const int nt = 256;
-  hipLaunchKernelGGL(insect_k<T>, dim3((g.nx + nt - 1) / nt, g.ny, g.nz), dim3(nt), 0, s, g, a, (T*)mask, (T*)us, color);
+  if (a.reg_lo[0] > a.reg_hi[0] || a.reg_lo[1] > a.reg_hi[1] || a.reg_lo[2] > a.reg_hi[2]) return;  // nothing to update
+  for (int i = 0; i < 3; ++i) {
+    const int n = i == 0 ? g.nx : (i == 1 ? g.ny : g.nz);
+    if (a.reg_lo[i] < 0 || a.reg_hi[i] >= n || a.reg_lo[i] > a.reg_hi[i])
+      throw std::runtime_error("insect mask: region outside the grid");
+  }
+  const int ex = a.reg_hi[0] - a.reg_lo[0] + 1, ey = a.reg_hi[1] - a.reg_lo[1] + 1, ez = a.reg_hi[2] - a.reg_lo[2] + 1;
+  hipLaunchKernelGGL(insect_k<T>, dim3((ex + nt - 1) / nt, ey, ez), dim3(nt), 0, s, g, a, (T*)mask, (T*)us, color);
 }
 
 // ------------------------------------------------------------- vortex ring

@@ -47,6 +47,7 @@ struct XPassArgs {
   int nf;              // fields for XM_INV / XM_FWD
   int nlines;          // ny*nz
   int nx, ny, nxh;
+  long prow;           // row stride of the spectral (P-layout) side, >= nxh
   const double* kx;    // nxh wavenumbers
   double kx_scale;     // kx = kx_scale * k
   const void* twM;     // M = nx/2 twiddles

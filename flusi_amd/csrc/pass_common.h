@@ -1,0 +1,40 @@
+// Helpers shared by the transform passes (passes.hip, zstep.hip).
+#pragma once
+#include "fft_core.h"
+#include <stdexcept>
+#include <string>
+
+namespace flusi {
+
+template <typename T> struct Tile { static constexpr int TL = 16; };
+template <> struct Tile<double> { static constexpr int TL = 8; };
+
+// threads of a tile block: one per radix-8 butterfly of the tile, 64..1024
+__host__ __device__ constexpr int pick_nt(int n) {
+  return (n / 8) >= 1024 ? 1024 : ((n / 8) <= 64 ? 64 : ((n / 8 + 63) / 64) * 64);
+}
+
+template <typename C> __device__ __forceinline__ C conj_(C a) { a.y = -a.y; return a; }
+template <typename T> __device__ __forceinline__ cplx<T> ik_(T k, cplx<T> v) { return mk<T>(-k * v.y, k * v.x); }
+
+template <typename KF> static void set_smem(KF kernel, size_t bytes) {
+  if (bytes > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  }
+}
+
+// -------------------------------------------------------------- wave/block sums
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+}  // namespace flusi

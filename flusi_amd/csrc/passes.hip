@@ -16,42 +16,12 @@
 // TL lines on the other, staging the tile through LDS.
 #include "fft_core.h"
 #include "flusi_kernels.h"
+#include "pass_common.h"
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
 
 namespace flusi {
-
-template <typename T> struct Tile { static constexpr int TL = 16; };
-template <> struct Tile<double> { static constexpr int TL = 8; };
-
-// threads of a tile block: one per radix-8 butterfly of the tile, 64..1024
-__host__ __device__ constexpr int pick_nt(int n) {
-  return (n / 8) >= 1024 ? 1024 : ((n / 8) <= 64 ? 64 : ((n / 8 + 63) / 64) * 64);
-}
-
-template <typename C> __device__ __forceinline__ C conj_(C a) { a.y = -a.y; return a; }
-template <typename T> __device__ __forceinline__ cplx<T> ik_(T k, cplx<T> v) { return mk<T>(-k * v.y, k * v.x); }
-
-template <typename KF> static void set_smem(KF kernel, size_t bytes) {
-  if (bytes > 65536) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) throw std::runtime_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-  }
-}
-
-// -------------------------------------------------------------- wave/block sums
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
 
 // ======================================================== generic tile pass
 // One workgroup transforms TL lines of length L for each job: the first
@@ -179,7 +149,7 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
 }
 
 static int tile_prefetch() {
-  static int v = [] { const char* e = getenv("FLUSI_TILE_PREFETCH"); return e ? atoi(e) : 0; }();
+  static int v = [] { const char* e = getenv("FLUSI_TILE_PREFETCH"); return e ? atoi(e) : 1; }();
   return v;
 }
 
@@ -290,7 +260,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
         const int line = e / nxh, k = e - line * nxh;
         const int gl = l0 + line;
         cplx<T> v = mk<T>(T(0), T(0));
-        if (gl < a.nlines) v = in[f * a.fstride_in + (long)gl * nxh + k];
+        if (gl < a.nlines) v = in[f * a.fstride_in + (long)gl * a.prow + k];
         L_(f, line)[lp<T>(k)] = v;
       }
     __syncthreads();
@@ -523,7 +493,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
       const cplx<T> zk = L_(f, line)[lp<T>(k)];
       const cplx<T> zm = L_(f, line)[lp<T>(k2 == M ? 0 : k2)];
       const cplx<T> s1 = add(zk, conj_(zm)), d1 = sub(zk, conj_(zm));
-      cplx<T>* o = out + f * a.fstride_out + (long)gl * nxh;
+      cplx<T>* o = out + f * a.fstride_out + (long)gl * a.prow;
       o[k] = scl(sub(s1, ik_<T>(T(1), cmul(wa, d1))), half);
       if (k2 != k) {
         const cplx<T> s2 = add(zm, conj_(zk)), d2 = sub(zm, conj_(zk));
@@ -592,285 +562,5 @@ void launch_x_pass(const XPassArgs& a, int mode, hipStream_t s) {
 template void launch_x_pass<float>(const XPassArgs&, int, hipStream_t);
 template void launch_x_pass<double>(const XPassArgs&, int, hipStream_t);
 
-// ======================================================== z forward + epilogue
-__device__ __forceinline__ double vis_factor(double dtv, double nu, double k2, bool alias) {
-  if (dtv < 0.0 || alias) return 0.0;
-  return exp(-dtv * nu * k2);
-}
-
-template <typename T, int L, int TL, int NT, bool SPONGE>
-__global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
-  constexpr int NSTG = plan_count_k(PLAN_TILE, L);
-  constexpr int R0 = plan_radix_k(PLAN_TILE, L, 0);
-  constexpr int RL = plan_radix_k(PLAN_TILE, L, NSTG - 1);
-  constexpr int NSL = L / RL;
-  constexpr int LSP = line_stride<T>(L);
-  constexpr int NB0 = TL * (L / R0), PER0 = (NB0 + NT - 1) / NT;
-  constexpr int NBL = TL * (L / RL), PERL = (NBL + NT - 1) / NT;
-  constexpr int NF = SPONGE ? 6 : 3;
-  extern __shared__ char smem_raw[];
-  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
-  __shared__ double red[NT / 64];
-  const int tid = threadIdx.x;
-  const int b0 = blockIdx.x * TL;
-  const int kx = blockIdx.y;
-  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw);
-  const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
-  // r[f][p][q]: field f at (ky = b0 + bb, kz = j + q L/RL) of last-stage butterfly p
-  cplx<T> r[NF][PERL][RL];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    cplx<T> v[PER0][R0];
-#pragma unroll
-    for (int p = 0; p < PER0; ++p) {
-      const int beta = tid + p * NT;
-      if (NB0 % NT == 0 || beta < NB0) {
-        const int bb = beta % TL, j = beta / TL;
-        const bool ok = b0 + bb < a.ny;
-#pragma unroll
-        for (int q = 0; q < R0; ++q)
-          v[p][q] = ok ? nq[f * a.fs + ((long)kx * a.nz + j + q * (L / R0)) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
-        Bfly<R0, -1, T>::run(v[p]);
-#pragma unroll
-        for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
-      }
-    }
-    __syncthreads();
-    Stages<T, L, TL, NT, PLAN_TILE, -1, LSP, 1, NSTG - 1, R0>::run(buf, tw);
-#pragma unroll
-    for (int p = 0; p < PERL; ++p) {
-      const int beta = tid + p * NT;
-      if (NBL % NT == 0 || beta < NBL) {
-        const int bb = beta / (L / RL), j = beta - bb * (L / RL);
-#pragma unroll
-        for (int q = 0; q < RL; ++q) r[f][p][q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
-        twiddle_bfly<T, L, RL, -1, NSL>(r[f][p], j, tw);
-      }
-    }
-    __syncthreads();
-  }
-
-  const double kxv = a.kx[kx];
-  const double kxt2 = a.kxt2[kx];
-  const double wE = (kx == 0 || kx == a.nx / 2) ? 0.5 : 1.0;
-  double esum = 0.0, ecomp = 0.0;
-  cplx<T>* U = reinterpret_cast<cplx<T>*>(a.u);
-  cplx<T>* UST = reinterpret_cast<cplx<T>*>(a.ustage);
-  const cplx<T>* UD = reinterpret_cast<const cplx<T>*>(a.udiff);
-  cplx<T>* ACC = reinterpret_cast<cplx<T>*>(a.acc);
-  cplx<T>* RG0 = reinterpret_cast<cplx<T>*>(a.r0);
-  const cplx<T>* RG1 = reinterpret_cast<const cplx<T>*>(a.r1);
-  cplx<T>* RO = reinterpret_cast<cplx<T>*>(a.rhs_out);
-  const bool final_stage = a.mode == SM_EULER || a.mode == SM_AB2 || a.mode == SM_RK2_2 || a.mode == SM_RK4_4;
-#pragma unroll
-  for (int p = 0; p < PERL; ++p)
-#pragma unroll
-  for (int q = 0; q < RL; ++q) {
-    const int beta = tid + p * NT;
-    if (!(NBL % NT == 0 || beta < NBL)) continue;
-    const int bb = beta / (L / RL), j = beta - bb * (L / RL);
-    const int kzi = j + q * (L / RL);
-    const int ky = b0 + bb;
-    if (ky >= a.ny) continue;
-    const long si = ((long)kx * a.ny + ky) * a.nz + kzi;
-    const double kyv = a.ky[ky], kzv = a.kz[kzi];
-    const double k2 = (kxv * kxv + kyv * kyv) + kzv * kzv;
-    const bool alias = ((kxt2 + a.kyt2[ky]) + a.kzt2[kzi]) >= 1.0;
-    double nr[3], ni[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { nr[c] = (double)r[c][p][q].x * a.scale; ni[c] = (double)r[c][p][q].y * a.scale; }
-    if constexpr (SPONGE) {
-      // sponge.f90:22-83: i k x (w/k^2), zero at k = 0
-      if (k2 != 0.0) {
-        double sr[3], sm[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { sr[c] = (double)r[3 + c][p][q].x * a.scale / k2; sm[c] = (double)r[3 + c][p][q].y * a.scale / k2; }
-        const double cr0 = kyv * sr[2] - kzv * sr[1], ci0 = kyv * sm[2] - kzv * sm[1];
-        const double cr1 = kzv * sr[0] - kxv * sr[2], ci1 = kzv * sm[0] - kxv * sm[2];
-        const double cr2 = kxv * sr[1] - kyv * sr[0], ci2 = kxv * sm[1] - kyv * sm[0];
-        nr[0] += -ci0; ni[0] += cr0;
-        nr[1] += -ci1; ni[1] += cr1;
-        nr[2] += -ci2; ni[2] += cr2;
-      }
-    }
-    if (kx == 0 && ky == 0 && kzi == 0) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        if (a.meanflow_dynamic[c]) { nr[c] = -a.pen_sum[c] * a.dynamic_scale; ni[c] = 0.0; }
-    }
-    if (a.explicit_diffusion) {
-      // rhs.f90:441-468
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const cplx<T> us_ = UD[c * a.fs + si];
-        nr[c] -= a.nu * k2 * (double)us_.x;
-        ni[c] -= a.nu * k2 * (double)us_.y;
-      }
-    }
-    if (k2 != 0.0) {
-      // rhs.f90:575-615
-      const double qr = (kxv * nr[0] + kyv * nr[1] + kzv * nr[2]) / k2;
-      const double qi = (kxv * ni[0] + kyv * ni[1] + kzv * ni[2]) / k2;
-      nr[0] -= kxv * qr; ni[0] -= kxv * qi;
-      nr[1] -= kyv * qr; ni[1] -= kyv * qi;
-      nr[2] -= kzv * qr; ni[2] -= kzv * qi;
-    }
-    double unr[3] = {0, 0, 0}, uni[3] = {0, 0, 0};
-    switch (a.mode) {
-      case SM_RHS_ONLY:
-#pragma unroll
-        for (int c = 0; c < 3; ++c) RO[c * a.fs + si] = mk<T>((T)nr[c], (T)ni[c]);
-        break;
-      case SM_EULER: {
-        // fluid_time_step.f90:718-727
-        const double E = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const cplx<T> u0 = U[c * a.fs + si];
-          unr[c] = ((double)u0.x + a.dt * nr[c]) * E;
-          uni[c] = ((double)u0.y + a.dt * ni[c]) * E;
-          U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-          RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
-        }
-        break;
-      }
-      case SM_AB2: {
-        // fluid_time_step.f90:802-811
-        const double E = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const cplx<T> u0 = U[c * a.fs + si];
-          const cplx<T> n1 = RG1[c * a.fs + si];
-          unr[c] = ((double)u0.x + a.b10 * nr[c] + a.b11 * (double)n1.x) * E;
-          uni[c] = ((double)u0.y + a.b10 * ni[c] + a.b11 * (double)n1.y) * E;
-          U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-          RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
-        }
-        break;
-      }
-      case SM_RK2_1: {
-        // fluid_time_step.f90:554-575 (N0 weighted with the IF of the previous dt)
-        const double Eo = vis_factor(a.vis_dt_old, a.nu, k2, a.dealias_on && alias);
-        const double En = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double n0r = nr[c] * Eo, n0i = ni[c] * Eo;
-          const cplx<T> u0 = U[c * a.fs + si];
-          RG0[c * a.fs + si] = mk<T>((T)n0r, (T)n0i);
-          U[c * a.fs + si] = mk<T>((T)((double)u0.x * En + a.dt * n0r), (T)((double)u0.y * En + a.dt * n0i));
-        }
-        break;
-      }
-      case SM_RK2_2: {
-        // fluid_time_step.f90:594-597
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const cplx<T> u0 = U[c * a.fs + si];
-          const cplx<T> n0 = RG0[c * a.fs + si];
-          unr[c] = (double)u0.x + 0.5 * a.dt * (-(double)n0.x + nr[c]);
-          uni[c] = (double)u0.y + 0.5 * a.dt * (-(double)n0.y + ni[c]);
-        }
-        break;
-      }
-      default: {
-        // RK4 (fluid_time_step.f90:636-669) with an accumulator register
-        const double kill = alias ? 0.0 : 1.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double kr = nr[c] * kill, ki = ni[c] * kill;
-          if (a.mode == SM_RK4_1) {
-            const cplx<T> u0 = U[c * a.fs + si];
-            ACC[c * a.fs + si] = mk<T>((T)((double)u0.x + (a.dt / 6.0) * kr), (T)((double)u0.y + (a.dt / 6.0) * ki));
-            UST[c * a.fs + si] = mk<T>((T)((double)u0.x + 0.5 * a.dt * kr), (T)((double)u0.y + 0.5 * a.dt * ki));
-          } else if (a.mode == SM_RK4_2 || a.mode == SM_RK4_3) {
-            const cplx<T> u0 = U[c * a.fs + si];
-            const cplx<T> ac = ACC[c * a.fs + si];
-            const double cst = (a.mode == SM_RK4_2) ? 0.5 * a.dt : a.dt;
-            ACC[c * a.fs + si] = mk<T>((T)((double)ac.x + (a.dt / 3.0) * kr), (T)((double)ac.y + (a.dt / 3.0) * ki));
-            UST[c * a.fs + si] = mk<T>((T)((double)u0.x + cst * kr), (T)((double)u0.y + cst * ki));
-          } else {
-            const cplx<T> ac = ACC[c * a.fs + si];
-            unr[c] = ((double)ac.x + (a.dt / 6.0) * kr) * kill;
-            uni[c] = ((double)ac.y + (a.dt / 6.0) * ki) * kill;
-          }
-        }
-        break;
-      }
-    }
-    if (final_stage) {
-      if (kx == 0 && ky == 0 && kzi == 0) {
-        // fluid_time_step.f90:1066-1090 set_mean_flow
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          if (a.set_mean[c]) { unr[c] = a.mean_value[c]; uni[c] = 0.0; }
-      }
-      if (a.mode == SM_RK2_2 || a.mode == SM_RK4_4 || (kx == 0 && ky == 0 && kzi == 0)) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-      }
-      // integrals.f90:522-552 Parseval sum of the stored (rounded) field
-      double ev = 0.0;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double xr = (double)(T)unr[c], xi = (double)(T)uni[c];
-        ev += xr * xr + xi * xi;
-      }
-      ev *= wE;
-      const double t = esum + ev;
-      ecomp += (fabs(esum) >= fabs(ev)) ? (esum - t) + ev : (ev - t) + esum;
-      esum = t;
-    }
-  }
-  if (final_stage && a.energy_partial) {
-    double v = wave_sum(esum + ecomp);
-    if ((tid & 63) == 0) red[tid >> 6] = v;
-    __syncthreads();
-    if (tid == 0) {
-      double s = 0.0;
-      for (int w = 0; w < NT / 64; ++w) s += red[w];
-      a.energy_partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = s;
-    }
-  }
-}
-
-int zstep_blocks(int nz, int ny, int nxh, bool fp64) {
-  const int TL = fp64 ? Tile<double>::TL : Tile<float>::TL;
-  return ((ny + TL - 1) / TL) * nxh;
-}
-
-template <typename T, int L, bool SP>
-static void run_z(const ZStepArgs& a, hipStream_t s) {
-  constexpr int TL = Tile<T>::TL;
-  constexpr int NT = pick_nt(TL * L);
-  const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
-  static bool init = false;
-  if (!init) { set_smem(zstep_k<T, L, TL, NT, SP>, smem); init = true; }
-  dim3 grid((a.ny + TL - 1) / TL, a.nxh);
-  hipLaunchKernelGGL((zstep_k<T, L, TL, NT, SP>), grid, dim3(NT), smem, s, a);
-}
-template <typename T, int L>
-static void run_z_sp(const ZStepArgs& a, hipStream_t s) {
-  if (a.sponge) run_z<T, L, true>(a, s); else run_z<T, L, false>(a, s);
-}
-
-template <typename T>
-void launch_zstep(const ZStepArgs& a, hipStream_t s) {
-  switch (a.nz) {
-    case 16: run_z_sp<T, 16>(a, s); break;
-    case 24: run_z_sp<T, 24>(a, s); break;
-    case 32: run_z_sp<T, 32>(a, s); break;
-    case 48: run_z_sp<T, 48>(a, s); break;
-    case 64: run_z_sp<T, 64>(a, s); break;
-    case 96: run_z_sp<T, 96>(a, s); break;
-    case 128: run_z_sp<T, 128>(a, s); break;
-    case 192: run_z_sp<T, 192>(a, s); break;
-    case 256: run_z_sp<T, 256>(a, s); break;
-    case 384: run_z_sp<T, 384>(a, s); break;
-    case 512: run_z_sp<T, 512>(a, s); break;
-    default: throw std::runtime_error("unsupported nz " + std::to_string(a.nz));
-  }
-}
-template void launch_zstep<float>(const ZStepArgs&, hipStream_t);
-template void launch_zstep<double>(const ZStepArgs&, hipStream_t);
 
 }  // namespace flusi

@@ -129,6 +129,7 @@ class Solver : public SolverBase {
   void* reg[2] = {nullptr, nullptr};
   void* Q = nullptr;
   void* P = nullptr;
+  long prow = 0, pc = 0;    // P-layout row stride (kx) and field stride
   void* mask = nullptr;
   void* us = nullptr;
   uint8_t* color = nullptr;
@@ -193,7 +194,12 @@ class Solver : public SolverBase {
       reg[0] = dalloc(3 * cs);
       reg[1] = dalloc(3 * cs);
       Q = dalloc(6 * cs);
-      P = dalloc(6 * cs);
+      // P rows padded to whole 128-byte lines: the y passes' 16-kx (8 for
+      // FP64) segments and the x pass's lines then start on line boundaries
+      const long seg = 128 / (long)sizeof(C);
+      prow = (nxh + seg - 1) / seg * seg;
+      pc = prow * ny * nz;
+      P = dalloc(6 * sizeof(C) * pc);
       launch_zero(uk, 3 * cs, stream);
       launch_zero(reg[0], 3 * cs, stream);
       launch_zero(reg[1], 3 * cs, stream);
@@ -282,6 +288,7 @@ class Solver : public SolverBase {
 
   // ------------------------------------------------------------ passes
   C* cf(void* base, int f) { return reinterpret_cast<C*>(base) + (long)f * nc; }
+  C* cfp(int f) { return reinterpret_cast<C*>(P) + (long)f * pc; }  // field f of P
 
   void zinv(const TileJob* jobs, int nj) {  // S -> Q
     TilePassArgs a{};
@@ -298,16 +305,16 @@ class Solver : public SolverBase {
     for (int j = 0; j < nj; ++j) a.jobs[j] = jobs[j];
     a.njobs = nj; a.nb = nxh; a.nc = nz;
     a.sin = Strides{1, (long)nz * ny, ny};
-    a.sout = Strides{nxh, 1, (long)ny * nxh};
+    a.sout = Strides{prow, 1, (long)ny * prow};
     a.kscale = ksy; a.scale = 1.0; a.tw = tw_y;
     timed(TM_YINV, [&] { launch_tile_pass<T>(a, ny, +1, true, stream); });
     kcheck("yinv");
   }
   void yfwd(int nf) {  // P -> Q
     TilePassArgs a{};
-    for (int j = 0; j < nf; ++j) a.jobs[j] = TileJob{cf(P, j), nullptr, cf(Q, j), OP_COPY, nullptr, 0};
+    for (int j = 0; j < nf; ++j) a.jobs[j] = TileJob{cfp(j), nullptr, cf(Q, j), OP_COPY, nullptr, 0};
     a.njobs = nf; a.nb = nxh; a.nc = nz;
-    a.sin = Strides{nxh, 1, (long)ny * nxh};
+    a.sin = Strides{prow, 1, (long)ny * prow};
     a.sout = Strides{1, (long)nz * ny, ny};
     a.kscale = ksy; a.scale = 1.0; a.tw = tw_y;
     timed(TM_YFWD, [&] { launch_tile_pass<T>(a, ny, -1, false, stream); });
@@ -325,7 +332,7 @@ class Solver : public SolverBase {
   }
   XPassArgs xargs() {
     XPassArgs a{};
-    a.nlines = ny * nz; a.nx = nx; a.ny = ny; a.nz = nz; a.nxh = nxh;
+    a.nlines = ny * nz; a.nx = nx; a.ny = ny; a.nz = nz; a.nxh = nxh; a.prow = prow;
     a.kx = kx_d; a.kx_scale = 2.0 * PI / p.xl; a.twM = tw_M; a.twN = tw_N;
     a.npts = npts;
     return a;
@@ -341,10 +348,10 @@ class Solver : public SolverBase {
                      {u + 2 * nc, nullptr, cf(Q, 2), OP_COPY, nullptr, 0}};
     zinv(zj, 3);
     // U_x gives V_x and dy u_x; (U_z, D_y) give V_z and w_x
-    TileJob yj[4] = {{cf(Q, 0), nullptr, cf(P, 0), OP_COPY, cf(P, 5), OP_DERIV},
-                     {cf(Q, 2), cf(Q, 4), cf(P, 2), OP_COPY, cf(P, 3), OP_DERIV_MINUS},
-                     {cf(Q, 1), nullptr, cf(P, 1), OP_COPY, nullptr, 0},
-                     {cf(Q, 3), nullptr, cf(P, 4), OP_COPY, nullptr, 0}};
+    TileJob yj[4] = {{cf(Q, 0), nullptr, cfp(0), OP_COPY, cfp(5), OP_DERIV},
+                     {cf(Q, 2), cf(Q, 4), cfp(2), OP_COPY, cfp(3), OP_DERIV_MINUS},
+                     {cf(Q, 1), nullptr, cfp(1), OP_COPY, nullptr, 0},
+                     {cf(Q, 3), nullptr, cfp(4), OP_COPY, nullptr, 0}};
     yinv(yj, 4);
   }
 
@@ -362,7 +369,7 @@ class Solver : public SolverBase {
     if (p.eps < 1.0e-11) throw FlusiError(77363800, "Value of eps is very small, maybe even zero.");
     inv_uw(u_in);
     XPassArgs a = xargs();
-    a.in = P; a.out = P; a.fstride_in = nc; a.fstride_out = nc; a.nf = 6;
+    a.in = P; a.out = P; a.fstride_in = pc; a.fstride_out = pc; a.nf = 6;
     a.mask = mask; a.us = us; a.eps_inv = 1.0 / p.eps;
     a.sponge = sponge_code();
     a.sponge_thickness = p.sponge_thickness;
@@ -648,8 +655,37 @@ class Solver : public SolverBase {
     const Vec3 wlo{-I.b_bot - sf, -sf, -0.5 * I.WingThickness - sf}, whi{I.b_top + sf, I.L_span + sf, 0.5 * I.WingThickness + sf};
     set_box(1, wlo, whi, &s.Mwr, I.x_pivot_r_b);
     set_box(2, wlo, whi, &s.Mwl, I.x_pivot_l_b);
+
+    // region visited: the parts drawn now and the wings of the previous call
+    // (the only places where wing colours can be left); everything on the
+    // first call and for periodic insects
+    const int nn[3] = {nx, ny, nz};
+    if (!a.use_box || a.full_reset) {
+      for (int i = 0; i < 3; ++i) { a.reg_lo[i] = 0; a.reg_hi[i] = nn[i] - 1; }
+    } else {
+      for (int i = 0; i < 3; ++i) { a.reg_lo[i] = nn[i]; a.reg_hi[i] = -1; }
+      auto include = [&](const int* lo, const int* hi) {
+        for (int i = 0; i < 3; ++i) if (lo[i] > hi[i]) return;  // part outside the grid
+        for (int i = 0; i < 3; ++i) { a.reg_lo[i] = std::min(a.reg_lo[i], lo[i]); a.reg_hi[i] = std::max(a.reg_hi[i], hi[i]); }
+      };
+      if (a.draw_body) {
+        include(a.box_lo[0], a.box_hi[0]);
+        for (int k = 0; k < a.nspheres; ++k) include(a.sph_lo[k], a.sph_hi[k]);
+      }
+      for (int w = 0; w < 2; ++w) {
+        if (a.draw_wing[w]) include(a.box_lo[1 + w], a.box_hi[1 + w]);
+        if (prev_wing_valid[w]) include(prev_wing_lo[w], prev_wing_hi[w]);
+      }
+    }
+    for (int w = 0; w < 2; ++w) {
+      prev_wing_valid[w] = a.draw_wing[w] != 0;
+      for (int i = 0; i < 3; ++i) { prev_wing_lo[w][i] = a.box_lo[1 + w][i]; prev_wing_hi[w][i] = a.box_hi[1 + w][i]; }
+    }
     return a;
   }
+  // index boxes of the wings drawn by the previous insect mask update
+  bool prev_wing_valid[2] = {false, false};
+  int prev_wing_lo[2][3] = {}, prev_wing_hi[2][3] = {};
 
   void get_mask(double* m, double* u_s, int16_t* col) override {
     HIPCHECK(hipStreamSynchronize(stream));
@@ -705,7 +741,7 @@ class Solver : public SolverBase {
     inv_uw(u_in);
     ensure_phys();
     XPassArgs a = xargs();
-    a.in = P; a.out = phys; a.fstride_in = nc; a.fstride_out = npts; a.nf = 6;
+    a.in = P; a.out = phys; a.fstride_in = pc; a.fstride_out = npts; a.nf = 6;
     timed(TM_X, [&] { launch_x_pass<T>(a, XM_INV_UW, stream); });
     kcheck("x inverse uw");
   }
@@ -715,19 +751,19 @@ class Solver : public SolverBase {
     TileJob zj[3], yj[3];
     for (int f = 0; f < nf; ++f) {
       zj[f] = TileJob{s + (long)f * nc, nullptr, cf(Q, f), OP_COPY, nullptr, 0};
-      yj[f] = TileJob{cf(Q, f), nullptr, cf(P, f), OP_COPY, nullptr, 0};
+      yj[f] = TileJob{cf(Q, f), nullptr, cfp(f), OP_COPY, nullptr, 0};
     }
     zinv(zj, nf);
     yinv(yj, nf);
     XPassArgs a = xargs();
-    a.in = P; a.out = dst; a.fstride_in = nc; a.fstride_out = npts; a.nf = nf;
+    a.in = P; a.out = dst; a.fstride_in = pc; a.fstride_out = npts; a.nf = nf;
     timed(TM_X, [&] { launch_x_pass<T>(a, XM_INV, stream); });
     kcheck("x inverse");
   }
   // 1 or 3 physical fields -> S-layout spectral (times 1/N)
   void fft_fields(const void* src, int nf, void* dst) {
     XPassArgs a = xargs();
-    a.in = src; a.out = P; a.fstride_in = npts; a.fstride_out = nc; a.nf = nf;
+    a.in = src; a.out = P; a.fstride_in = npts; a.fstride_out = pc; a.nf = nf;
     timed(TM_X, [&] { launch_x_pass<T>(a, XM_FWD, stream); });
     kcheck("x forward");
     yfwd(nf);
@@ -754,7 +790,7 @@ class Solver : public SolverBase {
     ifft_fields(reg_scratch(), 1, pf(0));
     copy_real_out(pf(0), f, npts);
   }
-  void* reg_scratch() { return cf(P, 3); }  // S-layout scratch outside the RHS pipeline
+  void* reg_scratch() { return cfp(3); }  // S-layout scratch (3 fields) outside the RHS pipeline
 
   // ------------------------------------------------------------ init
   void init_fields() override {

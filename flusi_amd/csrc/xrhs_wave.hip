@@ -150,7 +150,7 @@ __global__ __launch_bounds__(XW_NT) void x_rhs_wave_k(XPassArgs a) {
 
   // ---- loads: six spectra at k = l + qG and k = M
   cplx<T> X[6][4], XM[6];
-  const long lb = (long)(live ? gl : 0) * NXH;
+  const long lb = (long)(live ? gl : 0) * a.prow;
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
     const cplx<T>* p = in + f * a.fstride_in + lb;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(XW_NT) void x_rhs_wave_k(XPassArgs a) {
     for (int q = 0; q < 4; ++q) zm[q] = ln[lp<T>((M - l - q * G) & (M - 1))];
     wave_sync();
     if (live) {
-      cplx<T>* o = out + c * a.fstride_out + (long)gl * NXH;
+      cplx<T>* o = out + c * a.fstride_out + (long)gl * a.prow;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const cplx<T> zk = O[c][q];
