@@ -229,6 +229,7 @@ class Solver : public SolverBase {
     kx_d = tab; ky_d = kx_d + nxh; kz_d = ky_d + ny; kxt2_d = kz_d + nz; kyt2_d = kxt2_d + nxh; kzt2_d = kyt2_d + ny;
     if (fft_ok) {
       tw_z = make_tw(nz, nz); tw_y = make_tw(ny, ny); tw_M = make_tw(nx / 2, nx / 2); tw_N = make_tw(nxh, nx);
+      if (const char* e = getenv("FLUSI_TILE_WAVE")) twave = atoi(e) != 0;
       xwave = x_rhs_wave_supported(nx);
       if (const char* e = getenv("FLUSI_XWAVE")) xwave = xwave && atoi(e) != 0;
       nxblocks = xwave ? x_rhs_wave_blocks(nx, ny * nz) : x_pass_blocks(nx, ny * nz, FP64);
@@ -288,6 +289,11 @@ class Solver : public SolverBase {
 
   // ------------------------------------------------------------ passes
   C* cf(void* base, int f) { return reinterpret_cast<C*>(base) + (long)f * nc; }
+  bool twave = true;        // y/z passes with the line transform inside one wavefront
+  void tile(const TilePassArgs& a, int L, int dir, bool contig_in) {
+    if (twave && tile_wave_supported(L)) launch_tile_wave<T>(a, L, dir, contig_in, stream);
+    else launch_tile_pass<T>(a, L, dir, contig_in, stream);
+  }
   C* cfp(int f) { return reinterpret_cast<C*>(P) + (long)f * pc; }  // field f of P
 
   void zinv(const TileJob* jobs, int nj) {  // S -> Q
@@ -297,7 +303,7 @@ class Solver : public SolverBase {
     a.sin = Strides{1, nz, (long)ny * nz};
     a.sout = Strides{ny, 1, (long)nz * ny};
     a.kscale = ksz; a.scale = 1.0; a.tw = tw_z;
-    timed(TM_ZINV, [&] { launch_tile_pass<T>(a, nz, +1, true, stream); });
+    timed(TM_ZINV, [&] { tile(a, nz, +1, true); });
     kcheck("zinv");
   }
   void yinv(const TileJob* jobs, int nj) {  // Q -> P
@@ -307,7 +313,7 @@ class Solver : public SolverBase {
     a.sin = Strides{1, (long)nz * ny, ny};
     a.sout = Strides{prow, 1, (long)ny * prow};
     a.kscale = ksy; a.scale = 1.0; a.tw = tw_y;
-    timed(TM_YINV, [&] { launch_tile_pass<T>(a, ny, +1, true, stream); });
+    timed(TM_YINV, [&] { tile(a, ny, +1, true); });
     kcheck("yinv");
   }
   void yfwd(int nf) {  // P -> Q
@@ -317,7 +323,7 @@ class Solver : public SolverBase {
     a.sin = Strides{prow, 1, (long)ny * prow};
     a.sout = Strides{1, (long)nz * ny, ny};
     a.kscale = ksy; a.scale = 1.0; a.tw = tw_y;
-    timed(TM_YFWD, [&] { launch_tile_pass<T>(a, ny, -1, false, stream); });
+    timed(TM_YFWD, [&] { tile(a, ny, -1, false); });
     kcheck("yfwd");
   }
   void zfwd_plain(int nf, void* dst) {  // Q -> S with 1/N (cof_p3dfft.f90:52-56)
@@ -327,7 +333,7 @@ class Solver : public SolverBase {
     a.sin = Strides{ny, 1, (long)nz * ny};
     a.sout = Strides{1, nz, (long)ny * nz};
     a.kscale = ksz; a.scale = 1.0 / (double(nx) * double(ny) * double(nz)); a.tw = tw_z;
-    timed(TM_ZSTEP, [&] { launch_tile_pass<T>(a, nz, -1, false, stream); });
+    timed(TM_ZSTEP, [&] { tile(a, nz, -1, false); });
     kcheck("zfwd");
   }
   XPassArgs xargs() {
@@ -347,9 +353,11 @@ class Solver : public SolverBase {
                      {u + nc, nullptr, cf(Q, 1), OP_COPY, cf(Q, 4), OP_DERIV},
                      {u + 2 * nc, nullptr, cf(Q, 2), OP_COPY, nullptr, 0}};
     zinv(zj, 3);
-    // U_x gives V_x and dy u_x; (U_z, D_y) give V_z and w_x
-    TileJob yj[4] = {{cf(Q, 0), nullptr, cfp(0), OP_COPY, cfp(5), OP_DERIV},
-                     {cf(Q, 2), cf(Q, 4), cfp(2), OP_COPY, cfp(3), OP_DERIV_MINUS},
+    // (U_z, D_y) give V_z and w_x; U_x gives V_x and dy u_x.  The two-operand
+    // job goes first: both its reads start with the kernel, the later jobs'
+    // reads are prefetched behind the transforms
+    TileJob yj[4] = {{cf(Q, 2), cf(Q, 4), cfp(2), OP_COPY, cfp(3), OP_DERIV_MINUS},
+                     {cf(Q, 0), nullptr, cfp(0), OP_COPY, cfp(5), OP_DERIV},
                      {cf(Q, 1), nullptr, cfp(1), OP_COPY, nullptr, 0},
                      {cf(Q, 3), nullptr, cfp(4), OP_COPY, nullptr, 0}};
     yinv(yj, 4);

@@ -9,6 +9,8 @@
 #include "fft_core.h"
 #include "flusi_kernels.h"
 #include "pass_common.h"
+#include "wave_fft.h"
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -26,6 +28,176 @@ __host__ __device__ constexpr int mode_group(int mode) {
   return mode == SM_RHS_ONLY ? MG_RHS
        : (mode == SM_EULER || mode == SM_AB2) ? MG_AB2
        : (mode == SM_RK2_1 || mode == SM_RK2_2) ? MG_RK2 : MG_RK4;
+}
+
+// Epilogue at one spectral point (kx, ky, kz): rv[f] is field f of the
+// z-transformed right-hand side (unscaled).  Sponge curl (sponge.f90:22-83),
+// dynamic mean flow, explicit diffusion (rhs.f90:441-468), projection
+// (rhs.f90:575-615), dealiasing and the stepper update
+// (fluid_time_step.f90:519-832), the zero-mode overwrite and the Parseval
+// partial sum (integrals.f90:522-552).
+template <typename T, bool SPONGE, int MG>
+__device__ __forceinline__ void zepilogue(const ZStepArgs& a, int kx, int ky, int kzi, const cplx<T>* rv,
+                                          double kxv, double kxt2, double wE,
+                                          bool final_stage, double& esum, double& ecomp) {
+  cplx<T>* U = reinterpret_cast<cplx<T>*>(a.u);
+  cplx<T>* UST = reinterpret_cast<cplx<T>*>(a.ustage);
+  const cplx<T>* UD = reinterpret_cast<const cplx<T>*>(a.udiff);
+  cplx<T>* ACC = reinterpret_cast<cplx<T>*>(a.acc);
+  cplx<T>* RG0 = reinterpret_cast<cplx<T>*>(a.r0);
+  const cplx<T>* RG1 = reinterpret_cast<const cplx<T>*>(a.r1);
+  cplx<T>* RO = reinterpret_cast<cplx<T>*>(a.rhs_out);
+  const long si = ((long)kx * a.ny + ky) * a.nz + kzi;
+  const double kyv = a.ky[ky], kzv = a.kz[kzi];
+  const double k2 = (kxv * kxv + kyv * kyv) + kzv * kzv;
+  const bool alias = ((kxt2 + a.kyt2[ky]) + a.kzt2[kzi]) >= 1.0;
+  double nr[3], ni[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { nr[c] = (double)rv[c].x * a.scale; ni[c] = (double)rv[c].y * a.scale; }
+  if constexpr (SPONGE) {
+    // sponge.f90:22-83: i k x (w/k^2), zero at k = 0
+    if (k2 != 0.0) {
+      double sr[3], sm[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { sr[c] = (double)rv[3 + c].x * a.scale / k2; sm[c] = (double)rv[3 + c].y * a.scale / k2; }
+      const double cr0 = kyv * sr[2] - kzv * sr[1], ci0 = kyv * sm[2] - kzv * sm[1];
+      const double cr1 = kzv * sr[0] - kxv * sr[2], ci1 = kzv * sm[0] - kxv * sm[2];
+      const double cr2 = kxv * sr[1] - kyv * sr[0], ci2 = kxv * sm[1] - kyv * sm[0];
+      nr[0] += -ci0; ni[0] += cr0;
+      nr[1] += -ci1; ni[1] += cr1;
+      nr[2] += -ci2; ni[2] += cr2;
+    }
+  }
+  if (kx == 0 && ky == 0 && kzi == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (a.meanflow_dynamic[c]) { nr[c] = -a.pen_sum[c] * a.dynamic_scale; ni[c] = 0.0; }
+  }
+  if (a.explicit_diffusion) {
+    // rhs.f90:441-468
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const cplx<T> us_ = UD[c * a.fs + si];
+      nr[c] -= a.nu * k2 * (double)us_.x;
+      ni[c] -= a.nu * k2 * (double)us_.y;
+    }
+  }
+  if (k2 != 0.0) {
+    // rhs.f90:575-615
+    const double qr = (kxv * nr[0] + kyv * nr[1] + kzv * nr[2]) / k2;
+    const double qi = (kxv * ni[0] + kyv * ni[1] + kzv * ni[2]) / k2;
+    nr[0] -= kxv * qr; ni[0] -= kxv * qi;
+    nr[1] -= kyv * qr; ni[1] -= kyv * qi;
+    nr[2] -= kzv * qr; ni[2] -= kzv * qi;
+  }
+  double unr[3] = {0, 0, 0}, uni[3] = {0, 0, 0};
+  const int mode = MG == MG_RHS ? SM_RHS_ONLY : (MG == MG_RK4 && a.mode < SM_RK4_1 ? SM_RK4_1 : a.mode);
+  switch (mode) {
+    case SM_RHS_ONLY:
+      if constexpr (MG == MG_RHS) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) RO[c * a.fs + si] = mk<T>((T)nr[c], (T)ni[c]);
+      }
+      break;
+    case SM_EULER: if constexpr (MG == MG_AB2) {
+      // fluid_time_step.f90:718-727
+      const double E = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const cplx<T> u0 = U[c * a.fs + si];
+        unr[c] = ((double)u0.x + a.dt * nr[c]) * E;
+        uni[c] = ((double)u0.y + a.dt * ni[c]) * E;
+        U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
+        RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
+      }
+      break;
+    }
+    case SM_AB2: if constexpr (MG == MG_AB2) {
+      // fluid_time_step.f90:802-811
+      const double E = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const cplx<T> u0 = U[c * a.fs + si];
+        const cplx<T> n1 = RG1[c * a.fs + si];
+        unr[c] = ((double)u0.x + a.b10 * nr[c] + a.b11 * (double)n1.x) * E;
+        uni[c] = ((double)u0.y + a.b10 * ni[c] + a.b11 * (double)n1.y) * E;
+        U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
+        RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
+      }
+      break;
+    }
+    case SM_RK2_1: if constexpr (MG == MG_RK2) {
+      // fluid_time_step.f90:554-575 (N0 weighted with the IF of the previous dt)
+      const double Eo = vis_factor(a.vis_dt_old, a.nu, k2, a.dealias_on && alias);
+      const double En = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double n0r = nr[c] * Eo, n0i = ni[c] * Eo;
+        const cplx<T> u0 = U[c * a.fs + si];
+        RG0[c * a.fs + si] = mk<T>((T)n0r, (T)n0i);
+        U[c * a.fs + si] = mk<T>((T)((double)u0.x * En + a.dt * n0r), (T)((double)u0.y * En + a.dt * n0i));
+      }
+      break;
+    }
+    case SM_RK2_2: if constexpr (MG == MG_RK2) {
+      // fluid_time_step.f90:594-597
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const cplx<T> u0 = U[c * a.fs + si];
+        const cplx<T> n0 = RG0[c * a.fs + si];
+        unr[c] = (double)u0.x + 0.5 * a.dt * (-(double)n0.x + nr[c]);
+        uni[c] = (double)u0.y + 0.5 * a.dt * (-(double)n0.y + ni[c]);
+      }
+      break;
+    }
+    default: if constexpr (MG == MG_RK4) {
+      // RK4 (fluid_time_step.f90:636-669) with an accumulator register
+      const double kill = alias ? 0.0 : 1.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double kr = nr[c] * kill, ki = ni[c] * kill;
+        if (a.mode == SM_RK4_1) {
+          const cplx<T> u0 = U[c * a.fs + si];
+          ACC[c * a.fs + si] = mk<T>((T)((double)u0.x + (a.dt / 6.0) * kr), (T)((double)u0.y + (a.dt / 6.0) * ki));
+          UST[c * a.fs + si] = mk<T>((T)((double)u0.x + 0.5 * a.dt * kr), (T)((double)u0.y + 0.5 * a.dt * ki));
+        } else if (a.mode == SM_RK4_2 || a.mode == SM_RK4_3) {
+          const cplx<T> u0 = U[c * a.fs + si];
+          const cplx<T> ac = ACC[c * a.fs + si];
+          const double cst = (a.mode == SM_RK4_2) ? 0.5 * a.dt : a.dt;
+          ACC[c * a.fs + si] = mk<T>((T)((double)ac.x + (a.dt / 3.0) * kr), (T)((double)ac.y + (a.dt / 3.0) * ki));
+          UST[c * a.fs + si] = mk<T>((T)((double)u0.x + cst * kr), (T)((double)u0.y + cst * ki));
+        } else {
+          const cplx<T> ac = ACC[c * a.fs + si];
+          unr[c] = ((double)ac.x + (a.dt / 6.0) * kr) * kill;
+          uni[c] = ((double)ac.y + (a.dt / 6.0) * ki) * kill;
+        }
+      }
+      break;
+    }
+  }
+  if (final_stage) {
+    if (kx == 0 && ky == 0 && kzi == 0) {
+      // fluid_time_step.f90:1066-1090 set_mean_flow
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (a.set_mean[c]) { unr[c] = a.mean_value[c]; uni[c] = 0.0; }
+    }
+    if (a.mode == SM_RK2_2 || a.mode == SM_RK4_4 || (kx == 0 && ky == 0 && kzi == 0)) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
+    }
+    // integrals.f90:522-552 Parseval sum of the stored (rounded) field
+    double ev = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double xr = (double)(T)unr[c], xi = (double)(T)uni[c];
+      ev += xr * xr + xi * xi;
+    }
+    ev *= wE;
+    const double t = esum + ev;
+    ecomp += (fabs(esum) >= fabs(ev)) ? (esum - t) + ev : (ev - t) + esum;
+    esum = t;
+  }
 }
 
 template <typename T, int L, int TL, int NT, bool SPONGE, int MG>
@@ -84,13 +256,6 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
   const double kxt2 = a.kxt2[kx];
   const double wE = (kx == 0 || kx == a.nx / 2) ? 0.5 : 1.0;
   double esum = 0.0, ecomp = 0.0;
-  cplx<T>* U = reinterpret_cast<cplx<T>*>(a.u);
-  cplx<T>* UST = reinterpret_cast<cplx<T>*>(a.ustage);
-  const cplx<T>* UD = reinterpret_cast<const cplx<T>*>(a.udiff);
-  cplx<T>* ACC = reinterpret_cast<cplx<T>*>(a.acc);
-  cplx<T>* RG0 = reinterpret_cast<cplx<T>*>(a.r0);
-  const cplx<T>* RG1 = reinterpret_cast<const cplx<T>*>(a.r1);
-  cplx<T>* RO = reinterpret_cast<cplx<T>*>(a.rhs_out);
   const bool final_stage = MG != MG_RHS && (a.mode == SM_EULER || a.mode == SM_AB2 || a.mode == SM_RK2_2 || a.mode == SM_RK4_4);
 #pragma unroll
   for (int p = 0; p < PERL; ++p)
@@ -102,161 +267,109 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
     const int kzi = j + q * (L / RL);
     const int ky = b0 + bb;
     if (ky >= a.ny) continue;
-    const long si = ((long)kx * a.ny + ky) * a.nz + kzi;
-    const double kyv = a.ky[ky], kzv = a.kz[kzi];
-    const double k2 = (kxv * kxv + kyv * kyv) + kzv * kzv;
-    const bool alias = ((kxt2 + a.kyt2[ky]) + a.kzt2[kzi]) >= 1.0;
-    double nr[3], ni[3];
+    cplx<T> rv[NF];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { nr[c] = (double)r[c][p][q].x * a.scale; ni[c] = (double)r[c][p][q].y * a.scale; }
-    if constexpr (SPONGE) {
-      // sponge.f90:22-83: i k x (w/k^2), zero at k = 0
-      if (k2 != 0.0) {
-        double sr[3], sm[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { sr[c] = (double)r[3 + c][p][q].x * a.scale / k2; sm[c] = (double)r[3 + c][p][q].y * a.scale / k2; }
-        const double cr0 = kyv * sr[2] - kzv * sr[1], ci0 = kyv * sm[2] - kzv * sm[1];
-        const double cr1 = kzv * sr[0] - kxv * sr[2], ci1 = kzv * sm[0] - kxv * sm[2];
-        const double cr2 = kxv * sr[1] - kyv * sr[0], ci2 = kxv * sm[1] - kyv * sm[0];
-        nr[0] += -ci0; ni[0] += cr0;
-        nr[1] += -ci1; ni[1] += cr1;
-        nr[2] += -ci2; ni[2] += cr2;
-      }
-    }
-    if (kx == 0 && ky == 0 && kzi == 0) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        if (a.meanflow_dynamic[c]) { nr[c] = -a.pen_sum[c] * a.dynamic_scale; ni[c] = 0.0; }
-    }
-    if (a.explicit_diffusion) {
-      // rhs.f90:441-468
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const cplx<T> us_ = UD[c * a.fs + si];
-        nr[c] -= a.nu * k2 * (double)us_.x;
-        ni[c] -= a.nu * k2 * (double)us_.y;
-      }
-    }
-    if (k2 != 0.0) {
-      // rhs.f90:575-615
-      const double qr = (kxv * nr[0] + kyv * nr[1] + kzv * nr[2]) / k2;
-      const double qi = (kxv * ni[0] + kyv * ni[1] + kzv * ni[2]) / k2;
-      nr[0] -= kxv * qr; ni[0] -= kxv * qi;
-      nr[1] -= kyv * qr; ni[1] -= kyv * qi;
-      nr[2] -= kzv * qr; ni[2] -= kzv * qi;
-    }
-    double unr[3] = {0, 0, 0}, uni[3] = {0, 0, 0};
-    const int mode = MG == MG_RHS ? SM_RHS_ONLY : (MG == MG_RK4 && a.mode < SM_RK4_1 ? SM_RK4_1 : a.mode);
-    switch (mode) {
-      case SM_RHS_ONLY:
-        if constexpr (MG == MG_RHS) {
-#pragma unroll
-          for (int c = 0; c < 3; ++c) RO[c * a.fs + si] = mk<T>((T)nr[c], (T)ni[c]);
-        }
-        break;
-      case SM_EULER: if constexpr (MG == MG_AB2) {
-        // fluid_time_step.f90:718-727
-        const double E = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const cplx<T> u0 = U[c * a.fs + si];
-          unr[c] = ((double)u0.x + a.dt * nr[c]) * E;
-          uni[c] = ((double)u0.y + a.dt * ni[c]) * E;
-          U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-          RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
-        }
-        break;
-      }
-      case SM_AB2: if constexpr (MG == MG_AB2) {
-        // fluid_time_step.f90:802-811
-        const double E = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const cplx<T> u0 = U[c * a.fs + si];
-          const cplx<T> n1 = RG1[c * a.fs + si];
-          unr[c] = ((double)u0.x + a.b10 * nr[c] + a.b11 * (double)n1.x) * E;
-          uni[c] = ((double)u0.y + a.b10 * ni[c] + a.b11 * (double)n1.y) * E;
-          U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-          RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
-        }
-        break;
-      }
-      case SM_RK2_1: if constexpr (MG == MG_RK2) {
-        // fluid_time_step.f90:554-575 (N0 weighted with the IF of the previous dt)
-        const double Eo = vis_factor(a.vis_dt_old, a.nu, k2, a.dealias_on && alias);
-        const double En = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double n0r = nr[c] * Eo, n0i = ni[c] * Eo;
-          const cplx<T> u0 = U[c * a.fs + si];
-          RG0[c * a.fs + si] = mk<T>((T)n0r, (T)n0i);
-          U[c * a.fs + si] = mk<T>((T)((double)u0.x * En + a.dt * n0r), (T)((double)u0.y * En + a.dt * n0i));
-        }
-        break;
-      }
-      case SM_RK2_2: if constexpr (MG == MG_RK2) {
-        // fluid_time_step.f90:594-597
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const cplx<T> u0 = U[c * a.fs + si];
-          const cplx<T> n0 = RG0[c * a.fs + si];
-          unr[c] = (double)u0.x + 0.5 * a.dt * (-(double)n0.x + nr[c]);
-          uni[c] = (double)u0.y + 0.5 * a.dt * (-(double)n0.y + ni[c]);
-        }
-        break;
-      }
-      default: if constexpr (MG == MG_RK4) {
-        // RK4 (fluid_time_step.f90:636-669) with an accumulator register
-        const double kill = alias ? 0.0 : 1.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double kr = nr[c] * kill, ki = ni[c] * kill;
-          if (a.mode == SM_RK4_1) {
-            const cplx<T> u0 = U[c * a.fs + si];
-            ACC[c * a.fs + si] = mk<T>((T)((double)u0.x + (a.dt / 6.0) * kr), (T)((double)u0.y + (a.dt / 6.0) * ki));
-            UST[c * a.fs + si] = mk<T>((T)((double)u0.x + 0.5 * a.dt * kr), (T)((double)u0.y + 0.5 * a.dt * ki));
-          } else if (a.mode == SM_RK4_2 || a.mode == SM_RK4_3) {
-            const cplx<T> u0 = U[c * a.fs + si];
-            const cplx<T> ac = ACC[c * a.fs + si];
-            const double cst = (a.mode == SM_RK4_2) ? 0.5 * a.dt : a.dt;
-            ACC[c * a.fs + si] = mk<T>((T)((double)ac.x + (a.dt / 3.0) * kr), (T)((double)ac.y + (a.dt / 3.0) * ki));
-            UST[c * a.fs + si] = mk<T>((T)((double)u0.x + cst * kr), (T)((double)u0.y + cst * ki));
-          } else {
-            const cplx<T> ac = ACC[c * a.fs + si];
-            unr[c] = ((double)ac.x + (a.dt / 6.0) * kr) * kill;
-            uni[c] = ((double)ac.y + (a.dt / 6.0) * ki) * kill;
-          }
-        }
-        break;
-      }
-    }
-    if (final_stage) {
-      if (kx == 0 && ky == 0 && kzi == 0) {
-        // fluid_time_step.f90:1066-1090 set_mean_flow
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          if (a.set_mean[c]) { unr[c] = a.mean_value[c]; uni[c] = 0.0; }
-      }
-      if (a.mode == SM_RK2_2 || a.mode == SM_RK4_4 || (kx == 0 && ky == 0 && kzi == 0)) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-      }
-      // integrals.f90:522-552 Parseval sum of the stored (rounded) field
-      double ev = 0.0;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double xr = (double)(T)unr[c], xi = (double)(T)uni[c];
-        ev += xr * xr + xi * xi;
-      }
-      ev *= wE;
-      const double t = esum + ev;
-      ecomp += (fabs(esum) >= fabs(ev)) ? (esum - t) + ev : (ev - t) + esum;
-      esum = t;
-    }
+    for (int f = 0; f < NF; ++f) rv[f] = r[f][p][q];
+    zepilogue<T, SPONGE, MG>(a, kx, ky, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
   }
   if (final_stage && a.energy_partial) {
     double v = wave_sum(esum + ecomp);
     if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < NT / 64; ++w) s += red[w];
+      a.energy_partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = s;
+    }
+  }
+}
+
+// The same pass with each z line transformed inside one wavefront (nz = 64,
+// 128, 256, 512): per field, TL lines' 128-byte ky segments are staged
+// through the LDS tile (one block-wide wait each way), every wave transforms
+// its own line in registers, and the next field's loads are in flight
+// meanwhile.  A lane ends with positions kz = l + qG of all fields of its
+// line, where the epilogue runs.
+template <typename T, int L> struct ZWShape {
+  static constexpr int TL = Tile<T>::TL;
+  static constexpr int NW = TL / wavefft::TWPlan<L>::LPW;
+  static constexpr int NT = NW * 64;
+  static constexpr int LS = line_stride<T>(L);
+  static constexpr int K = TL * L / NT;
+  static constexpr bool TWL = 2 * (TL * LS + L) * (int)sizeof(cplx<T>) <= 160 * 1024;
+  static constexpr size_t smem = (size_t)(TL * LS + (TWL ? L : 0)) * sizeof(cplx<T>);
+};
+
+template <typename T, int L, bool SPONGE, int MG>
+__global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a) {
+  using namespace wavefft;
+  using Sh = ZWShape<T, L>;
+  constexpr int TL = Sh::TL, NT = Sh::NT, LS = Sh::LS, K = Sh::K;
+  constexpr int G = TWPlan<L>::G;
+  constexpr int NF = SPONGE ? 6 : 3;
+  extern __shared__ char smem_raw[];
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
+  __shared__ double red[NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int line = wv * TWPlan<L>::LPW + lane / G, l = lane % G;
+  cplx<T>* ln = tile + line * LS;
+  const int b0 = blockIdx.x * TL;
+  const int kx = blockIdx.y;
+  const cplx<T>* tw;
+  if constexpr (Sh::TWL) {
+    cplx<T>* twl = tile + TL * LS;
+    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+    for (int i = tid; i < L; i += NT) twl[i] = twg[i];
+    tw = twl;
+  } else {
+    tw = reinterpret_cast<const cplx<T>*>(a.tw);
+  }
+  const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
+  cplx<T> x[K];
+  auto load = [&](int f) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = tid + k * NT, bb = e % TL, aa = e / TL;
+      const bool ok = b0 + bb < a.ny;
+      x[k] = ok ? nq[f * a.fs + ((long)kx * a.nz + aa) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
+    }
+  };
+  load(0);
+  cplx<T> r[NF][8];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = tid + k * NT, bb = e % TL, aa = e / TL;
+      tile[bb * LS + lp<T>(aa)] = x[k];
+    }
+    __syncthreads();
+    if (f + 1 < NF) load(f + 1);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r[f][q] = ln[lp<T>(l + q * G)];
+    wave_sync();
+    twfft<T, L, -1>(r[f], tw, ln, l);
+    __syncthreads();
+  }
+
+  const double kxv = a.kx[kx];
+  const double kxt2 = a.kxt2[kx];
+  const double wE = (kx == 0 || kx == a.nx / 2) ? 0.5 : 1.0;
+  double esum = 0.0, ecomp = 0.0;
+  const bool final_stage = MG != MG_RHS && (a.mode == SM_EULER || a.mode == SM_AB2 || a.mode == SM_RK2_2 || a.mode == SM_RK4_4);
+  const int ky = b0 + line;
+  if (ky < a.ny) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      cplx<T> rv[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) rv[f] = r[f][q];
+      zepilogue<T, SPONGE, MG>(a, kx, ky, l + q * G, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
+    }
+  }
+  if (final_stage && a.energy_partial) {
+    double v = wave_sum(esum + ecomp);
+    if (lane == 0) red[wv] = v;
     __syncthreads();
     if (tid == 0) {
       double s = 0.0;
@@ -271,8 +384,23 @@ int zstep_blocks(int nz, int ny, int nxh, bool fp64) {
   return ((ny + TL - 1) / TL) * nxh;
 }
 
+static bool zwave_enabled() {
+  static int v = [] { const char* e = getenv("FLUSI_ZWAVE"); return e ? atoi(e) : 0; }();
+  return v != 0;
+}
+
 template <typename T, int L, bool SP, int MG>
 static void run_z(const ZStepArgs& a, hipStream_t s) {
+  if constexpr (L == 64 || L == 128 || L == 256 || L == 512) {
+    if (zwave_enabled()) {
+      using Sh = ZWShape<T, L>;
+      static bool winit = false;
+      if (!winit) { set_smem(zstep_wave_k<T, L, SP, MG>, Sh::smem); winit = true; }
+      dim3 grid((a.ny + Sh::TL - 1) / Sh::TL, a.nxh);
+      hipLaunchKernelGGL((zstep_wave_k<T, L, SP, MG>), grid, dim3(Sh::NT), Sh::smem, s, a);
+      return;
+    }
+  }
   constexpr int TL = Tile<T>::TL;
   constexpr int NT = pick_nt(TL * L);
   const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);

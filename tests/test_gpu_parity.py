@@ -22,7 +22,8 @@ def _base_ini(tmp_path, **kw):
     return rewrite_ini(os.path.join(GOLDEN, "vortex_ring1_AB2", "vortexring1_AB2.ini"), str(tmp_path), **defaults)
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 24, 64), (16, 96, 16), (128, 64, 48), (512, 32, 16)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 24, 64), (16, 96, 16), (128, 64, 48), (512, 32, 16),
+                                   (64, 128, 256), (32, 512, 64)])
 @pytest.mark.parametrize("prec,tol", PRECS)
 def test_fft_ifft_match_oracle(tmp_path, shape, prec, tol):
     nx, ny, nz = shape
