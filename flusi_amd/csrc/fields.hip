@@ -288,87 +288,128 @@ void launch_set_mode0(const GridGeom& g, void* uk, int comp, double re, double i
 static constexpr int DIAG_NT = 256;
 static constexpr int DIAG_PPT = 8;  // points per thread
 
+// Colour-resolved sums (force, torques, residual) only receive contributions
+// where mask > 0: a wave adds them, reduced over its lanes in a fixed tree, to
+// its own LDS row only for point batches that touch the mask.  The other sums
+// stay in registers.  All reductions have a fixed order (reproducible).
+constexpr int DIAG_NCOL = 40;  // DG_FORCE .. DG_TORQUE at 0..35, the 4 DG_URES minima at 36..39
+static_assert(DG_FORCE == 0 && DG_TORQUE0 == 12 && DG_TORQUE == 24 && DG_POWER == 36, "colour slots");
+
+__device__ __forceinline__ double wred(double x, int kind) {  // 0 sum, 1 max, 2 min
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double y = __shfl_xor(x, o, 64);
+    x = kind == 2 ? fmin(x, y) : (kind == 1 ? fmax(x, y) : x + y);
+  }
+  return x;
+}
+
 template <typename T>
 __global__ __launch_bounds__(DIAG_NT) void diag_k(GridGeom g, DiagArgs a) {
   const T* u = (const T*)a.u; const T* w = (const T*)a.w; const T* dv = (const T*)a.div;
   const T* mask = (const T*)a.mask; const T* us = (const T*)a.us;
   const long n = g.npts;
-  double acc[DIAG_NV];
-#pragma unroll
-  for (int v = 0; v < DIAG_NV; ++v) acc[v] = 0.0;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) acc[DG_URES + c] = 9.0e9;
-  acc[DG_DIVMAX] = -1.0e300; acc[DG_DIVMAXF] = -1.0e300;
+  constexpr int NW = DIAG_NT / 64;
+  __shared__ double colacc[NW][DIAG_NCOL];
+  __shared__ double red[NW][DIAG_NV];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (lane < DIAG_NCOL) colacc[wv][lane] = (lane >= 36) ? 9.0e9 : 0.0;
+  // per-thread sums of the colour-independent quantities
+  double pw[3] = {0, 0, 0}, ek[3] = {0, 0, 0}, ekf[3] = {0, 0, 0}, ds[3] = {0, 0, 0}, dsf[3] = {0, 0, 0},
+         mus[3] = {0, 0, 0};
+  double dmax = -1.0e300, dmaxf = -1.0e300, mvol = 0.0;
   const long base = ((long)blockIdx.x * DIAG_NT) * DIAG_PPT + threadIdx.x;
   for (int q = 0; q < DIAG_PPT; ++q) {
     const long p = base + (long)q * DIAG_NT;
-    if (p >= n) break;
-    const int ix = p % g.nx, iy = (p / g.nx) % g.ny, iz = p / ((long)g.nx * g.ny);
-    const double m = (double)mask[p];
-    const double uu[3] = {(double)u[p], (double)u[n + p], (double)u[2 * n + p]};
-    const double ww[3] = {(double)w[p], (double)w[n + p], (double)w[2 * n + p]};
-    const double ss[3] = {(double)us[p], (double)us[n + p], (double)us[2 * n + p]};
+    const bool live = p < n;
+    const long pp = live ? p : 0;
+    const double m = live ? (double)mask[pp] : 0.0;
+    const double uu[3] = {(double)u[pp], (double)u[n + pp], (double)u[2 * n + pp]};
+    const double ww[3] = {(double)w[pp], (double)w[n + pp], (double)w[2 * n + pp]};
+    const double ss[3] = {(double)us[pp], (double)us[n + pp], (double)us[2 * n + pp]};
     const double du[3] = {uu[0] - ss[0], uu[1] - ss[1], uu[2] - ss[2]};
     const double pen[3] = {-m * du[0] * a.eps_inv, -m * du[1] * a.eps_inv, -m * du[2] * a.eps_inv};
-    const int col = color_clamp(a.color[p]);
-    double lev[3] = {ix * g.dx - a.x0[0], iy * g.dy - a.x0[1], iz * g.dz - a.x0[2]};
-    periodize_d(lev, g);
-    const double t0[3] = {-(lev[1] * pen[2] - lev[2] * pen[1]), -(lev[2] * pen[0] - lev[0] * pen[2]),
-                          -(lev[0] * pen[1] - lev[1] * pen[0])};
-    if (a.insect && (col == 2 || col == 3)) {
-      const double* pv = (col == 2) ? a.piv_l : a.piv_r;
-      lev[0] -= pv[0]; lev[1] -= pv[1]; lev[2] -= pv[2];
-      periodize_d(lev, g);
-    }
-    const double tp[3] = {-(lev[1] * pen[2] - lev[2] * pen[1]), -(lev[2] * pen[0] - lev[0] * pen[2]),
-                          -(lev[0] * pen[1] - lev[1] * pen[0])};
-    const double d2 = du[0] * du[0] + du[1] * du[1] + du[2] * du[2];
+    if (live) {
+      const double fm = 1.0 - m;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (col == c) {
+      for (int i = 0; i < 3; ++i) {
+        pw[i] += ss[i] * pen[i];
+        ek[i] += uu[i] * uu[i];
+        ekf[i] += (uu[i] * fm) * (uu[i] * fm);
+        ds[i] += ww[i] * ww[i];
+        dsf[i] += (ww[i] * fm) * (ww[i] * fm);
+        mus[i] += m * ss[i];
+      }
+      const double d = (double)dv[pp];
+      dmax = fmax(dmax, d);
+      dmaxf = fmax(dmaxf, d * fm);
+      mvol += m;
+    }
+    if (__any(m > 0.0)) {
+      const int ix = (int)(pp % g.nx), iy = (int)((pp / g.nx) % g.ny), iz = (int)(pp / ((long)g.nx * g.ny));
+      const int col = live ? color_clamp(a.color[pp]) : 4;
+      double lev[3] = {ix * g.dx - a.x0[0], iy * g.dy - a.x0[1], iz * g.dz - a.x0[2]};
+      periodize_d(lev, g);
+      const double t0[3] = {-(lev[1] * pen[2] - lev[2] * pen[1]), -(lev[2] * pen[0] - lev[0] * pen[2]),
+                            -(lev[0] * pen[1] - lev[1] * pen[0])};
+      if (a.insect && (col == 2 || col == 3)) {
+        const double* pv = (col == 2) ? a.piv_l : a.piv_r;
+        lev[0] -= pv[0]; lev[1] -= pv[1]; lev[2] -= pv[2];
+        periodize_d(lev, g);
+      }
+      const double tp[3] = {-(lev[1] * pen[2] - lev[2] * pen[1]), -(lev[2] * pen[0] - lev[0] * pen[2]),
+                            -(lev[0] * pen[1] - lev[1] * pen[0])};
+      const double d2 = du[0] * du[0] + du[1] * du[1] + du[2] * du[2];
+      for (int c = 0; c < 4; ++c) {
+        const bool on = (col == c);
+        if (!__any(on)) continue;
+        double r[10];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          acc[DG_FORCE + 3 * c + i] -= pen[i];
-          acc[DG_TORQUE0 + 3 * c + i] += t0[i];
-          acc[DG_TORQUE + 3 * c + i] += tp[i];
+          r[i] = wred(on ? -pen[i] : 0.0, 0);
+          r[3 + i] = wred(on ? t0[i] : 0.0, 0);
+          r[6 + i] = wred(on ? tp[i] : 0.0, 0);
         }
-        if (m > 0.0) acc[DG_URES + c] = fmin(acc[DG_URES + c], d2);
+        r[9] = wred((on && m > 0.0) ? d2 : 9.0e9, 2);
+        if (lane == 0) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            colacc[wv][DG_FORCE + 3 * c + i] += r[i];
+            colacc[wv][DG_TORQUE0 + 3 * c + i] += r[3 + i];
+            colacc[wv][DG_TORQUE + 3 * c + i] += r[6 + i];
+          }
+          colacc[wv][36 + c] = fmin(colacc[wv][36 + c], r[9]);  // residual slots 36..39
+        }
       }
     }
-    const double fm = 1.0 - m;
+  }
+  // wave totals of the register sums
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    pw[i] = wred(pw[i], 0); ek[i] = wred(ek[i], 0); ekf[i] = wred(ekf[i], 0);
+    ds[i] = wred(ds[i], 0); dsf[i] = wred(dsf[i], 0); mus[i] = wred(mus[i], 0);
+  }
+  dmax = wred(dmax, 1); dmaxf = wred(dmaxf, 1); mvol = wred(mvol, 0);
+  if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      acc[DG_POWER + i] += ss[i] * pen[i];
-      acc[DG_EK + i] += uu[i] * uu[i];
-      acc[DG_EKF + i] += (uu[i] * fm) * (uu[i] * fm);
-      acc[DG_DISS + i] += ww[i] * ww[i];
-      acc[DG_DISSF + i] += (ww[i] * fm) * (ww[i] * fm);
-      acc[DG_MASKUS + i] += m * ss[i];
+      red[wv][DG_POWER + i] = pw[i]; red[wv][DG_EK + i] = ek[i]; red[wv][DG_EKF + i] = ekf[i];
+      red[wv][DG_DISS + i] = ds[i]; red[wv][DG_DISSF + i] = dsf[i]; red[wv][DG_MASKUS + i] = mus[i];
     }
-    const double d = (double)dv[p];
-    acc[DG_DIVMAX] = fmax(acc[DG_DIVMAX], d);
-    acc[DG_DIVMAXF] = fmax(acc[DG_DIVMAXF], d * fm);
-    acc[DG_MASKVOL] += m;
+    red[wv][DG_DIVMAX] = dmax; red[wv][DG_DIVMAXF] = dmaxf; red[wv][DG_MASKVOL] = mvol;
   }
-  __shared__ double red[DIAG_NT / 64];
-  const int tid = threadIdx.x;
-  for (int v = 0; v < DIAG_NV; ++v) {
+  __syncthreads();
+  if (tid < DIAG_NV) {
+    const int v = tid;
+    const bool colv = v < 36 || (v >= DG_URES && v < DG_URES + 4);
     const bool is_min = (v >= DG_URES && v < DG_URES + 4);
     const bool is_max = (v == DG_DIVMAX || v == DG_DIVMAXF);
-    double x = acc[v];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double y = __shfl_xor(x, o, 64);
-      x = is_min ? fmin(x, y) : (is_max ? fmax(x, y) : x + y);
+    double r = colv ? colacc[0][v < 36 ? v : 36 + (v - DG_URES)] : red[0][v];
+    for (int i = 1; i < NW; ++i) {
+      const double x = colv ? colacc[i][v < 36 ? v : 36 + (v - DG_URES)] : red[i][v];
+      r = is_min ? fmin(r, x) : (is_max ? fmax(r, x) : r + x);
     }
-    if ((tid & 63) == 0) red[tid >> 6] = x;
-    __syncthreads();
-    if (tid == 0) {
-      double r = red[0];
-      for (int i = 1; i < DIAG_NT / 64; ++i) r = is_min ? fmin(r, red[i]) : (is_max ? fmax(r, red[i]) : r + red[i]);
-      a.partial[(long)blockIdx.x * DIAG_NV + v] = r;
-    }
-    __syncthreads();
+    a.partial[(long)blockIdx.x * DIAG_NV + v] = r;
   }
 }
 

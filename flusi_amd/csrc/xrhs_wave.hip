@@ -18,6 +18,7 @@
 //   per block: max |u|^2 and the compensated sums of mask/eta (u - us)
 #include "fft_core.h"
 #include "flusi_kernels.h"
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -123,8 +124,9 @@ __device__ __forceinline__ void wfft(cplx<T> (&v)[4], const cplx<T>* tw, cplx<T>
   if constexpr (WPlan<M>::NST > 3) wstage<T, M, S, 3>(v, tw, ln, l);
 }
 
-template <typename T, int M, bool SPONGE, bool STATS>
-__global__ __launch_bounds__(XW_NT) void x_rhs_wave_k(XPassArgs a) {
+// WPE: minimum waves per SIMD the register allocation must allow (0: none)
+template <typename T, int M, bool SPONGE, bool STATS, int WPE>
+__global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1))) void x_rhs_wave_k(XPassArgs a) {
   constexpr int G = M / 4, LPW = 64 / G, NXH = M + 1;
   constexpr int LS = line_stride<T>(M + 1);
   constexpr int WPB = XW_NT / 64;
@@ -318,11 +320,14 @@ void run_wave(const XPassArgs& a, hipStream_t s) {
   constexpr int LPB = (XW_NT / 64) * (256 / M);  // lines per block
   const int blocks = (a.nlines + LPB - 1) / LPB;
   if (a.sponge) {
-    if (a.want_stats) hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, true>), dim3(blocks), dim3(XW_NT), 0, s, a);
-    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, false>), dim3(blocks), dim3(XW_NT), 0, s, a);
+    if (a.want_stats) hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, true, 0>), dim3(blocks), dim3(XW_NT), 0, s, a);
+    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, false, 0>), dim3(blocks), dim3(XW_NT), 0, s, a);
+  } else if (a.want_stats) {
+    hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, true, 0>), dim3(blocks), dim3(XW_NT), 0, s, a);
   } else {
-    if (a.want_stats) hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, true>), dim3(blocks), dim3(XW_NT), 0, s, a);
-    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false>), dim3(blocks), dim3(XW_NT), 0, s, a);
+    static const int wpe = [] { const char* e = getenv("FLUSI_XWAVE_WPE"); return e ? atoi(e) : 3; }();
+    if (wpe == 3) hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false, 3>), dim3(blocks), dim3(XW_NT), 0, s, a);
+    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false, 0>), dim3(blocks), dim3(XW_NT), 0, s, a);
   }
 }
 
