@@ -589,6 +589,94 @@ class WingKinematics:
         raise FlusiAbort(10624, f"flapping motion protocol {pr} not in scope")
 
 
+def bumblebee_capsules():
+    """Legs, antennae and proboscis of the bumblebee body
+    (body_geometry.f90:277-345): (a, b, radius) in the body frame.  The
+    coordinates are Fortran default-real literals and the radii products of
+    default reals (single precision), stored as doubles."""
+    f = np.float32
+    def arr(v):
+        return np.array(v, dtype=np.float32).astype(np.float64)
+    xl1, yl1, zl1 = arr([-0.74, -0.63, -0.4, -0.1, 0.1]), arr([0.32, 0.32, 0.31, 0.3, 0.12]), arr([-0.35, -0.37, -0.2, -0.1, -0.16])
+    rl1 = (np.array([0.015, 0.03, 0.04, 0.03], dtype=np.float32) * f(1.3)).astype(np.float64)
+    xl2, yl2, zl2 = arr([-0.24, -0.15, 0.02, 0.17, 0.19]), arr([0.33, 0.33, 0.32, 0.3, 0.15]), arr([-0.29, -0.28, -0.2, -0.15, -0.19])
+    rl2 = (np.array([0.015, 0.03, 0.04, 0.03], dtype=np.float32) * f(1.3)).astype(np.float64)
+    xl3, yl3, zl3 = arr([0.28, 0.35, 0.45, 0.4, 0.35]), arr([0.31, 0.30, 0.28, 0.2, 0.15]), arr([-0.3, -0.28, -0.25, -0.18, -0.18])
+    rl3 = (np.array([0.015, 0.02, 0.03, 0.02], dtype=np.float32) * f(1.3)).astype(np.float64)
+    xf, yf, zf = arr([0.43, 0.6]), arr([0.0, 0.0]), arr([-0.28, -0.23])
+    rf = float(f(0.017) * f(1.3))
+    xan, yan, zan = arr([0.63, 0.8]), arr([0.05, 0.27]), arr([-0.03, 0.1])
+    ran = float(f(0.015) * f(1.3))
+    out = []
+    for j in range(4):
+        for (xl, yl, zl, rl) in ((xl1, yl1, zl1, rl1), (xl2, yl2, zl2, rl2), (xl3, yl3, zl3, rl3)):
+            out.append((np.array([xl[j], yl[j], zl[j]]), np.array([xl[j + 1], yl[j + 1], zl[j + 1]]), float(rl[j])))
+        for (xl, yl, zl, rl) in ((xl1, yl1, zl1, rl1), (xl2, yl2, zl2, rl2), (xl3, yl3, zl3, rl3)):
+            out.append((np.array([xl[j], -yl[j], zl[j]]), np.array([xl[j + 1], -yl[j + 1], zl[j + 1]]), float(rl[j])))
+    out.append((np.array([xan[0], yan[0], zan[0]]), np.array([xan[1], yan[1], zan[1]]), ran))
+    out.append((np.array([xan[0], -yan[0], zan[0]]), np.array([xan[1], -yan[1], zan[1]]), ran))
+    out.append((np.array([xf[0], yf[0], zf[0]]), np.array([xf[1], yf[1], zf[1]]), rf))
+    return out
+
+
+class FourierWing:
+    """Wing outline R(theta) given by a Fourier series in the normalised angle
+    (wings_geometry.f90:819-858 Radius_Fourier, :910-1321 the hard-coded
+    coefficients, :1438-1511 the bounding box)."""
+    N_RADIUS = 25000
+
+    def __init__(self, shape: str, thickness: float):
+        if shape != "bumblebee":
+            raise FlusiAbort(554329, "Fourier wing shape not in scope: " + shape)
+        # wings_geometry.f90:1115-1144
+        self.a0 = 0.594557593733011
+        self.ai = np.array([-0.0128037920989526, -0.106777418654552, 0.0380851289321982,
+                            0.0330548081081197, -0.0178496286355627, -0.00328588543359649,
+                            0.0108246924336137, -0.00489302388329943, -0.00708808441172961,
+                            0.00518244772891516, 0.00445979960844562, -0.000108072056165527,
+                            0.00204437845603716, 0.00147176382618797, -0.00229559463098105,
+                            -0.000514633972391526, 0.00134150515430486, -0.000149860228261824,
+                            9.01456938813568e-05, 0.00150639712261487, 0.000914624010720407,
+                            -0.000737650894315551, -0.000843849923321745, -0.000354971670482499,
+                            -0.000382956472432449])
+        self.bi = np.array([-0.0158061138788171, 0.0308243584184200, -0.00903330410923372,
+                            -0.0185758334697500, -0.000924452934252486, -0.00242101213359519,
+                            -0.00204549530064489, 0.00291468131401423, -0.000140755032337495,
+                            -0.00135036427128534, 0.00141285439042451, -0.000334215276598231,
+                            -0.00161521722061879, -0.000164055684312904, -0.000256278551727569,
+                            -0.000740258481681094, 0.000847498161852221, 0.00157442110960973,
+                            -0.000559835622451578, -0.000617498559228280, 0.00115413452523474,
+                            0.000322564770099778, -0.000917375185844477, 4.44819399488798e-05,
+                            0.000710028654602170])
+        self.xc = -0.1
+        self.yc = 0.501549263807117
+        # radius table on nodes (j-1)*dphi, dphi = 2 pi/(n-1), the normalised angle
+        n = self.N_RADIUS
+        self.dphi = (2.0 * PI) / float(n - 1)
+        th = np.arange(n, dtype=np.float64) * self.dphi
+        R = np.full(n, self.a0 / 2.0)
+        for i in range(1, len(self.ai) + 1):
+            R = R + self.ai[i - 1] * np.cos(2.0 * PI * float(i) * th) + self.bi[i - 1] * np.sin(2.0 * PI * float(i) * th)
+        self.table = R
+        # bounding box: theta = 0, 1e-3, ... (accumulated) below 1
+        xmin = ymin = 999.0e9
+        xmax = ymax = -999.0e9
+        theta = 0.0
+        while theta < 1.0:
+            r = float(self.radius(np.array([theta]))[0])
+            tp = 2.0 * PI * theta - PI
+            xx = self.xc + r * math.cos(tp)
+            yy = self.yc + r * math.sin(tp)
+            xmin, ymin, xmax, ymax = min(xmin, xx), min(ymin, yy), max(xmax, xx), max(ymax, yy)
+            theta = theta + 1.0e-3
+        self.bbox = (xmin, xmax, ymin, ymax, -thickness / 2.0, thickness / 2.0)
+
+    def radius(self, theta):
+        j0 = np.floor(theta / self.dphi).astype(np.int64)
+        t = self.table
+        return t[j0] + ((theta - j0.astype(np.float64) * self.dphi) / self.dphi) * (t[j0 + 1] - t[j0])
+
+
 class Insect:
     """Tethered insect state (module_insects.f90:89-227 type diptera) with
     insect_init (insect_init_clean.f90:1-250)."""
@@ -607,6 +695,8 @@ class Insect:
         self.kine_r = self._kine(fr, infile)
         self.kine_l = self._kine(fl, infile)
         self.BodyType = ini.str("Insects", "BodyType", "ellipsoid")
+        self.HasDetails = ini.str("Insects", "HasDetails", "all")
+        self._fourier_wing = None
         self.BodyMotion = ini.str("Insects", "BodyMotion", "yes")
         self.LeftWing = ini.str("Insects", "LeftWing", "yes")
         self.RightWing = ini.str("Insects", "RightWing", "yes")
@@ -776,6 +866,9 @@ class Insect:
             self.x_head = np.array([0.5 * self.L_body, 0.0, 0.0])
             self.x_eye_r = self.x_head + math.sin(45.0 * PI / 180.0) * self.R_head * 0.8 * np.array([1.0, 1.0, 1.0])
             self.x_eye_l = self.x_head + math.sin(45.0 * PI / 180.0) * self.R_head * 0.8 * np.array([1.0, -1.0, 1.0])
+        elif bt == "bumblebee":
+            self._draw_body_bumblebee(g, mask, color)
+            return
         elif bt != "ellipsoid_plain":
             raise FlusiAbort(10623, "Insect%BodyType unknown..." + bt)
         Mb = self.st["M_body"]
@@ -832,6 +925,99 @@ class Insect:
         m[sel] = tmp[sel]
         c[sel] = self.color_body
 
+    # ---- bumblebee (body_geometry.f90:128-347, BB1 of Dudley & Ellington 1990)
+    def _draw_body_bumblebee(self, g, mask, color):
+        """Body along an arc centreline, ellipsoidal head, and (HasDetails) legs,
+        antennae and proboscis as capsules (cylinder plus end spheres)."""
+        s_ = self.st
+        Mb = s_["M_body"]
+        x, y, z = self._matvec(Mb, self._xglob(g))
+        sf, sm = self.safety, self.smooth
+        # body (:157-224)
+        rbc = 1.3
+        thbc1 = 112.0 * PI / 180.0
+        thbc2 = 53.0 * PI / 180.0
+        x0bc = 0.0782301255230126
+        z0bc = -1.26512552301255
+        s = (np.arctan2(z - z0bc, -(x - x0bc)) - thbc1) / (thbc2 - thbc1)
+        rr = np.sqrt((x - x0bc) ** 2 + (z - z0bc) ** 2)
+        xcs = x0bc + (x - x0bc) * rbc / rr
+        zcs = z0bc + (z - z0bc) * rbc / rr
+        inside = (s >= -sf) & (s <= 1.075 + sf)
+        s1 = 1.0 - (s + 0.08 * np.tanh(30.0 * s)) / (1.0 + 0.08 * math.tanh(30.0))
+        s1 = (s1 + 0.04 * np.tanh(60.0 * s1)) / (1.0 + 0.04 * math.tanh(60.0))
+        s1 = np.sin(1.2 * s1) / math.sin(1.2)
+        s1 = np.copysign(np.abs(s1) ** 1.25, s1)
+        x1 = 1.075 * s1
+        abd = x1 < 0.6333
+        thx = (x1 >= 0.6333) & (x1 <= 1.075)
+        R0 = np.where(abd, np.maximum(-1.2990 * x1 ** 2 + 0.9490 * x1 + 0.0267, 0.0),
+                      np.where(thx, np.maximum(-2.1667 * x1 ** 2 + 3.4661 * x1 - 1.2194, 0.0), 0.0))
+        a_body = np.where(abd, 1.0 - 0.07 * (x1 - 0.6333) * x1 / 0.0488, 1.0)
+        R0 = 1.2 * (1.0 + 0.6 * (1.0 - s) ** 2) * R0
+        R = np.sqrt((x - xcs) ** 2 + y ** 2 + (a_body * (z - zcs)) ** 2)
+        sel = inside & (R < R0 + sf) & (R0 > 0.0)
+        mask[sel] = np.maximum(steps(R[sel], R0[sel], sm), mask[sel])
+        color[sel] = self.color_body
+        # head (:229-268)
+        a_head = 1.04
+        xx_head, zz_head = 0.58125, -0.1
+        dx_head, dz_head = 0.5 * 0.2035, 0.5 * 0.297
+        sel = ((np.abs(y) <= dz_head + sf) & (np.abs(z - zz_head) <= dz_head + sf)
+               & (np.abs(x - xx_head) < dx_head + sf) & (((x - xx_head) / dx_head) ** 2 <= 1.0))
+        R = np.sqrt((a_head * y) ** 2 + (z - zz_head) ** 2)
+        with np.errstate(invalid="ignore"):
+            R0 = dz_head * np.sqrt(1.0 - ((x - xx_head) / dx_head) ** 2)
+        sel &= R < R0 + sf
+        mask[sel] = np.maximum(steps(R[sel], R0[sel], sm), mask[sel])
+        color[sel] = self.color_body
+        # legs, antennae, proboscis (:276-345); the coordinates are default-real
+        # (single precision) literals, the radii single-precision products
+        if self.HasDetails in ("all", "legs", "antennae_proboscis"):
+            for a, b, r in bumblebee_capsules():
+                xa = s_["xc"] + Mb.T @ a
+                xb = s_["xc"] + Mb.T @ b
+                self._draw_cylinder(g, xa, xb, r, mask, color)
+
+    def _draw_cylinder(self, g, x1, x2, R0, mask, color):
+        """body_geometry.f90:1531-1647 draw_cylinder_new (global coordinates,
+        not periodized) and its two end spheres.  The reference bounds the loop
+        by a box built from a random perpendicular; with its margin of
+        nint(safety/dx) points the box holds every point of the spherical
+        bound below, so the box is taken here from that sphere."""
+        ddx = (g.dx, g.dy, g.dz)
+        sf = self.safety
+        mid = 0.5 * (x1 + x2)
+        half = 0.5 * math.sqrt(float(np.sum((x1 - x2) ** 2)))
+        lo, hi = [], []
+        for i, n in enumerate((g.nx, g.ny, g.nz)):
+            lo.append(max(int(math.floor((mid[i] - half) / ddx[i])) - 1, 0))
+            hi.append(min(int(math.ceil((mid[i] + half) / ddx[i])) + 1, n - 1))
+        if any(l > h for l, h in zip(lo, hi)):
+            return
+        ix = np.arange(lo[0], hi[0] + 1)
+        iy = np.arange(lo[1], hi[1] + 1)
+        iz = np.arange(lo[2], hi[2] + 1)
+        X = (ix * g.dx)[None, None, :] + np.zeros((len(iz), len(iy), 1))
+        Y = (iy * g.dy)[None, :, None] + np.zeros((len(iz), 1, len(ix)))
+        Z = (iz * g.dz)[:, None, None] + np.zeros((1, len(iy), len(ix)))
+        cb = [mid[0] - X, mid[1] - Y, mid[2] - Z]
+        rb = x1 - x2
+        inb = (cb[0] ** 2 + cb[1] ** 2 + cb[2] ** 2) < 0.25 * float(np.sum(rb ** 2))
+        ab = [X - x1[0], Y - x1[1], Z - x1[2]]
+        u = x2 - x1
+        vp = [ab[1] * u[2] - ab[2] * u[1], ab[2] * u[0] - ab[0] * u[2], ab[0] * u[1] - ab[1] * u[0]]
+        R = np.sqrt((vp[0] ** 2 + vp[1] ** 2 + vp[2] ** 2) / float(np.sum(u ** 2)))
+        sub = (slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
+        m = mask[sub]
+        c = color[sub]
+        t = steps(R, R0, self.smooth)
+        sel = inb & (R <= R0 + sf) & (t >= m)
+        m[sel] = t[sel]
+        c[sel] = self.color_body
+        self._drawsphere(g, x1, R0, mask, color)
+        self._drawsphere(g, x2, R0, mask, color)
+
     def _draw_wings(self, g, mask, color, us):
         """wings_geometry.f90:5-110 draw_insect_wings (stage I right then left
         wing, stage II transform of us to the global frame)."""
@@ -873,6 +1059,9 @@ class Insect:
         elif self.WingShape == "rectangular":
             x_top = np.full_like(xw[0], self.b_top)
             x_bot = np.full_like(xw[0], -self.b_bot)
+        elif self.WingShape == "bumblebee":
+            self._draw_wing_fourier(mask, color, us, cw, Mw, rot, xw)
+            return
         else:
             raise FlusiAbort(10626, f"WingShape {self.WingShape} not in scope")
         sel &= (xw[0] > x_bot - self.safety) & (xw[0] < x_top + self.safety)
@@ -887,6 +1076,32 @@ class Insect:
         MT = Mw.T
         for i in range(3):
             us[i][sel] = (MT[i, 0] * v[0] + MT[i, 1] * v[1] + MT[i, 2] * v[2])[sel]
+
+    def _draw_wing_fourier(self, mask, color, us, cw, Mw, rot, xw):
+        """wings_geometry.f90:161-260 draw_wing_fourier (flat wing of constant
+        thickness, no corrugation) with the radius of FourierWing."""
+        fw = self._fourier_wing
+        if fw is None:
+            fw = self._fourier_wing = FourierWing(self.WingShape, self.WingThickness)
+        sf, sm = self.safety, self.smooth
+        bb = fw.bbox
+        sel = ((xw[0] >= bb[0] - sf) & (xw[0] <= bb[1] + sf) & (xw[1] >= bb[2] - sf) & (xw[1] <= bb[3] + sf)
+               & (xw[2] >= bb[4] - sf) & (xw[2] <= bb[5] + sf))
+        theta = np.arctan2(xw[1][sel] - fw.yc, xw[0][sel] - fw.xc)
+        theta = (theta + PI) / (2.0 * PI)
+        R0 = fw.radius(theta)
+        R = np.sqrt((xw[0][sel] - fw.xc) ** 2 + (xw[1][sel] - fw.yc) ** 2)
+        mt = steps(np.abs(xw[2][sel] - 0.0), 0.5 * self.WingThickness, sm) * steps(R, R0, sm)
+        msel = mask[sel]
+        upd = (msel < mt) & (mt > 0.0)
+        idx = tuple(a[upd] for a in np.nonzero(sel))
+        mask[idx] = mt[upd]
+        color[idx] = cw
+        xs = [c[idx] for c in xw]
+        v = [rot[1] * xs[2] - rot[2] * xs[1], rot[2] * xs[0] - rot[0] * xs[2], rot[0] * xs[1] - rot[1] * xs[0]]
+        MT = Mw.T
+        for i in range(3):
+            us[i][idx] = MT[i, 0] * v[0] + MT[i, 1] * v[1] + MT[i, 2] * v[2]
 
     def aero_inert_power(self, torque_part):
         """module_insects.f90:469-586 aero_power / inert_power."""
@@ -1342,7 +1557,8 @@ class Simulation:
 
     def field_keys(self, time: float, name_it: Optional[str] = None):
         """Key values (postprocessing/keyvalues.f90:7-80) of the fields
-        save_fields writes (velocity, vorticity, mask, us), computed on the
+        save_fields writes (velocity, vorticity and its magnitude
+        (save_fields.f90:144-163), mask, us), computed on the
         float32-rounded field as the default field_precision=single does."""
         g = self.g
         tag = name_it or "%06d" % int(math.floor(time * 1000.0))
@@ -1351,6 +1567,7 @@ class Simulation:
         out = {}
         for nm, fld in (("ux", u[0]), ("uy", u[1]), ("uz", u[2]),
                         ("vorx", vort[0]), ("vory", vort[1]), ("vorz", vort[2]),
+                        ("vorabs", np.sqrt(vort[0] ** 2 + vort[1] ** 2 + vort[2] ** 2)),
                         ("mask", self.mask), ("usx", self.us[0]), ("usy", self.us[1]), ("usz", self.us[2])):
             out[f"{nm}_{tag}"] = keyvalues(fld, time, self.p)
         self.keys.update(out)

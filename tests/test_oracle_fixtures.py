@@ -60,6 +60,37 @@ def test_insects_mask_jerry(golden):
     assert ok, worst
 
 
+def test_insects_mask_BB_tethered(golden):
+    """Bumblebee body with legs, antennae and proboscis, Fourier bumblebee
+    wings, kinematics from file; dry run at t=0..1 (reference
+    tests/insects_mask_BB_tethered.sh)."""
+    d = os.path.join(golden, "insects_mask_BB_tethered")
+    sim = fo.Simulation(os.path.join(d, "insect_test.ini"))
+    sim.insect = fo.Insect(sim.ini, sim.p, d)
+    rows = []
+    for t in (0.0, 0.25, 0.5, 0.75, 1.0):
+        sim.create_mask(t)
+        rows.append(sim.insect.kinematics_row(t))
+        tag = "%06d" % int(round(t * 1000))
+        for nm, f in (("mask", sim.mask), ("usx", sim.us[0]), ("usy", sim.us[1]), ("usz", sim.us[2])):
+            ok, errs = fo.compare_key(fo.keyvalues(f, t, sim.p), fo.read_key(os.path.join(d, f"{nm}_{tag}.ref")))
+            assert ok, (nm, t, errs)
+            assert max(errs) < 1e-8, (nm, t, errs)  # the keys agree to their printed digits
+    ok, worst = fo.compare_timeseries(rows, fo.read_timeseries(os.path.join(d, "kinematics.dry-run.t")))
+    assert ok, worst
+
+
+@pytest.mark.slow
+def test_insects_flow_BB_tethered_RK4(golden):
+    """64^3 RK4 with a moving bumblebee, dynamic mean flow, CFL-adapted
+    intelligent dt and a periodic insect: forces, per-part forces,
+    kinematics, energy and dt series plus mask/us/|w| keys
+    (insects_flow_BB_tethered_RK4.sh:80-100)."""
+    _check_run(os.path.join(golden, "insects_flow_BB_tethered_RK4"), "insect_test.ini",
+               ["forces.t", "forces_part1.t", "forces_part2.t", "forces_part3.t", "kinematics.t", "energy.t",
+                "dt.t"])
+
+
 def test_vortex_ring1_AB2(golden):
     _check_run(os.path.join(golden, "vortex_ring1_AB2"), "vortexring1_AB2.ini", ["dt.t", "energy.t", "meanflow.t"])
 
