@@ -24,14 +24,20 @@ struct CavityArgs { int size; double ux, uy, uz; };
 struct InsectArgs {
   int full_reset;           // delete_old_mask: reset everything (else wing colours only)
   int draw_body;            // body drawn in this call (once for a tethered insect)
-  int body_type;            // 1 = jerry ellipsoid + head/eyes, 2 = ellipsoid only
+  int body_type;            // 1 = jerry ellipsoid + head/eyes, 2 = ellipsoid only, 3 = bumblebee
   double xc[3];             // body centre (global)
   double Mb[9];             // body rotation (row-major, global -> body)
   double b_body, L_body, smooth, safety;
   int nspheres;             // head and eyes (jerry)
   double sph_c[3][3], sph_r[3];
   int sph_lo[3][3], sph_hi[3][3];  // index bounding boxes [sphere][axis]
-  int wing_shape;           // 1 = TwoEllipses, 2 = rectangular
+  // bumblebee legs, antennae and proboscis: ncaps capsule records of
+  // CAPSULE_W doubles (layout at CapsuleField)
+  int ncaps;
+  const double* caps;
+  int wing_shape;           // 1 = TwoEllipses, 2 = rectangular, 3 = Fourier outline
+  const double* fw_table;   // Fourier outline: radius table on nodes j*fw_dphi
+  double fw_xc, fw_yc, fw_dphi, fw_bbox[6];
   double L_span, b_top, b_bot, thickness;
   int draw_wing[2];         // right, left
   double Mw[2][9];          // wing rotation (body -> wing)
@@ -47,6 +53,11 @@ struct InsectArgs {
   // full reset or for periodic insects).
   int reg_lo[3], reg_hi[3];
 };
+
+// capsule record: segment end points (global, not periodized), radius, and
+// the end spheres' centres (periodized) with their index boxes
+enum CapsuleField { CAP_A = 0, CAP_B = 3, CAP_R = 6, CAP_SA = 7, CAP_SB = 10,
+                    CAP_SA_LO = 13, CAP_SA_HI = 16, CAP_SB_LO = 19, CAP_SB_HI = 22, CAPSULE_W = 25 };
 
 // drag.f90:17-454 + integrals.f90:48-238 reductions over physical u, w, div
 struct DiagArgs {

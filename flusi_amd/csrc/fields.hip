@@ -109,7 +109,7 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
     return !a.use_box || (ix >= a.box_lo[k][0] && ix <= a.box_hi[k][0] && iy >= a.box_lo[k][1] &&
                           iy <= a.box_hi[k][1] && iz >= a.box_lo[k][2] && iz <= a.box_hi[k][2]);
   };
-  const bool any = (a.draw_body && (in_box(0) || a.nspheres > 0)) || (a.draw_wing[0] && in_box(1)) ||
+  const bool any = (a.draw_body && (in_box(0) || a.nspheres > 0 || a.ncaps > 0)) || (a.draw_wing[0] && in_box(1)) ||
                    (a.draw_wing[1] && in_box(2));
   if (!any) {
     // nothing can be drawn here: only the deletion of old wing points remains
@@ -124,7 +124,51 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
   double xb[3];
   matvec(a.Mb, xg, xb);
 
-  if (a.draw_body && in_box(0)) {
+  if (a.draw_body && in_box(0) && a.body_type == 3) {
+    // body_geometry.f90:157-268 bumblebee: body along an arc centreline, head
+    const double x = xb[0], y = xb[1], z = xb[2], sf = a.safety;
+    const double rbc = 1.3, thbc1 = 112.0 * PI_D / 180.0, thbc2 = 53.0 * PI_D / 180.0;
+    const double x0bc = 0.0782301255230126, z0bc = -1.26512552301255;
+    const double s = (atan2(z - z0bc, -(x - x0bc)) - thbc1) / (thbc2 - thbc1);
+    if (s >= -sf && s <= 1.075 + sf) {
+      const double rr = sqrt((x - x0bc) * (x - x0bc) + (z - z0bc) * (z - z0bc));
+      const double xcs = x0bc + (x - x0bc) * rbc / rr, zcs = z0bc + (z - z0bc) * rbc / rr;
+      double s1 = 1.0 - (s + 0.08 * tanh(30.0 * s)) / (1.0 + 0.08 * tanh(30.0));
+      s1 = (s1 + 0.04 * tanh(60.0 * s1)) / (1.0 + 0.04 * tanh(60.0));
+      s1 = sin(1.2 * s1) / sin(1.2);
+      s1 = copysign(pow(fabs(s1), 1.25), s1);
+      const double x1 = 1.075 * s1;
+      double R0 = 0.0, a_body = 1.0;
+      if (x1 < 0.6333) {
+        R0 = fmax(-1.2990 * (x1 * x1) + 0.9490 * x1 + 0.0267, 0.0);
+        a_body = 1.0 - 0.07 * (x1 - 0.6333) * x1 / 0.0488;
+      } else if (x1 >= 0.6333 && x1 <= 1.075) {
+        R0 = fmax(-2.1667 * (x1 * x1) + 3.4661 * x1 - 1.2194, 0.0);
+      }
+      R0 = 1.2 * (1.0 + 0.6 * ((1.0 - s) * (1.0 - s))) * R0;
+      const double az = a_body * (z - zcs);
+      const double R = sqrt((x - xcs) * (x - xcs) + y * y + az * az);
+      if (R < R0 + sf && R0 > 0.0) {
+        if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
+        m = fmax(steps_d(R, R0, a.smooth), m);
+        c = 1; changed = true;
+      }
+    }
+    const double a_head = 1.04, xx_head = 0.58125, zz_head = -0.1, dx_head = 0.5 * 0.2035, dz_head = 0.5 * 0.297;
+    if (fabs(y) <= dz_head + sf && fabs(z - zz_head) <= dz_head + sf && fabs(x - xx_head) < dx_head + sf) {
+      const double qx = (x - xx_head) / dx_head;
+      const double R = sqrt((a_head * y) * (a_head * y) + (z - zz_head) * (z - zz_head));
+      if (qx * qx <= 1.0) {
+        const double R0 = dz_head * sqrt(1.0 - qx * qx);
+        if (R < R0 + sf) {
+          if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
+          m = fmax(steps_d(R, R0, a.smooth), m);
+          c = 1; changed = true;
+        }
+      }
+    }
+  }
+  if (a.draw_body && in_box(0) && a.body_type != 3) {
     const double ab = a.L_body / 2.0, b = a.b_body;
     if (fabs(xb[1]) <= b + a.safety && fabs(xb[2]) <= b + a.safety && fabs(xb[0]) < a.L_body / 2 + a.safety) {
       const double rx = xb[0] / ab;
@@ -154,11 +198,81 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
     }
   }
 
+  if (a.draw_body && a.ncaps > 0) {
+    // body_geometry.f90:1531-1647 draw_cylinder_new and its end spheres
+    // (:1657-1724 drawsphere), in the reference's drawing order
+    const double xa[3] = {ix * g.dx, iy * g.dy, iz * g.dz};
+    const int ii[3] = {ix, iy, iz};
+    for (int k = 0; k < a.ncaps; ++k) {
+      const double* cp = a.caps + (long)k * CAPSULE_W;
+      const double r = cp[CAP_R], rr0 = r + a.safety;
+      double cb2 = 0.0, rb2 = 0.0, u2 = 0.0, vp[3], ab[3], u[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double cb = 0.5 * (cp[CAP_A + i] + cp[CAP_B + i]) - xa[i];
+        const double rb = cp[CAP_A + i] - cp[CAP_B + i];
+        cb2 += cb * cb; rb2 += rb * rb;
+        ab[i] = xa[i] - cp[CAP_A + i];
+        u[i] = cp[CAP_B + i] - cp[CAP_A + i];
+        u2 += u[i] * u[i];
+      }
+      if (cb2 < 0.25 * rb2) {
+        vp[0] = ab[1] * u[2] - ab[2] * u[1];
+        vp[1] = ab[2] * u[0] - ab[0] * u[2];
+        vp[2] = ab[0] * u[1] - ab[1] * u[0];
+        const double R = sqrt((vp[0] * vp[0] + vp[1] * vp[1] + vp[2] * vp[2]) / u2);
+        if (R <= rr0) {
+          const double t = steps_d(R, r, a.smooth);
+          if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
+          if (t >= m) { m = t; c = 1; changed = true; }
+        }
+      }
+      for (int e = 0; e < 2; ++e) {
+        const double* sc = cp + (e ? CAP_SB : CAP_SA);
+        const double* lo = cp + (e ? CAP_SB_LO : CAP_SA_LO);
+        const double* hi = cp + (e ? CAP_SB_HI : CAP_SA_HI);
+        bool inb = true;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) inb = inb && ii[i] >= (int)lo[i] && ii[i] <= (int)hi[i];
+        if (!inb) continue;
+        double xs[3] = {xa[0] - sc[0], xa[1] - sc[1], xa[2] - sc[2]};
+        periodize_d(xs, g);
+        const double R = sqrt(xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2]);
+        if (R <= rr0) {
+          const double t = steps_d(R, r, a.smooth);
+          if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
+          if (t >= m) { m = t; c = 1; changed = true; }
+        }
+      }
+    }
+  }
+
   for (int w = 0; w < 2; ++w) {
     if (!a.draw_wing[w] || !in_box(1 + w)) continue;
     const double d[3] = {xb[0] - a.piv[w][0], xb[1] - a.piv[w][1], xb[2] - a.piv[w][2]};
     double xw[3];
     matvec(a.Mw[w], d, xw);
+    if (a.wing_shape == 3) {
+      // wings_geometry.f90:161-260 draw_wing_fourier (flat, constant thickness)
+      const double sf = a.safety;
+      const double* bb = a.fw_bbox;
+      if (!(xw[0] >= bb[0] - sf && xw[0] <= bb[1] + sf && xw[1] >= bb[2] - sf && xw[1] <= bb[3] + sf &&
+            xw[2] >= bb[4] - sf && xw[2] <= bb[5] + sf)) continue;
+      double theta = atan2(xw[1] - a.fw_yc, xw[0] - a.fw_xc);
+      theta = (theta + PI_D) / (2.0 * PI_D);
+      const long j = (long)floor(theta / a.fw_dphi);
+      const double R0 = a.fw_table[j] + ((theta - (double)j * a.fw_dphi) / a.fw_dphi) * (a.fw_table[j + 1] - a.fw_table[j]);
+      const double R = sqrt((xw[0] - a.fw_xc) * (xw[0] - a.fw_xc) + (xw[1] - a.fw_yc) * (xw[1] - a.fw_yc));
+      const double mt = steps_d(fabs(xw[2] - 0.0), 0.5 * a.thickness, a.smooth) * steps_d(R, R0, a.smooth);
+      if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
+      if (m < mt && mt > 0.0) {
+        m = mt; c = (uint8_t)a.color_w[w]; changed = true;
+        const double* r = a.rot[w];
+        const double v[3] = {r[1] * xw[2] - r[2] * xw[1], r[2] * xw[0] - r[0] * xw[2], r[0] * xw[1] - r[1] * xw[0]};
+        matTvec(a.Mw[w], v, u_s);
+      }
+      continue;
+    }
     if (!(xw[1] >= -a.safety && xw[1] <= a.L_span + a.safety)) continue;
     if (!(fabs(xw[2]) <= 0.5 * a.thickness + a.safety)) continue;
     double x_top, x_bot;

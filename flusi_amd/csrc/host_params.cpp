@@ -317,6 +317,7 @@ void Insect::init(const IniFile& ini_c, const Params& p, const std::string& bdir
   kine_r = mk_kine(ini.str("Insects", "FlappingMotion_right", "none"), infile);
   kine_l = mk_kine(ini.str("Insects", "FlappingMotion_left", "none"), infile);
   BodyType = ini.str("Insects", "BodyType", "ellipsoid");
+  HasDetails = ini.str("Insects", "HasDetails", "all");
   BodyMotion = ini.str("Insects", "BodyMotion", "yes");
   LeftWing = ini.str("Insects", "LeftWing", "yes");
   RightWing = ini.str("Insects", "RightWing", "yes");
@@ -343,9 +344,11 @@ void Insect::init(const IniFile& ini_c, const Params& p, const std::string& bdir
   x_pivot_l_b = v3(ini.vct("Insects", "x_pivot_l", {0.0, +b_body, 0.0}));
   x_pivot_r_b = v3(ini.vct("Insects", "x_pivot_r", {0.0, -b_body, 0.0}));
   if (BodyMotion != "tethered") throw FlusiError(10625, "BodyMotion not supported: " + BodyMotion);
-  if (BodyType != "jerry" && BodyType != "Jerry" && BodyType != "ellipsoid_plain" && BodyType != "nobody")
+  if (BodyType != "jerry" && BodyType != "Jerry" && BodyType != "ellipsoid_plain" && BodyType != "nobody" &&
+      BodyType != "bumblebee")
     throw FlusiError(10623, "Insect%BodyType unknown..." + BodyType);
-  if (WingShape != "TwoEllipses" && WingShape != "rectangular")
+  if (WingShape == "bumblebee") fwing.setup(WingShape, WingThickness);
+  else if (WingShape != "TwoEllipses" && WingShape != "rectangular")
     throw FlusiError(10626, "WingShape not supported: " + WingShape);
   update(0.0);
 }
@@ -411,6 +414,86 @@ std::vector<double> Insect::kinematics_row(double t) const {
           s.alpha_r, s.phi_r, s.theta_r, s.rot_rel_l_w[0], s.rot_rel_l_w[1], s.rot_rel_l_w[2],
           s.rot_rel_r_w[0], s.rot_rel_r_w[1], s.rot_rel_r_w[2], s.rot_dt_l_w[0], s.rot_dt_l_w[1],
           s.rot_dt_l_w[2], s.rot_dt_r_w[0], s.rot_dt_r_w[1], s.rot_dt_r_w[2]};
+}
+
+
+// body_geometry.f90:277-345.  The coordinates are Fortran default-real
+// literals and the radii products of default reals: single precision,
+// widened to double.
+std::vector<Capsule> bumblebee_capsules() {
+  auto f = [](float v) { return (double)v; };
+  const float xl1[5] = {-0.74f, -0.63f, -0.4f, -0.1f, 0.1f}, yl1[5] = {0.32f, 0.32f, 0.31f, 0.3f, 0.12f},
+              zl1[5] = {-0.35f, -0.37f, -0.2f, -0.1f, -0.16f};
+  const float xl2[5] = {-0.24f, -0.15f, 0.02f, 0.17f, 0.19f}, yl2[5] = {0.33f, 0.33f, 0.32f, 0.3f, 0.15f},
+              zl2[5] = {-0.29f, -0.28f, -0.2f, -0.15f, -0.19f};
+  const float xl3[5] = {0.28f, 0.35f, 0.45f, 0.4f, 0.35f}, yl3[5] = {0.31f, 0.30f, 0.28f, 0.2f, 0.15f},
+              zl3[5] = {-0.3f, -0.28f, -0.25f, -0.18f, -0.18f};
+  const float r12[4] = {0.015f, 0.03f, 0.04f, 0.03f}, r3[4] = {0.015f, 0.02f, 0.03f, 0.02f};
+  volatile float k13 = 1.3f;  // keep the single-precision products
+  std::vector<Capsule> out;
+  for (int j = 0; j < 4; ++j) {
+    const float* xs[3] = {xl1, xl2, xl3};
+    const float* ys[3] = {yl1, yl2, yl3};
+    const float* zs[3] = {zl1, zl2, zl3};
+    const float rr[3] = {r12[j] * k13, r12[j] * k13, r3[j] * k13};
+    for (int side = 0; side < 2; ++side) {
+      const double sg = side ? -1.0 : 1.0;
+      for (int l = 0; l < 3; ++l)
+        out.push_back(Capsule{Vec3{f(xs[l][j]), sg * f(ys[l][j]), f(zs[l][j])},
+                              Vec3{f(xs[l][j + 1]), sg * f(ys[l][j + 1]), f(zs[l][j + 1])}, (double)rr[l]});
+    }
+  }
+  const float ran = 0.015f * k13, rf = 0.017f * k13;
+  out.push_back(Capsule{Vec3{f(0.63f), f(0.05f), f(-0.03f)}, Vec3{f(0.8f), f(0.27f), f(0.1f)}, (double)ran});
+  out.push_back(Capsule{Vec3{f(0.63f), -f(0.05f), f(-0.03f)}, Vec3{f(0.8f), -f(0.27f), f(0.1f)}, (double)ran});
+  out.push_back(Capsule{Vec3{f(0.43f), f(0.0f), f(-0.28f)}, Vec3{f(0.6f), f(0.0f), f(-0.23f)}, (double)rf});
+  return out;
+}
+
+void FourierWing::setup(const std::string& shape, double thickness) {
+  if (shape != "bumblebee") throw FlusiError(554329, "Fourier wing shape not supported: " + shape);
+  // wings_geometry.f90:1115-1144
+  static const double ai[25] = {-0.0128037920989526, -0.106777418654552, 0.0380851289321982,
+      0.0330548081081197, -0.0178496286355627, -0.00328588543359649, 0.0108246924336137,
+      -0.00489302388329943, -0.00708808441172961, 0.00518244772891516, 0.00445979960844562,
+      -0.000108072056165527, 0.00204437845603716, 0.00147176382618797, -0.00229559463098105,
+      -0.000514633972391526, 0.00134150515430486, -0.000149860228261824, 9.01456938813568e-05,
+      0.00150639712261487, 0.000914624010720407, -0.000737650894315551, -0.000843849923321745,
+      -0.000354971670482499, -0.000382956472432449};
+  static const double bi[25] = {-0.0158061138788171, 0.0308243584184200, -0.00903330410923372,
+      -0.0185758334697500, -0.000924452934252486, -0.00242101213359519, -0.00204549530064489,
+      0.00291468131401423, -0.000140755032337495, -0.00135036427128534, 0.00141285439042451,
+      -0.000334215276598231, -0.00161521722061879, -0.000164055684312904, -0.000256278551727569,
+      -0.000740258481681094, 0.000847498161852221, 0.00157442110960973, -0.000559835622451578,
+      -0.000617498559228280, 0.00115413452523474, 0.000322564770099778, -0.000917375185844477,
+      4.44819399488798e-05, 0.000710028654602170};
+  const double a0 = 0.594557593733011;
+  xc = -0.1;
+  yc = 0.501549263807117;
+  dphi = (2.0 * PI) / double(N_RADIUS - 1);
+  table.assign(N_RADIUS, 0.0);
+  for (int j = 0; j < N_RADIUS; ++j) {
+    const double th = double(j) * dphi;
+    double R = a0 / 2.0;
+    for (int i = 1; i <= 25; ++i)
+      R = R + ai[i - 1] * std::cos(2.0 * PI * double(i) * th) + bi[i - 1] * std::sin(2.0 * PI * double(i) * th);
+    table[j] = R;
+  }
+  double xmin = 999.0e9, ymin = 999.0e9, xmax = -999.0e9, ymax = -999.0e9, theta = 0.0;
+  while (theta < 1.0) {
+    const double r = radius(theta), tp = 2.0 * PI * theta - PI;
+    const double x = xc + r * std::cos(tp), y = yc + r * std::sin(tp);
+    xmin = std::min(xmin, x); ymin = std::min(ymin, y);
+    xmax = std::max(xmax, x); ymax = std::max(ymax, y);
+    theta = theta + 1.0e-3;
+  }
+  bbox[0] = xmin; bbox[1] = xmax; bbox[2] = ymin; bbox[3] = ymax;
+  bbox[4] = -thickness / 2.0; bbox[5] = thickness / 2.0;
+}
+
+double FourierWing::radius(double theta) const {
+  const long j = (long)std::floor(theta / dphi);
+  return table[j] + ((theta - double(j) * dphi) / dphi) * (table[j + 1] - table[j]);
 }
 
 }  // namespace flusi

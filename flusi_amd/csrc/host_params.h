@@ -97,8 +97,26 @@ struct InsectMotion {
 };
 
 // module_insects.f90 type diptera, restricted to a tethered insect
+// Capsule of the bumblebee's legs, antennae and proboscis: segment a-b in
+// the body frame and radius (body_geometry.f90:277-345)
+struct Capsule { Vec3 a, b; double r; };
+std::vector<Capsule> bumblebee_capsules();
+
+// Wing outline R(theta) as a Fourier series in the normalised angle, sampled
+// into a table and interpolated linearly (wings_geometry.f90:819-858,
+// :1115-1144, :1438-1511)
+struct FourierWing {
+  static constexpr int N_RADIUS = 25000;
+  double xc = 0, yc = 0, dphi = 0;
+  double bbox[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> table;
+  void setup(const std::string& shape, double thickness);
+  double radius(double theta) const;
+};
+
 struct Insect {
-  std::string WingShape, BodyType, BodyMotion, LeftWing, RightWing;
+  std::string WingShape, BodyType, BodyMotion, LeftWing, RightWing, HasDetails;
+  FourierWing fwing;
   double b_top = 0, b_bot = 0, L_span = 0, b_body = 0.1, L_body = 1.0, R_head = 0.1, R_eye = 1.0;
   double WingThickness = 0, eta0 = 0, Jxx = 0, Jyy = 0, Jzz = 0, Jxy = 0, smooth = 0, safety = 0;
   Vec3 x0, yawpitchroll_0, x_head, x_eye_r, x_eye_l, x_pivot_l_b, x_pivot_r_b;

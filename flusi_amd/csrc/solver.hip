@@ -138,6 +138,8 @@ class Solver : public SolverBase {
   double *kx_d, *ky_d, *kz_d, *kxt2_d, *kyt2_d, *kzt2_d;
   void* tw_z = nullptr; void* tw_y = nullptr; void* tw_M = nullptr; void* tw_N = nullptr;
   double* xpart = nullptr; int nxblocks = 0;
+  double* caps_d = nullptr;   // bumblebee capsule records
+  double* fw_d = nullptr;     // Fourier wing radius table
   bool xwave = false;       // right-hand-side x pass with one wavefront per line
   double* zpart = nullptr; int nzblocks = 0;
   double* dpart = nullptr; int ndblocks = 0;
@@ -155,7 +157,7 @@ class Solver : public SolverBase {
   ~Solver() override {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto& e : ev) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
-    void* ptrs[] = {uk, reg[0], reg[1], Q, P, mask, us, color, phys, tab, tw_z, tw_y, tw_M, tw_N,
+    void* ptrs[] = {uk, reg[0], reg[1], Q, P, mask, us, color, phys, tab, tw_z, tw_y, tw_M, tw_N, caps_d, fw_d,
                     xpart, zpart, dpart, red, rmid, kinds, stage};
     for (void* q : ptrs) if (q) (void)hipFree(q);
     if (stream) (void)hipStreamDestroy(stream);
@@ -598,7 +600,7 @@ class Solver : public SolverBase {
       }
     }
     const InsectMotion& s = I.st;
-    a.body_type = (I.BodyType == "jerry" || I.BodyType == "Jerry") ? 1 : 2;
+    a.body_type = (I.BodyType == "jerry" || I.BodyType == "Jerry") ? 1 : (I.BodyType == "bumblebee" ? 3 : 2);
     for (int i = 0; i < 3; ++i) a.xc[i] = s.xc[i];
     for (int i = 0; i < 9; ++i) a.Mb[i] = s.Mb[i];
     a.b_body = I.b_body; a.L_body = I.L_body; a.smooth = I.smooth; a.safety = I.safety;
@@ -626,7 +628,52 @@ class Solver : public SolverBase {
       }
       a.nspheres = 3;
     }
-    a.wing_shape = (I.WingShape == "TwoEllipses") ? 1 : 2;
+    if (a.draw_body && a.body_type == 3 &&
+        (I.HasDetails == "all" || I.HasDetails == "legs" || I.HasDetails == "antennae_proboscis")) {
+      // legs, antennae, proboscis (body_geometry.f90:276-345): capsules in the
+      // global frame, end spheres periodized with drawsphere's index boxes
+      const std::vector<Capsule> cl = bumblebee_capsules();
+      std::vector<double> rec(cl.size() * CAPSULE_W, 0.0);
+      const double box[3] = {p.xl, p.yl, p.zl};
+      const double ddx[3] = {p.dx(), p.dy(), p.dz()};
+      const int nn[3] = {nx, ny, nz};
+      const double mind = std::min(ddx[0], std::min(ddx[1], ddx[2]));
+      const Mat3 MbT = transpose(s.Mb);
+      for (size_t k = 0; k < cl.size(); ++k) {
+        double* r = rec.data() + k * CAPSULE_W;
+        Vec3 A = mat_vec(MbT, cl[k].a), B = mat_vec(MbT, cl[k].b);
+        for (int i = 0; i < 3; ++i) { A[i] += s.xc[i]; B[i] += s.xc[i]; }
+        for (int i = 0; i < 3; ++i) { r[CAP_A + i] = A[i]; r[CAP_B + i] = B[i]; }
+        r[CAP_R] = cl[k].r;
+        const int ns = fortran_nint((cl[k].r + I.safety) / mind);
+        for (int e = 0; e < 2; ++e) {
+          Vec3 c = e ? B : A;
+          for (int i = 0; i < 3; ++i) if (c[i] < 0.0) c[i] += box[i];
+          for (int i = 0; i < 3; ++i) if (c[i] >= box[i]) c[i] -= box[i];
+          for (int i = 0; i < 3; ++i) {
+            const int ci = fortran_nint(c[i] / ddx[i]);
+            r[(e ? CAP_SB : CAP_SA) + i] = c[i];
+            r[(e ? CAP_SB_LO : CAP_SA_LO) + i] = std::max(ci - ns, 0);
+            r[(e ? CAP_SB_HI : CAP_SA_HI) + i] = std::min(ci + ns, nn[i] - 1);
+          }
+        }
+      }
+      if (!caps_d) caps_d = (double*)dalloc(sizeof(double) * rec.size());
+      HIPCHECK(hipMemcpyAsync(caps_d, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      a.ncaps = (int)cl.size();
+      a.caps = caps_d;
+    }
+    if (I.WingShape == "bumblebee") {
+      if (!fw_d) {
+        fw_d = (double*)dalloc(sizeof(double) * I.fwing.table.size());
+        HIPCHECK(hipMemcpy(fw_d, I.fwing.table.data(), sizeof(double) * I.fwing.table.size(), hipMemcpyHostToDevice));
+      }
+      a.fw_table = fw_d;
+      a.fw_xc = I.fwing.xc; a.fw_yc = I.fwing.yc; a.fw_dphi = I.fwing.dphi;
+      for (int i = 0; i < 6; ++i) a.fw_bbox[i] = I.fwing.bbox[i];
+    }
+    a.wing_shape = (I.WingShape == "TwoEllipses") ? 1 : (I.WingShape == "bumblebee" ? 3 : 2);
     a.L_span = I.L_span; a.b_top = I.b_top; a.b_bot = I.b_bot; a.thickness = I.WingThickness;
     a.draw_wing[0] = I.RightWing == "yes"; a.draw_wing[1] = I.LeftWing == "yes";
     for (int i = 0; i < 9; ++i) { a.Mw[0][i] = s.Mwr[i]; a.Mw[1][i] = s.Mwl[i]; }
@@ -658,9 +705,17 @@ class Solver : public SolverBase {
       }
     };
     const double sf = I.safety;
-    set_box(0, {-I.L_body / 2 - sf, -I.b_body - sf, -I.b_body - sf}, {I.L_body / 2 + sf, I.b_body + sf, I.b_body + sf},
-            nullptr, {0, 0, 0});
-    const Vec3 wlo{-I.b_bot - sf, -sf, -0.5 * I.WingThickness - sf}, whi{I.b_top + sf, I.L_span + sf, 0.5 * I.WingThickness + sf};
+    if (a.body_type == 3)  // bumblebee body, head and appendages lie well inside this body-frame box
+      set_box(0, {-1.5 - sf, -0.8 - sf, -1.0 - sf}, {1.5 + sf, 0.8 + sf, 0.8 + sf}, nullptr, {0, 0, 0});
+    else
+      set_box(0, {-I.L_body / 2 - sf, -I.b_body - sf, -I.b_body - sf}, {I.L_body / 2 + sf, I.b_body + sf, I.b_body + sf},
+              nullptr, {0, 0, 0});
+    Vec3 wlo{-I.b_bot - sf, -sf, -0.5 * I.WingThickness - sf}, whi{I.b_top + sf, I.L_span + sf, 0.5 * I.WingThickness + sf};
+    if (a.wing_shape == 3) {
+      const double* bb = I.fwing.bbox;
+      wlo = Vec3{bb[0] - sf, bb[2] - sf, bb[4] - sf};
+      whi = Vec3{bb[1] + sf, bb[3] + sf, bb[5] + sf};
+    }
     set_box(1, wlo, whi, &s.Mwr, I.x_pivot_r_b);
     set_box(2, wlo, whi, &s.Mwl, I.x_pivot_l_b);
 

@@ -73,17 +73,13 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, int kx, int ky, in
     for (int c = 0; c < 3; ++c)
       if (a.meanflow_dynamic[c]) { nr[c] = -a.pen_sum[c] * a.dynamic_scale; ni[c] = 0.0; }
   }
-  // the diffusion operand; when it is the state itself (first RK4 stage) the
-  // update below reuses it instead of reading the state again
-  cplx<T> udv[3];
-  const bool ud_is_u = a.explicit_diffusion && a.udiff == a.u;
   if (a.explicit_diffusion) {
     // rhs.f90:441-468
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      udv[c] = UD[c * a.fs + si];
-      nr[c] -= a.nu * k2 * (double)udv[c].x;
-      ni[c] -= a.nu * k2 * (double)udv[c].y;
+      const cplx<T> us_ = UD[c * a.fs + si];
+      nr[c] -= a.nu * k2 * (double)us_.x;
+      ni[c] -= a.nu * k2 * (double)us_.y;
     }
   }
   if (k2 != 0.0) {
@@ -161,7 +157,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, int kx, int ky, in
       for (int c = 0; c < 3; ++c) {
         const double kr = nr[c] * kill, ki = ni[c] * kill;
         if (a.mode == SM_RK4_1) {
-          const cplx<T> u0 = ud_is_u ? udv[c] : U[c * a.fs + si];
+          const cplx<T> u0 = U[c * a.fs + si];
           ACC[c * a.fs + si] = mk<T>((T)((double)u0.x + (a.dt / 6.0) * kr), (T)((double)u0.y + (a.dt / 6.0) * ki));
           UST[c * a.fs + si] = mk<T>((T)((double)u0.x + 0.5 * a.dt * kr), (T)((double)u0.y + 0.5 * a.dt * ki));
         } else if (a.mode == SM_RK4_2 || a.mode == SM_RK4_3) {
