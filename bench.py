@@ -4,6 +4,11 @@ stage) on one MI355X, FP32-mixed, with the FP64 path and the FP64 CPU oracle
 beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512] [--no-cpu] [--no-fp64]
+    python bench.py --workload ensemble [--steps K] [--warmup W] [--no-cpu]
+
+The ensemble workload is 48 independent penalized simulations (16 each at
+64^3, 96^3 and 128^3, half spheres, half flapping plates, RK2, FP32-mixed)
+advanced as batched launches replayed from one captured hipGraph.
 
 Prints one JSON line (rank 0).  Data are synthetic: the flow starts from rest
 and is driven by the flapping wings (quiescent initial condition), with the
@@ -24,6 +29,8 @@ sys.path.insert(0, ROOT)
 METRIC = "grid-pt·steps/s at 512^3, FP32-mixed vs FP64 (force rel err≤1e-4); % HBM roofline"
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 B_PT_RK4 = {4: 895.2, 8: 1790.5}   # algorithmic bytes per grid-point-step, RK4 (penalised)
+B_PT_RK2 = {4: 423.0, 8: 846.0}    # the same for RK2 (ensemble members)
+ENS_METRIC = "ensemble grid-pt·steps/s (48 sims: 16x64^3+16x96^3+16x128^3, RK2, FP32-mixed, one hipGraph); % HBM roofline"
 
 
 def make_ini(tmpdir, n, nt):
@@ -40,9 +47,9 @@ def make_ini(tmpdir, n, nt):
     return path
 
 
-def pass_bytes(n, s):
+def pass_bytes(n, s, stepper="RK4"):
     """Algorithmic HBM bytes per launch of each pass at an n^3 grid (complex
-    field C = 2*(n/2+1)*n*n*s bytes, real field R = n^3*s), RK4 stages."""
+    field C = 2*(n/2+1)*n*n*s bytes, real field R = n^3*s)."""
     C = 2 * (n // 2 + 1) * n * n * s
     R = n ** 3 * s
     return {
@@ -50,7 +57,8 @@ def pass_bytes(n, s):
         "YINV": 11 * C,           # 5 in, 6 out
         "XPASS": 9 * C + R,       # 6 in, 3 out, mask (us only inside the mask)
         "YFWD": 6 * C,            # 3 in, 3 out
-        "ZSTEP": 15 * C,          # RK4 average of 12/18/18/12 C
+        # RK4: average of 12/18/18/12 C; RK2: 12 C in both stages
+        "ZSTEP": (15 if stepper == "RK4" else 12) * C,
     }
 
 
@@ -111,6 +119,107 @@ print(json.dumps({{"t_rhs": t1 - t0}}))
     return json.loads(r.stdout.strip().splitlines()[-1])["t_rhs"], None
 
 
+def ensemble_cpu_baseline(sizes, per_size, workers):
+    """One right-hand side of one plate member per grid size with the FP64
+    oracle in a child process; the ensemble's CPU rate is its points over
+    sum_size(per_size * 2 * t_RHS(size)) (RK2: two right-hand sides a step)."""
+    code = f"""
+import sys, time, json, tempfile
+sys.path.insert(0, {ROOT!r})
+from oracle import flusi_oracle as fo
+from flusi_amd.ensemble_configs import write_ensemble
+out = {{}}
+for n in {list(sizes)!r}:
+    paths = write_ensemble(tempfile.mkdtemp(), sizes=(n,), per_size=2)
+    sim = fo.Simulation(paths[1], workers={workers})
+    sim.init_fields()
+    t0 = time.perf_counter()
+    sim.cal_nlk(0.0, sim.uk)
+    out[n] = time.perf_counter() - t0
+print(json.dumps(out))
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        return None, r.stderr[-400:]
+    t = {int(k): v for k, v in json.loads(r.stdout.strip().splitlines()[-1]).items()}
+    pts = sum(per_size * n ** 3 for n in sizes)
+    return pts / sum(per_size * 2.0 * t[n] for n in sizes), t
+
+
+def run_ensemble(args):
+    import flusi_amd
+    from flusi_amd.ensemble_configs import write_ensemble
+    sizes, per_size = (64, 96, 128), 16
+    paths = write_ensemble(tempfile.mkdtemp(), sizes=sizes, per_size=per_size,
+                           nt=args.warmup + 2 * args.steps + 20)
+    pts = sum(per_size * n ** 3 for n in sizes)
+    e = flusi_amd.Ensemble(paths, flusi_amd.FP32_MIXED)
+    e.init_fields()
+    tune = e.autotune(reps=2)
+    print(f"ensemble: {len(paths)} members, batches {e.batches()}, autotune {tune}", file=sys.stderr, flush=True)
+    e.advance(args.warmup, flusi_amd.ENSEMBLE_GRAPH)  # first step eager, then the capture
+    t0 = time.perf_counter()
+    e.advance(args.steps, flusi_amd.ENSEMBLE_GRAPH)     # synchronises before returning
+    dt = time.perf_counter() - t0
+    value = pts * args.steps / dt
+    # per-pass device times of the batched launches (eager, event-timed on
+    # each batch's stream) for the roofline of the dominant pass
+    leads = e.batches(with_lead=True)
+    for b in leads:
+        e.member(b[4]).timers(1)
+    k_eager = max(2, args.steps // 4)
+    t1 = time.perf_counter()
+    e.advance(k_eager, flusi_amd.ENSEMBLE_BATCHED)
+    dt_eager = time.perf_counter() - t1
+    per_pass, per_bytes, per_batch = {}, {}, []
+    for (nm, nx, ny, nz, lead) in leads:
+        ms = e.member(lead).timers(0)
+        per_batch.append({"members": nm, "grid": nx, "lead": lead,
+                          "ms_per_step": {k: round(v / k_eager, 4) for k, v in ms.items() if v > 0}})
+        pb = pass_bytes(nx, 4, "RK2")
+        for k, v in ms.items():
+            if k in pb:
+                per_pass[k] = per_pass.get(k, 0.0) + v
+                per_bytes[k] = per_bytes.get(k, 0.0) + pb[k] * nm * 2 * k_eager  # 2 launches a step
+    t2 = time.perf_counter()
+    e.advance(2, flusi_amd.ENSEMBLE_MEMBERS)
+    dt_members = (time.perf_counter() - t2) / 2
+    e.close()
+    best = max(per_pass, key=lambda k: per_pass[k])
+    nlaunch = 2 * k_eager * len(leads)
+    ach = per_bytes[best] / (per_pass[best] / 1e3)
+    out = {
+        "metric": ENS_METRIC, "value": value, "unit": "grid-pt·steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "48-member ensemble: 16x64^3, 16x96^3, 16x128^3, box 2pi, half spheres "
+                               "(r~U(0.3,0.8), default_rng(5)), half flapping rectangular plates, eta=1e-3, "
+                               "nu=1e-2, RK2, FP32-mixed, batched launches in one captured hipGraph",
+                   "members": len(paths), "points": pts, "batches": len(leads)},
+        "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK, "traffic": None, "kernel": best,
+                     "algorithmic_bytes_per_launch": per_bytes[best] / nlaunch,
+                     "avg_launch_ms": per_pass[best] / nlaunch},
+        "step_roofline_frac": B_PT_RK2[4] * value / HBM_PEAK,
+        "batched_eager_gps": pts * k_eager / dt_eager,
+        "members_alone_gps": pts / dt_members,
+        "pass_ms_per_step": {k: v / k_eager for k, v in per_pass.items()},
+        "batch_ms_per_step": per_batch,
+        "autotune": tune,
+    }
+    if not args.no_cpu:
+        workers = min(16, os.cpu_count() or 1)
+        v, t = ensemble_cpu_baseline(sizes, per_size, workers)
+        if v:
+            out["cpu_baseline"] = {"value": v, "unit": "grid-pt·steps/s", "cores": workers, "kind": "port",
+                                   "sample": "one FP64 right-hand side of one plate member per grid size by the "
+                                             f"numpy/scipy oracle (t_RHS {t}); rate = points / "
+                                             "sum(16 * 2 * t_RHS)"}
+        else:
+            out["cpu_baseline"] = {"value": None, "error": t}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +228,7 @@ def main():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", choices=["insect", "ensemble"], default="insect")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for HBM bytes)")
     args = ap.parse_args()
@@ -135,6 +245,9 @@ def main():
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
     import flusi_amd
     n = args.grid
     N = n ** 3

@@ -20,12 +20,16 @@ from typing import Dict, Optional
 
 import numpy as np
 
-__all__ = ["Flusi", "FlusiError", "FP64", "FP32_MIXED", "load_library", "INTEGRALS_NV", "EXPORTED"]
+__all__ = ["Flusi", "Ensemble", "FlusiError", "FP64", "FP32_MIXED", "load_library", "INTEGRALS_NV", "EXPORTED",
+           "ENSEMBLE_MEMBERS", "ENSEMBLE_BATCHED", "ENSEMBLE_GRAPH"]
 
 FP64 = 0
 FP32_MIXED = 1
 INTEGRALS_NV = 61
 TIMER_SLOTS = 8
+ENSEMBLE_MEMBERS = 0   # every member steps on its own
+ENSEMBLE_BATCHED = 1   # one launch per pass and batch
+ENSEMBLE_GRAPH = 2     # the batched step captured once into a hipGraph and replayed
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libflusi_hip.so")
@@ -59,6 +63,17 @@ EXPORTED: Dict[str, tuple] = {
     "flusi_fft": (_I, [_P, _D, _D]),
     "flusi_ifft": (_I, [_P, _D, _D]),
     "flusi_timers": (_I, [_P, _I, _D, ctypes.POINTER(ctypes.c_char_p)]),
+    "flusi_autotune": (_I, [_P, _I, ctypes.POINTER(_I), _D]),
+    "flusi_ensemble_create": (_I, [_I, ctypes.POINTER(ctypes.c_char_p), _I, ctypes.POINTER(_P)]),
+    "flusi_ensemble_free": (None, [_P]),
+    "flusi_ensemble_last_error": (ctypes.c_char_p, [_P]),
+    "flusi_ensemble_size": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "flusi_ensemble_batch": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                  ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "flusi_ensemble_member": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "flusi_ensemble_init_fields": (_I, [_P]),
+    "flusi_ensemble_autotune": (_I, [_P, _I, ctypes.POINTER(_I), _D]),
+    "flusi_ensemble_advance": (_I, [_P, _I, _I]),
 }
 
 _lib = None
@@ -99,16 +114,22 @@ def _dp(a: np.ndarray):
 class Flusi:
     """One solver context (one GPU, one stream) for a FLUSI .ini file."""
 
-    def __init__(self, ini_path: str, precision: int = FP64, output_dir: Optional[str] = None):
+    def __init__(self, ini_path: Optional[str], precision: int = FP64, output_dir: Optional[str] = None,
+                 _borrow=None):
         self._lib = load_library()
         self._ctx = _P()
-        rc = self._lib.flusi_init(os.fsencode(ini_path), int(precision), ctypes.byref(self._ctx))
-        if rc != 0:
-            msg = self._lib.flusi_last_error(self._ctx).decode() if self._ctx else ""
-            if self._ctx:
-                self._lib.flusi_free(self._ctx)
-                self._ctx = _P()
-            raise FlusiError(rc, msg)
+        self._owner = None
+        if _borrow is not None:  # a member context owned by an Ensemble
+            self._owner, ctx = _borrow
+            self._ctx = ctx
+        else:
+            rc = self._lib.flusi_init(os.fsencode(ini_path), int(precision), ctypes.byref(self._ctx))
+            if rc != 0:
+                msg = self._lib.flusi_last_error(self._ctx).decode() if self._ctx else ""
+                if self._ctx:
+                    self._lib.flusi_free(self._ctx)
+                    self._ctx = _P()
+                raise FlusiError(rc, msg)
         nx, ny, nz = _I(), _I(), _I()
         self._check(self._lib.flusi_get_dims(self._ctx, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
         self.nx, self.ny, self.nz = nx.value, ny.value, nz.value
@@ -124,7 +145,8 @@ class Flusi:
 
     def close(self):
         if getattr(self, "_ctx", None):
-            self._lib.flusi_free(self._ctx)
+            if self._owner is None:
+                self._lib.flusi_free(self._ctx)
             self._ctx = _P()
 
     def __del__(self):
@@ -246,3 +268,88 @@ class Flusi:
         names = (ctypes.c_char_p * TIMER_SLOTS)()
         self._check(self._lib.flusi_timers(self._ctx, int(enable), _dp(ms), names))
         return {names[i].decode(): float(ms[i]) for i in range(TIMER_SLOTS)}
+
+    def autotune(self, reps: int = 3):
+        """Picks the fastest kernel variant for this grid (cached per grid,
+        precision and batch size); returns (variant[5], ms per right-hand side)."""
+        v = (_I * 5)()
+        ms = ctypes.c_double()
+        self._check(self._lib.flusi_autotune(self._ctx, int(reps), v, ctypes.byref(ms)))
+        return list(v), ms.value
+
+
+class Ensemble:
+    """Many independent simulations (one .ini each) advanced together; members
+    with equal grids and solver parameters form batches whose passes run as
+    one launch.  ``member(i)`` is an ordinary ``Flusi`` context for reading
+    fields and integrals between advances."""
+
+    def __init__(self, ini_paths, precision: int = FP32_MIXED):
+        self._lib = load_library()
+        self._e = _P()
+        paths = (ctypes.c_char_p * len(ini_paths))(*[os.fsencode(p) for p in ini_paths])
+        rc = self._lib.flusi_ensemble_create(len(ini_paths), paths, int(precision), ctypes.byref(self._e))
+        if rc != 0:
+            msg = self._lib.flusi_ensemble_last_error(self._e).decode() if self._e else ""
+            if self._e:
+                self._lib.flusi_ensemble_free(self._e)
+                self._e = _P()
+            raise FlusiError(rc, msg)
+        n, nb = _I(), _I()
+        self._check(self._lib.flusi_ensemble_size(self._e, ctypes.byref(n), ctypes.byref(nb)))
+        self.n, self.nbatches = n.value, nb.value
+        self.precision = precision
+        self._members = [None] * self.n
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise FlusiError(rc, self._lib.flusi_ensemble_last_error(self._e).decode())
+
+    def close(self):
+        if getattr(self, "_e", None):
+            for m in self._members:
+                if m is not None:
+                    m._ctx = _P()
+            self._lib.flusi_ensemble_free(self._e)
+            self._e = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def batches(self, with_lead: bool = False):
+        """[(members, nx, ny, nz)] per batch, with the index of the member
+        that launches for the batch appended when with_lead is set."""
+        out = []
+        for b in range(self.nbatches):
+            v = [_I() for _ in range(5)]
+            self._check(self._lib.flusi_ensemble_batch(self._e, b, *[ctypes.byref(x) for x in v]))
+            out.append(tuple(x.value for x in (v if with_lead else v[:4])))
+        return out
+
+    def member(self, i: int) -> Flusi:
+        if self._members[i] is None:
+            ctx = _P()
+            self._check(self._lib.flusi_ensemble_member(self._e, int(i), ctypes.byref(ctx)))
+            self._members[i] = Flusi(None, self.precision, _borrow=(self, ctx))
+        return self._members[i]
+
+    def init_fields(self):
+        self._check(self._lib.flusi_ensemble_init_fields(self._e))
+
+    def autotune(self, reps: int = 3):
+        v = (_I * (5 * self.nbatches))()
+        ms = (ctypes.c_double * self.nbatches)()
+        self._check(self._lib.flusi_ensemble_autotune(self._e, int(reps), v, ms))
+        return [(list(v[5 * b:5 * b + 5]), ms[b]) for b in range(self.nbatches)]
+
+    def advance(self, nsteps: int, mode: int = ENSEMBLE_GRAPH):
+        self._check(self._lib.flusi_ensemble_advance(self._e, int(nsteps), int(mode)))

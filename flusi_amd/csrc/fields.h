@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "flusi_kernels.h"
 
 namespace flusi {
 
@@ -88,7 +89,11 @@ enum {
 
 template <typename T> void launch_sphere(const GridGeom& g, const SphereArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s);
 template <typename T> void launch_cavity(const GridGeom& g, const CavityArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s);
-template <typename T> void launch_insect(const GridGeom& g, const InsectArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s);
+// a: the arguments on the host (the launch region comes from them); a_dev:
+// the same arguments in device memory, read by the kernel, one record per
+// batch member (bat.nmem > 1: every member's region must equal a's)
+template <typename T> void launch_insect(const GridGeom& g, const InsectArgs& a, const InsectArgs* a_dev, void* mask,
+                                         void* us, uint8_t* color, hipStream_t s, const Batch& bat);
 template <typename T> void launch_vortex_ring(const GridGeom& g, double x0, double y0, double z0, double r00, double a, double gamma0, void* vort, hipStream_t s);
 // spectral helpers on S-layout fields (3 components, component stride ncplx)
 template <typename T> void launch_biot_savart(const GridGeom& g, const double* kx, const double* ky, const double* kz, const void* vk, void* uk, hipStream_t s);

@@ -89,10 +89,18 @@ void launch_cavity(const GridGeom& g, const CavityArgs& a, void* mask, void* us,
 // draw_wing_twoellipses / _rectangular (wings_geometry.f90:527-712) and the
 // stage II body-motion transform (wings_geometry.f90:71-108).
 template <typename T>
-__global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* color) {
-  // grid over the region: x in blocks of blockDim.x, y = blockIdx.y, z = blockIdx.z
+__global__ void insect_k(GridGeom g, const InsectArgs* __restrict__ ap, T* mask, T* us, uint8_t* color, int ez,
+                         long mstride) {
+  // the arguments live in device memory, so that a captured launch reads the
+  // kinematics the host wrote for this stage.  Batch member m = blockIdx.z / ez
+  // has its own record ap[m] and its fields m * mstride bytes further on; the
+  // members share the region's extents.
+  const int mem = blockIdx.z / ez;
+  const InsectArgs& a = ap[mem];
+  mask = mshift(mask, mem * mstride); us = mshift(us, mem * mstride); color = mshift(color, mem * mstride);
+  // grid over the region: x in blocks of blockDim.x, y = blockIdx.y, z = blockIdx.z % ez
   const int ix = a.reg_lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int iy = a.reg_lo[1] + blockIdx.y, iz = a.reg_lo[2] + blockIdx.z;
+  const int iy = a.reg_lo[1] + blockIdx.y, iz = a.reg_lo[2] + (blockIdx.z - mem * ez);
   if (ix > a.reg_hi[0]) return;
   const long p = ((long)iz * g.ny + iy) * g.nx + ix;
   const uint8_t c_old = color[p];
@@ -312,7 +320,8 @@ __global__ void insect_k(GridGeom g, InsectArgs a, T* mask, T* us, uint8_t* colo
 }
 
 template <typename T>
-void launch_insect(const GridGeom& g, const InsectArgs& a, void* mask, void* us, uint8_t* color, hipStream_t s) {
+void launch_insect(const GridGeom& g, const InsectArgs& a, const InsectArgs* a_dev, void* mask, void* us, uint8_t* color,
+                   hipStream_t s, const Batch& bat) {
   const int nt = 256;
   if (a.reg_lo[0] > a.reg_hi[0] || a.reg_lo[1] > a.reg_hi[1] || a.reg_lo[2] > a.reg_hi[2]) return;  // nothing to update
   for (int i = 0; i < 3; ++i) {
@@ -321,7 +330,10 @@ void launch_insect(const GridGeom& g, const InsectArgs& a, void* mask, void* us,
       throw std::runtime_error("insect mask: region outside the grid");
   }
   const int ex = a.reg_hi[0] - a.reg_lo[0] + 1, ey = a.reg_hi[1] - a.reg_lo[1] + 1, ez = a.reg_hi[2] - a.reg_lo[2] + 1;
-  hipLaunchKernelGGL(insect_k<T>, dim3((ex + nt - 1) / nt, ey, ez), dim3(nt), 0, s, g, a, (T*)mask, (T*)us, color);
+  const int nm = batch_n(bat);
+  if ((long)ez * nm > 65535) throw std::runtime_error("insect mask: batch too large for one launch");
+  hipLaunchKernelGGL(insect_k<T>, dim3((ex + nt - 1) / nt, ey, ez * nm), dim3(nt), 0, s, g, a_dev, (T*)mask, (T*)us, color,
+                     ez, nm > 1 ? bat.mstride : 0L);
 }
 
 // ------------------------------------------------------------- vortex ring
@@ -639,7 +651,8 @@ void launch_zero(void* p, size_t bytes, hipStream_t s) {
 #define INST(T)                                                                                              \
   template void launch_sphere<T>(const GridGeom&, const SphereArgs&, void*, void*, uint8_t*, hipStream_t);   \
   template void launch_cavity<T>(const GridGeom&, const CavityArgs&, void*, void*, uint8_t*, hipStream_t);   \
-  template void launch_insect<T>(const GridGeom&, const InsectArgs&, void*, void*, uint8_t*, hipStream_t);   \
+  template void launch_insect<T>(const GridGeom&, const InsectArgs&, const InsectArgs*, void*, void*, uint8_t*, hipStream_t, \
+                                 const Batch&);                                                                           \
   template void launch_vortex_ring<T>(const GridGeom&, double, double, double, double, double, double, void*, hipStream_t); \
   template void launch_biot_savart<T>(const GridGeom&, const double*, const double*, const double*, const void*, void*, hipStream_t); \
   template void launch_divergence<T>(const GridGeom&, const double*, const double*, const double*, const void*, void*, hipStream_t); \

@@ -10,6 +10,20 @@ namespace flusi {
 // b = tile axis, c = outer axis).
 struct Strides { long a, b, c; };
 
+// Batched launches.  The members of a batch (simulations of one grid and one
+// parameter set) keep identical buffer layouts in slabs `mstride` bytes
+// apart.  A batched launch adds one grid dimension over its nmem members and
+// shifts every device pointer of the argument block by member * mstride;
+// nmem <= 1 is a plain launch.
+template <typename P> __host__ __device__ __forceinline__ P* mshift(P* p, long off) {
+  return p ? reinterpret_cast<P*>(reinterpret_cast<uintptr_t>(p) + off) : p;
+}
+struct Batch {
+  int nmem;      // members in the launch (0 or 1: not batched)
+  long mstride;  // bytes between two members' slabs (a multiple of 256)
+};
+__host__ __device__ __forceinline__ int batch_n(const Batch& b) { return b.nmem > 1 ? b.nmem : 1; }
+
 enum TileOp { OP_COPY = 0, OP_DERIV = 1, OP_DERIV_MINUS = 2 };
 
 // One job reads its operands once and writes one or two transforms of them.
@@ -30,6 +44,8 @@ struct TilePassArgs {
   double kscale;       // k_a = kscale * wrap(a): wavenumbers along the FFT axis
   double scale;        // applied to every stored value
   const void* tw;      // L twiddles exp(-2 pi i m / L), element type of the pass
+  int pf;              // block passes: load the next job's operand while the current one is transformed
+  Batch bat;           // grid z: member
 };
 
 // x pass modes
@@ -50,6 +66,7 @@ struct XPassArgs {
   long prow;           // row stride of the spectral (P-layout) side, >= nxh
   const double* kx;    // nxh wavenumbers
   double kx_scale;     // kx = kx_scale * k
+  int wpe;             // one-wave-per-line RHS pass: register budget for 3 waves per SIMD (3) or none (0)
   const void* twM;     // M = nx/2 twiddles
   const void* twN;     // nxh twiddles exp(-2 pi i k / nx)
   // RHS extras
@@ -64,7 +81,16 @@ struct XPassArgs {
   int nz;
   double* partial;     // per block: umax^2, sum chi(u-us) x,y,z (4 doubles)
   int want_stats;      // write the per-block statistics (adaptive dt / dynamic mean flow)
+  Batch bat;           // grid y: member
 };
+
+// XPassArgs with its pointers moved to batch member m
+__host__ __device__ __forceinline__ void to_member(XPassArgs& a, int m) {
+  const long o = (long)m * a.bat.mstride;
+  a.in = mshift(a.in, o); a.out = mshift(a.out, o); a.kx = mshift(a.kx, o);
+  a.twM = mshift(a.twM, o); a.twN = mshift(a.twN, o);
+  a.mask = mshift(a.mask, o); a.us = mshift(a.us, o); a.partial = mshift(a.partial, o);
+}
 
 // z-forward pass with the fused spectral epilogue (projection, explicit
 // diffusion, dealiasing, stepper update, zero-mode forcing, Parseval sum)
@@ -111,6 +137,8 @@ struct ZStepArgs {
   int set_mean[3];       // post-step zero-mode overwrite (fixed/sinusoidal)
   double mean_value[3];
   double* energy_partial;  // per block Parseval partial sum (final stages only)
+  int wave;              // 1: transform each z line inside one wavefront (nz = 64..512)
+  Batch bat;             // grid z: member
 };
 
 // launchers (passes.hip)

@@ -49,13 +49,14 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
   const int tid = threadIdx.x;
   const int b0 = blockIdx.x * TL;
   const long c = blockIdx.y;
-  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw);
+  const long moff = (long)blockIdx.z * a.bat.mstride;  // batch member's slab
+  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(mshift(a.tw, moff));
   const T sc = (T)a.scale;
 
   cplx<T> r0[PER0][R0], r1[PER0][R0];
   // operand `which` (0: in0, prefetched; 1: in1, read when its job starts)
   auto load = [&](int jb, int which) {
-    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(which ? a.jobs[jb].in1 : a.jobs[jb].in0);
+    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(mshift(which ? a.jobs[jb].in1 : a.jobs[jb].in0, moff));
 #pragma unroll
     for (int p = 0; p < PER0; ++p) {
       const int beta = tid + p * NT;
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
     if (a.jobs[jb].in1) load(jb, 1);
     for (int o = 0; o < nout; ++o) {
       const int op = o ? a.jobs[jb].op2 : a.jobs[jb].op;
-      cplx<T>* out = reinterpret_cast<cplx<T>*>(o ? a.jobs[jb].out2 : a.jobs[jb].out);
+      cplx<T>* out = reinterpret_cast<cplx<T>*>(mshift(o ? a.jobs[jb].out2 : a.jobs[jb].out, moff));
 #pragma unroll
       for (int p = 0; p < PER0; ++p) {
         const int beta = tid + p * NT;
@@ -148,11 +149,6 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
   }
 }
 
-static int tile_prefetch() {
-  static int v = [] { const char* e = getenv("FLUSI_TILE_PREFETCH"); return e ? atoi(e) : 1; }();
-  return v;
-}
-
 template <typename T, int L, int S, bool CI, bool PF>
 static void run_tile_pf(const TilePassArgs& a, hipStream_t s) {
   constexpr int TL = Tile<T>::TL;
@@ -160,12 +156,12 @@ static void run_tile_pf(const TilePassArgs& a, hipStream_t s) {
   const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
   static bool init = false;
   if (!init) { set_smem(tile_pass_k<T, L, TL, NT, S, CI, PF>, smem); init = true; }
-  dim3 grid((a.nb + TL - 1) / TL, a.nc);
+  dim3 grid((a.nb + TL - 1) / TL, a.nc, batch_n(a.bat));
   hipLaunchKernelGGL((tile_pass_k<T, L, TL, NT, S, CI, PF>), grid, dim3(NT), smem, s, a);
 }
 template <typename T, int L, int S, bool CI>
 static void run_tile(const TilePassArgs& a, hipStream_t s) {
-  if (tile_prefetch()) run_tile_pf<T, L, S, CI, true>(a, s);
+  if (a.pf) run_tile_pf<T, L, S, CI, true>(a, s);
   else run_tile_pf<T, L, S, CI, false>(a, s);
 }
 
@@ -235,6 +231,7 @@ __host__ __device__ constexpr int x_threads(int nb) { return nb >= 256 ? 256 : (
 //   E  forward remaining stages (LDS), split into nx/2+1 entries, store
 template <typename T, int M, int TLX, int NT, int MODE, int NFI, int NFO, bool SPONGE>
 __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
+  to_member(a, blockIdx.y);
   constexpr int NST = plan_count_k(PLAN_XF, M);
   static_assert(NST >= 2, "x pass needs at least two stages");
   constexpr int RI0 = plan_radix_k(PLAN_XI, M, 0);   // first inverse radix
@@ -518,7 +515,7 @@ static void run_x(const XPassArgs& a, hipStream_t s) {
   static bool init = false;
   if (!init) { set_smem(x_pass_k<T, M, TLX, NT, MODE, NFI, NFO, SP>, smem); init = true; }
   const int blocks = (a.nlines + TLX - 1) / TLX;
-  hipLaunchKernelGGL((x_pass_k<T, M, TLX, NT, MODE, NFI, NFO, SP>), dim3(blocks), dim3(NT), smem, s, a);
+  hipLaunchKernelGGL((x_pass_k<T, M, TLX, NT, MODE, NFI, NFO, SP>), dim3(blocks, batch_n(a.bat)), dim3(NT), smem, s, a);
 }
 
 template <typename T, int M>

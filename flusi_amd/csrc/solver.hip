@@ -13,7 +13,10 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -95,6 +98,7 @@ class SolverBase {
   virtual void fft1(const double* f, double* fk) = 0;
   virtual void ifft1(const double* fk, double* f) = 0;
   virtual void timers(int enable, double* ms) = 0;
+  virtual void autotune_v(int reps, int* variant, double* ms) = 0;
 
   // time_step.f90:57-167 loop body
   void step() {
@@ -137,8 +141,32 @@ class Solver : public SolverBase {
   double* tab = nullptr;    // kx ky kz kxt2 kyt2 kzt2
   double *kx_d, *ky_d, *kz_d, *kxt2_d, *kyt2_d, *kzt2_d;
   void* tw_z = nullptr; void* tw_y = nullptr; void* tw_M = nullptr; void* tw_N = nullptr;
-  double* xpart = nullptr; int nxblocks = 0;
+  double* xpart = nullptr;  // x-pass partial rows (max|u|^2, penalization sums)
   double* caps_d = nullptr;   // bumblebee capsule records
+  // insect mask arguments live in device memory, one record per batch member:
+  // eager calls cycle through INS_SLOTS slots, each filled from a pinned host
+  // copy whose event says when it may be rewritten; a captured step reads
+  // slot s of ins_cap for its s-th mask update, which the host fills before
+  // every replay
+  static constexpr int INS_SLOTS = 8;
+  InsectArgs* ins_d = nullptr;
+  InsectArgs* ins_h = nullptr;
+  int ins_per_slot = 0;
+  hipEvent_t ins_ev[INS_SLOTS] = {};
+  int ins_next = 0;
+  InsectArgs* ins_cap = nullptr;
+  bool capture = false;        // recording a graph: fixed launch shapes, no host reads
+  int stage_slot = 0;
+  // batches: a leader launches every pass once for all members of `batch`
+  // (itself first), whose buffers sit in slabs bat.mstride bytes apart
+  std::vector<Solver<T>*> batch;
+  Batch bat{1, 0};             // the launch batch (nmem > 1 only while a batched step runs)
+  bool ens_step = false;       // an ensemble step is being launched (it records the outputs)
+  char* slab = nullptr;        // setup() places its buffers here when set
+  size_t slab_cap = 0, slab_used = 0;
+  bool in_setup = false;
+  std::vector<void*> owned;    // device allocations freed by this solver
+  bool zwave = false;          // z-forward step pass with in-wave line transforms
   double* fw_d = nullptr;     // Fourier wing radius table
   bool xwave = false;       // right-hand-side x pass with one wavefront per line
   double* zpart = nullptr; int nzblocks = 0;
@@ -157,16 +185,33 @@ class Solver : public SolverBase {
   ~Solver() override {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto& e : ev) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
-    void* ptrs[] = {uk, reg[0], reg[1], Q, P, mask, us, color, phys, tab, tw_z, tw_y, tw_M, tw_N, caps_d, fw_d,
-                    xpart, zpart, dpart, red, rmid, kinds, stage};
-    for (void* q : ptrs) if (q) (void)hipFree(q);
+    for (void* q : owned) (void)hipFree(q);
+    for (hipEvent_t e : ins_ev) if (e) (void)hipEventDestroy(e);
+    if (ins_h) (void)hipHostFree(ins_h);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // device memory: from the slab while setup() runs with one, else its own
+  // allocation; slab_used counts setup()'s bytes either way
   void* dalloc(size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 256) + 255) / 256 * 256;
+    if (in_setup) {
+      slab_used += bytes;
+      if (slab) {
+        if (slab_used > slab_cap) throw FlusiError(900013, "batch slab too small");
+        return slab + (slab_used - bytes);
+      }
+    }
     void* q = nullptr;
-    HIPCHECK(hipMalloc(&q, bytes < 256 ? 256 : bytes));
+    HIPCHECK(hipMalloc(&q, bytes));
+    owned.push_back(q);
     return q;
+  }
+  template <typename P> void dfree(P*& q) {
+    if (!q) return;
+    for (auto& o : owned)
+      if (o == (void*)q) { (void)hipFree(o); o = owned.back(); owned.pop_back(); break; }
+    q = nullptr;
   }
 
   bool fft_ok = false;
@@ -178,6 +223,12 @@ class Solver : public SolverBase {
   }
 
   void setup() {
+    in_setup = true;
+    slab_used = 0;
+    setup_buffers();
+    in_setup = false;
+  }
+  void setup_buffers() {
     nx = p.nx; ny = p.ny; nz = p.nz; nxh = nx / 2 + 1;
     fft_ok = nx % 2 == 0 && x_length_supported(nx / 2) && fft_length_supported(ny) && fft_length_supported(nz);
     npts = (long)nx * ny * nz;
@@ -232,10 +283,15 @@ class Solver : public SolverBase {
     if (fft_ok) {
       tw_z = make_tw(nz, nz); tw_y = make_tw(ny, ny); tw_M = make_tw(nx / 2, nx / 2); tw_N = make_tw(nxh, nx);
       if (const char* e = getenv("FLUSI_TILE_WAVE")) twave = atoi(e) != 0;
+      if (const char* e = getenv("FLUSI_ZWAVE")) zwave = atoi(e) != 0;
+      if (const char* e = getenv("FLUSI_TILE_PREFETCH")) tpf = atoi(e) != 0;
+      if (const char* e = getenv("FLUSI_XWAVE_WPE")) xwpe = atoi(e);
       xwave = x_rhs_wave_supported(nx);
       if (const char* e = getenv("FLUSI_XWAVE")) xwave = xwave && atoi(e) != 0;
-      nxblocks = xwave ? x_rhs_wave_blocks(nx, ny * nz) : x_pass_blocks(nx, ny * nz, FP64);
-      xpart = (double*)dalloc(sizeof(double) * 4 * nxblocks);
+      // partial rows for either x-pass form, so the variant can change later
+      const int nxb = std::max(x_rhs_wave_supported(nx) ? x_rhs_wave_blocks(nx, ny * nz) : 0,
+                               x_pass_blocks(nx, ny * nz, FP64));
+      xpart = (double*)dalloc(sizeof(double) * 4 * nxb);
       nzblocks = zstep_blocks(nz, ny, nxh, FP64);
       zpart = (double*)dalloc(sizeof(double) * nzblocks);
       ndblocks = diag_blocks<T>(geom);
@@ -291,8 +347,14 @@ class Solver : public SolverBase {
 
   // ------------------------------------------------------------ passes
   C* cf(void* base, int f) { return reinterpret_cast<C*>(base) + (long)f * nc; }
+  // kernel variants (chosen by autotune(), or by the FLUSI_* environment overrides)
   bool twave = true;        // y/z passes with the line transform inside one wavefront
-  void tile(const TilePassArgs& a, int L, int dir, bool contig_in) {
+  bool tpf = true;          // block y/z passes: prefetch the next job's operand
+  int xwpe = 3;             // one-wave-per-line x pass: waves-per-SIMD register budget (3 or 0)
+  void tile(const TilePassArgs& a0, int L, int dir, bool contig_in) {
+    TilePassArgs a = a0;
+    a.pf = tpf ? 1 : 0;
+    a.bat = bat;
     if (twave && tile_wave_supported(L)) launch_tile_wave<T>(a, L, dir, contig_in, stream);
     else launch_tile_pass<T>(a, L, dir, contig_in, stream);
   }
@@ -342,7 +404,8 @@ class Solver : public SolverBase {
     XPassArgs a{};
     a.nlines = ny * nz; a.nx = nx; a.ny = ny; a.nz = nz; a.nxh = nxh; a.prow = prow;
     a.kx = kx_d; a.kx_scale = 2.0 * PI / p.xl; a.twM = tw_M; a.twN = tw_N;
-    a.npts = npts;
+    a.npts = npts; a.wpe = xwpe;
+    a.bat = bat;
     return a;
   }
   // the five fields of the z-inverse pass and the six of the y-inverse pass:
@@ -374,8 +437,8 @@ class Solver : public SolverBase {
 
   // rhs.f90:2-115 up to the forward transform: mask, inverse passes, x pass,
   // and the reduction of max|u|^2 and sum chi (u - us) into red[0..3]
-  void pre_rhs(const void* u_in, double t) {
-    if (p.iMoving == 1 && p.iPenalization == 1) create_mask(t);
+  void pre_rhs(const void* u_in, double t, bool with_mask = true) {
+    if (with_mask && p.iMoving == 1 && p.iPenalization == 1) create_mask(t);
     if (p.eps < 1.0e-11) throw FlusiError(77363800, "Value of eps is very small, maybe even zero.");
     inv_uw(u_in);
     XPassArgs a = xargs();
@@ -393,7 +456,13 @@ class Solver : public SolverBase {
       else launch_x_pass<T>(a, XM_RHS, stream);
     });
     kcheck("x pass");
-    if (a.want_stats) launch_reduce_rows(xpart, nxblocks, 4, kinds, rmid, red, stream);
+    if (a.want_stats) {
+      const int nxblocks = xwave ? x_rhs_wave_blocks(nx, ny * nz) : x_pass_blocks(nx, ny * nz, FP64);
+      for (int m = 0; m < batch_n(bat); ++m) {
+        const long o = m * bat.mstride;
+        launch_reduce_rows(mshift(xpart, o), nxblocks, 4, mshift(kinds, o), mshift(rmid, o), mshift(red, o), stream);
+      }
+    }
     kcheck("reduce x");
   }
 
@@ -410,6 +479,8 @@ class Solver : public SolverBase {
     if (p.startup_conditioner == "yes") sc *= startup_conditioner(time, p.T_release_meanflow, p.tau_meanflow);
     z.dynamic_scale = sc / p.m_fluid;
     for (int c = 0; c < 3; ++c) z.meanflow_dynamic[c] = p.iMeanFlow[c] == "dynamic";
+    z.wave = zwave ? 1 : 0;
+    z.bat = bat;
     return z;
   }
   void post_rhs(ZStepArgs& z) {
@@ -466,14 +537,110 @@ class Solver : public SolverBase {
     return d;
   }
 
-  void record_ekin() {
-    if (out_dir.empty()) return;
+  // force: record also inside an ensemble step (whose bookkeeping calls this
+  // for every member after the step)
+  void record_ekin(bool force = false) {
+    if (out_dir.empty() || (ens_step && !force)) return;
     launch_reduce_rows(zpart, nzblocks, 1, kinds + 4, rmid, red + 8, stream);
     double e = 0.0;
     HIPCHECK(hipMemcpyAsync(&e, red + 8, sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     append("ekin.t", {time + dt1, e * p.xl * p.yl * p.zl}, "%          time            ekin");
   }
+
+  // ------------------------------------------------------------ autotuning
+  // The right-hand side has interchangeable kernel variants: in-wave or block
+  // y/z passes, operand prefetch in the block passes, the one-wave-per-line or
+  // the block x pass with its register budget, and the in-wave or block z
+  // step.  autotune() runs `reps` right-hand sides of the current state (the
+  // whole batch when one is active) for every variant the grid allows and
+  // keeps the fastest; results are cached per grid, precision, sponge and
+  // batch size for the life of the process.
+  struct Variant { int twave, tpf, xwave, xwpe, zwave; };
+  Variant variant() const { return Variant{twave, tpf, xwave, xwpe, zwave}; }
+  void apply(const Variant& v) {
+    twave = v.twave != 0; tpf = v.tpf != 0; xwave = v.xwave != 0; xwpe = v.xwpe; zwave = v.zwave != 0;
+  }
+  double time_rhs(int reps) {
+    auto rhs = [&] {
+      pre_rhs(uk, time, false);
+      ZStepArgs z = zargs(SM_RHS_ONLY);
+      z.rhs_out = P; z.udiff = uk;
+      post_rhs(z);
+    };
+    rhs();  // first launches of a variant set its LDS limits
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0)); HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; ++r) rhs();
+    HIPCHECK(hipEventRecord(e1, stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    return ms / std::max(reps, 1);
+  }
+  std::vector<Variant> candidates() const {
+    const bool tw_any = tile_wave_supported(ny) || tile_wave_supported(nz);
+    const bool tw_all = tile_wave_supported(ny) && tile_wave_supported(nz);
+    const bool xw = x_rhs_wave_supported(nx), zw = tile_wave_supported(nz);
+    std::vector<Variant> c;
+    for (int tw : {1, 0}) {
+      if (tw && !tw_any) continue;
+      for (int pf : {1, 0}) {
+        if (!pf && tw && tw_all) continue;  // prefetch only acts in block passes
+        for (int xv : {1, 0}) {
+          if (xv && !xw) continue;
+          for (int wpe : {3, 0}) {
+            if (!xv && wpe == 0) continue;  // register budget only acts in the wave x pass
+            for (int zv : {0, 1}) {
+              if (zv && !zw) continue;
+              c.push_back(Variant{tw, pf, xv, wpe, zv});
+            }
+          }
+        }
+      }
+    }
+    return c;
+  }
+  static std::map<std::string, std::pair<Variant, double>>& tune_cache() {
+    static std::map<std::string, std::pair<Variant, double>> m;
+    return m;
+  }
+  static std::mutex& tune_mutex() { static std::mutex mu; return mu; }
+  std::string tune_key() const {
+    return std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz) + (FP64 ? ":f64" : ":f32") +
+           ":sp" + std::to_string(sponge_code()) + ":m" + std::to_string(batch_n(bat));
+  }
+  // v: the chosen variant (5 ints: twave, tpf, xwave, xwpe, zwave); ms: its
+  // time per right-hand side; force: tune again even when cached
+  void autotune(int reps, int* v, double* ms, bool force) {
+    need_fft();
+    if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    const std::string key = tune_key();
+    std::pair<Variant, double> best{variant(), 1e300};
+    bool hit = false;
+    {
+      std::lock_guard<std::mutex> g(tune_mutex());
+      auto it = tune_cache().find(key);
+      if (it != tune_cache().end() && !force) { best = it->second; hit = true; }
+    }
+    if (!hit) {
+      const Variant keep = variant();
+      for (const Variant& c : candidates()) {
+        apply(c);
+        const double t = time_rhs(reps);
+        if (t < best.second) best = {c, t};
+      }
+      apply(keep);
+      std::lock_guard<std::mutex> g(tune_mutex());
+      tune_cache()[key] = best;
+    }
+    apply(best.first);
+    if (v) { v[0] = best.first.twave; v[1] = best.first.tpf; v[2] = best.first.xwave; v[3] = best.first.xwpe; v[4] = best.first.zwave; }
+    if (ms) *ms = best.second;
+  }
+  void autotune_v(int reps, int* v, double* ms) override { autotune(reps, v, ms, false); }
 
   // ------------------------------------------------------------ steppers
   void fluid_timestep() override {
@@ -545,18 +712,56 @@ class Solver : public SolverBase {
   void create_mask(double t) override {
     if (p.iMask == "Insect" && (p.iMoving != 1 || p.iPenalization != 1))
       throw FlusiError(4453, "insects.f90::DrawInsect: the parameters iMoving or iPenalization are wrong.");
+    const int M = batch_n(bat);
+    auto member = [&](int m) -> Solver<T>& { return M > 1 ? *batch[m] : *this; };
     timed(TM_MASK, [&] {
       if (p.iPenalization == 1) {
         if (p.iMask == "sphere" || p.iMask == "Sphere") {
-          SphereArgs a{};
-          a.x0 = p.x0; a.y0 = p.y0; a.z0 = p.z0; a.radius = 0.5 * p.length;
-          const double h = std::max(p.dx(), std::max(p.dy(), p.dz()));
-          a.smoothing = 1.5 * h; a.safety = 2.0 * 1.5 * h; a.color_thr = (double)1.0e-12f;
-          launch_sphere<T>(geom, a, mask, us, color, stream);
+          for (int m = 0; m < M; ++m) {  // each member's own sphere
+            const Params& q = member(m).p;
+            SphereArgs a{};
+            a.x0 = q.x0; a.y0 = q.y0; a.z0 = q.z0; a.radius = 0.5 * q.length;
+            const double h = std::max(q.dx(), std::max(q.dy(), q.dz()));
+            a.smoothing = 1.5 * h; a.safety = 2.0 * 1.5 * h; a.color_thr = (double)1.0e-12f;
+            const long o = m * bat.mstride;
+            launch_sphere<T>(geom, a, mshift((char*)mask, o), mshift((char*)us, o), mshift(color, o), stream);
+          }
         } else if (p.iMask == "Insect" || p.iMask == "insect") {
-          insect.update(t);
-          InsectArgs a = insect_args();
-          launch_insect<T>(geom, a, mask, us, color, stream);
+          // one argument record per member, all over the same region (the
+          // whole grid when batched or captured)
+          const bool full = M > 1 || capture;
+          std::vector<InsectArgs> recs(M);
+          for (int m = 0; m < M; ++m) {
+            member(m).insect.update(t);
+            recs[m] = member(m).insect_args(full);
+          }
+          const InsectArgs* dev;
+          if (capture) {
+            if (stage_slot >= INS_SLOTS) throw FlusiError(900012, "too many mask updates in a captured step");
+            if (!ins_cap) ins_cap = (InsectArgs*)dalloc(sizeof(InsectArgs) * INS_SLOTS * M);
+            dev = ins_cap + (size_t)stage_slot++ * M;
+          } else {
+            if (ins_per_slot < M) {  // (re)size the ring for M records per slot
+              HIPCHECK(hipStreamSynchronize(stream));
+              dfree(ins_d);
+              if (ins_h) HIPCHECK(hipHostFree(ins_h));
+              ins_h = nullptr;
+              ins_per_slot = M;
+              ins_d = (InsectArgs*)dalloc(sizeof(InsectArgs) * INS_SLOTS * M);
+              HIPCHECK(hipHostMalloc((void**)&ins_h, sizeof(InsectArgs) * INS_SLOTS * M, hipHostMallocDefault));
+            }
+            const int slot = ins_next;
+            ins_next = (ins_next + 1) % INS_SLOTS;
+            if (ins_ev[slot]) HIPCHECK(hipEventSynchronize(ins_ev[slot]));
+            else HIPCHECK(hipEventCreateWithFlags(&ins_ev[slot], hipEventDisableTiming));
+            InsectArgs* h = ins_h + (size_t)slot * ins_per_slot;
+            std::copy(recs.begin(), recs.end(), h);
+            InsectArgs* d = ins_d + (size_t)slot * ins_per_slot;
+            HIPCHECK(hipMemcpyAsync(d, h, sizeof(InsectArgs) * M, hipMemcpyHostToDevice, stream));
+            HIPCHECK(hipEventRecord(ins_ev[slot], stream));
+            dev = d;
+          }
+          launch_insect<T>(geom, recs[0], dev, mask, us, color, stream, bat);
         } else if (p.iMask == "none" || p.iMask == "empty" || p.iMask == "no") {
         } else {
           throw FlusiError(3333, "create_mask_fsi(): unkown mask function iMask");
@@ -568,13 +773,18 @@ class Solver : public SolverBase {
         if (p.iCavity == "yes") { c.ux = c.uy = c.uz = 0.0; }
         else if (p.iCavity == "freestream") { c.ux = p.umean[0]; c.uy = p.umean[1]; c.uz = p.umean[2]; }
         else throw FlusiError(3334, "iCavity not supported: " + p.iCavity);
-        launch_cavity<T>(geom, c, mask, us, color, stream);
+        for (int m = 0; m < M; ++m) {
+          const long o = m * bat.mstride;
+          launch_cavity<T>(geom, c, mshift((char*)mask, o), mshift((char*)us, o), mshift(color, o), stream);
+        }
       }
     });
     kcheck("create_mask");
   }
 
-  InsectArgs insect_args() {
+  // full: visit the whole grid (a batched or captured launch has one fixed
+  // region for every member and stage)
+  InsectArgs insect_args(bool full = false) {
     Insect& I = insect;
     InsectArgs a{};
     const double h = std::max(p.dx(), std::max(p.dy(), p.dz()));
@@ -723,7 +933,7 @@ class Solver : public SolverBase {
     // (the only places where wing colours can be left); everything on the
     // first call and for periodic insects
     const int nn[3] = {nx, ny, nz};
-    if (!a.use_box || a.full_reset) {
+    if (!a.use_box || a.full_reset || full) {
       for (int i = 0; i < 3; ++i) { a.reg_lo[i] = 0; a.reg_hi[i] = nn[i] - 1; }
     } else {
       for (int i = 0; i < 3; ++i) { a.reg_lo[i] = nn[i]; a.reg_hi[i] = -1; }
@@ -764,7 +974,7 @@ class Solver : public SolverBase {
   // ------------------------------------------------------------ host copies
   double* staging(size_t n) {
     if (n > stage_n) {
-      if (stage) HIPCHECK(hipFree(stage));
+      dfree(stage);
       stage = (double*)dalloc(n * sizeof(double));
       stage_n = n;
     }
@@ -1045,9 +1255,376 @@ class Solver : public SolverBase {
 struct flusi_ctx {
   std::unique_ptr<flusi::SolverBase> s;
   std::string err;
+  bool borrowed = false;  // a member of an ensemble, freed with it
 };
 
+namespace flusi {
+
+// ------------------------------------------------------------ ensembles
+// Many independent simulations advanced together.  Members whose grids and
+// kernel-relevant parameters agree form a batch: each member's device state
+// lives in one slab, the slabs of a batch sit a fixed stride apart, and every
+// pass of a step is one launch over the whole batch (one grid dimension per
+// member).  One step of all batches can be captured into a hipGraph (one
+// branch per batch, forked from an origin stream) and replayed; before each
+// replay the host writes only the moving insects' per-stage mask arguments.
+
+// parameters that enter the batched kernels' arguments (the mask geometry
+// does not: spheres are drawn per member, insects get one record per member)
+static std::string batch_key(const Params& p, int precision) {
+  char b[1024];
+  std::snprintf(b, sizeof(b),
+                "%d|%d %d %d|%.17g %.17g %.17g|%s|%.17g|%.17g|%d|%d %d|%s|%.17g|%s %d|%s %s %.17g %d|%s %s %s|%.17g|"
+                "%.17g %.17g %.17g|%.17g|%s %.17g %.17g|%d",
+                precision, p.nx, p.ny, p.nz, p.xl, p.yl, p.zl, p.method.c_str(), p.dt_fixed, p.nu, p.iDealias,
+                p.iPenalization, p.iMoving, p.iMask.c_str(), p.eps, p.iCavity.c_str(), p.cavity_size,
+                p.iVorticitySponge.c_str(), p.iSpongeType.c_str(), p.eps_sponge, p.sponge_thickness,
+                p.iMeanFlow[0].c_str(), p.iMeanFlow[1].c_str(), p.iMeanFlow[2].c_str(), p.m_fluid, p.umean[0],
+                p.umean[1], p.umean[2], p.umean_freq, p.startup_conditioner.c_str(), p.tau_meanflow,
+                p.T_release_meanflow, p.periodic ? 1 : 0);
+  return b;
+}
+
+class EnsembleBase {
+ public:
+  virtual ~EnsembleBase() = default;
+  std::string err;
+  std::vector<flusi_ctx*> ctxs;
+  virtual int nbatches() const = 0;
+  virtual void batch_info(int b, int* nmem, int* nx, int* ny, int* nz, int* lead) const = 0;
+  virtual void init_fields() = 0;
+  virtual void autotune(int reps, int* variants, double* ms) = 0;
+  virtual void advance(int nsteps, int mode) = 0;
+};
+
+template <typename T>
+class Ensemble : public EnsembleBase {
+  struct Group {
+    std::vector<int> idx;   // member indices, leader first
+    char* slab = nullptr;
+    size_t stride = 0;
+    int nstage = 0;         // insect mask updates per captured step
+    hipEvent_t join = nullptr;
+    Solver<T>* lead = nullptr;
+  };
+  std::vector<Group> groups;
+  hipStream_t origin = nullptr;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  static constexpr int RING = 4;  // pinned buffers of per-replay mask arguments
+  InsectArgs* ring_h[RING] = {};
+  hipEvent_t ring_ev[RING] = {};
+  size_t ring_n = 0;
+  int ring_next = 0;
+
+  Solver<T>& S(int i) { return *static_cast<Solver<T>*>(ctxs[i]->s.get()); }
+
+ public:
+  // separate from the constructor so that a failure part-way is cleaned up
+  // by the destructor
+  void build(const std::vector<std::string>& paths, int precision) {
+    HIPCHECK(hipStreamCreate(&origin));
+    HIPCHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    const int n = (int)paths.size();
+    std::vector<std::unique_ptr<IniFile>> inis(n);
+    std::vector<Params> ps(n);
+    std::vector<std::string> keys(n);
+    for (int i = 0; i < n; ++i) {
+      inis[i].reset(new IniFile(paths[i]));
+      ps[i] = load_params(*inis[i]);
+      keys[i] = batch_key(ps[i], precision);
+    }
+    for (int i = 0; i < n; ++i) {
+      bool placed = false;
+      for (Group& g : groups)
+        if (keys[g.idx[0]] == keys[i]) { g.idx.push_back(i); placed = true; break; }
+      if (!placed) { groups.emplace_back(); groups.back().idx.push_back(i); }
+    }
+    ctxs.assign(n, nullptr);
+    for (Group& g : groups) {
+      {  // one member's setup() bytes, from a throw-away solver
+        Solver<T> probe;
+        probe.p = ps[g.idx[0]];
+        probe.setup();
+        probe.need_fft();
+        g.stride = (probe.slab_used + 65535) / 65536 * 65536;
+      }
+      HIPCHECK(hipMalloc((void**)&g.slab, g.stride * g.idx.size()));
+      HIPCHECK(hipEventCreateWithFlags(&g.join, hipEventDisableTiming));
+      for (size_t k = 0; k < g.idx.size(); ++k) {
+        const int i = g.idx[k];
+        auto c = new flusi_ctx();
+        c->borrowed = true;
+        ctxs[i] = c;
+        auto s = new Solver<T>();
+        c->s.reset(s);
+        s->ini = std::move(inis[i]);
+        s->p = ps[i];
+        const size_t sl = paths[i].find_last_of('/');
+        s->base_dir = (sl == std::string::npos) ? "." : paths[i].substr(0, sl);
+        s->slab = g.slab + k * g.stride;
+        s->slab_cap = g.stride;
+        s->setup();
+      }
+      // every device pointer of a member must sit k * stride after the leader's
+      Solver<T>& L = S(g.idx[0]);
+      for (size_t k = 1; k < g.idx.size(); ++k) {
+        Solver<T>& m = S(g.idx[k]);
+        const void* a[] = {L.uk, L.reg[0], L.reg[1], L.Q, L.P, L.mask, L.us, L.color, L.tab, L.tw_z, L.tw_y, L.tw_M,
+                           L.tw_N, L.xpart, L.zpart, L.dpart, L.red, L.rmid, L.kinds};
+        const void* b[] = {m.uk, m.reg[0], m.reg[1], m.Q, m.P, m.mask, m.us, m.color, m.tab, m.tw_z, m.tw_y, m.tw_M,
+                           m.tw_N, m.xpart, m.zpart, m.dpart, m.red, m.rmid, m.kinds};
+        for (size_t j = 0; j < sizeof(a) / sizeof(a[0]); ++j)
+          if ((const char*)b[j] - (const char*)a[j] != (long)(k * g.stride))
+            throw FlusiError(900014, "ensemble: member buffers do not follow the batch layout");
+      }
+      g.lead = &L;
+      for (int i : g.idx) L.batch.push_back(&S(i));
+    }
+  }
+  ~Ensemble() override {
+    if (origin) (void)hipStreamSynchronize(origin);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (flusi_ctx* c : ctxs) delete c;  // solvers first: their buffers live in the slabs
+    for (Group& g : groups) {
+      if (g.slab) (void)hipFree(g.slab);
+      if (g.join) (void)hipEventDestroy(g.join);
+    }
+    for (int r = 0; r < RING; ++r) {
+      if (ring_ev[r]) (void)hipEventDestroy(ring_ev[r]);
+      if (ring_h[r]) (void)hipHostFree(ring_h[r]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (origin) (void)hipStreamDestroy(origin);
+  }
+
+  int nbatches() const override { return (int)groups.size(); }
+  void batch_info(int b, int* nmem, int* nx, int* ny, int* nz, int* lead) const override {
+    if (b < 0 || b >= (int)groups.size()) throw FlusiError(900015, "ensemble: batch index out of range");
+    const Group& g = groups[b];
+    if (nmem) *nmem = (int)g.idx.size();
+    if (lead) *lead = g.idx[0];
+    if (nx) *nx = g.lead->nx;
+    if (ny) *ny = g.lead->ny;
+    if (nz) *nz = g.lead->nz;
+  }
+
+  void init_fields() override {
+    for (flusi_ctx* c : ctxs) c->s->init_fields();
+  }
+
+  struct BatchScope {  // the leader launches for its whole batch while alive
+    Solver<T>& L;
+    BatchScope(Solver<T>& l, size_t n, size_t stride) : L(l) {
+      L.bat = Batch{(int)n, (long)stride};
+      L.ens_step = true;
+    }
+    ~BatchScope() { L.bat = Batch{1, 0}; L.ens_step = false; }
+  };
+
+  void autotune(int reps, int* variants, double* ms) override {
+    for (size_t b = 0; b < groups.size(); ++b) {
+      Group& g = groups[b];
+      BatchScope bs(*g.lead, g.idx.size(), g.stride);
+      g.lead->autotune(reps, variants ? variants + 5 * b : nullptr, ms ? ms + b : nullptr, false);
+      for (int i : g.idx) S(i).apply(g.lead->variant());
+    }
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; (void)hipGraphDestroy(graph); graph = nullptr; }
+  }
+
+  // host bookkeeping of one step for every member (time_step.f90:57-167),
+  // and their output at output times
+  void after_step(bool synced) {
+    for (Group& g : groups) {
+      Solver<T>& L = *g.lead;
+      for (int i : g.idx) {
+        Solver<T>& m = S(i);
+        if (&m != &L) { m.dt0 = L.dt0; m.dt1 = L.dt1; m.vis_dt = L.vis_dt; m.n0 = L.n0; m.n1 = L.n1; }
+      }
+    }
+    for (flusi_ctx* c : ctxs) {
+      Solver<T>& m = *static_cast<Solver<T>*>(c->s.get());
+      if (!m.out_dir.empty()) {
+        if (!synced) { HIPCHECK(hipDeviceSynchronize()); synced = true; }
+        m.record_ekin(true);
+      }
+      m.time = m.time + m.dt1;
+      m.it += 1;
+      if (!m.out_dir.empty() &&
+          time_for_output(m.time, m.dt1, m.it, m.p.tintegral, m.p.itdrag, m.p.tmax, 0.0, m.p.intelligent_dt == "yes")) {
+        if (!synced) { HIPCHECK(hipDeviceSynchronize()); synced = true; }
+        m.write_integrals(m.time, nullptr);
+      }
+    }
+  }
+
+  void batched_step() {
+    for (Group& g : groups) {
+      Solver<T>& L = *g.lead;
+      if (L.p.dt_fixed <= 0.0 && g.idx.size() > 1)
+        throw FlusiError(900016, "ensemble: batched steps need dt_fixed > 0 (one time step for the whole batch)");
+      BatchScope bs(L, g.idx.size(), g.stride);
+      L.dt0 = L.dt1;
+      L.fluid_timestep();
+    }
+    after_step(false);
+  }
+
+  void check_capturable(const Solver<T>& L) const {
+    const Params& p = L.p;
+    if (p.method != "RK2" && p.method != "RK4")
+      throw FlusiError(900017, "ensemble graph: RK2 or RK4 only (AB2 alternates its registers)");
+    if (p.dt_fixed <= 0.0) throw FlusiError(900017, "ensemble graph: dt_fixed > 0 required");
+    if (p.startup_conditioner == "yes") throw FlusiError(900017, "ensemble graph: startup_conditioner depends on time");
+    for (int c = 0; c < 3; ++c)
+      if (p.iMeanFlow[c] == "sinusoidal") throw FlusiError(900017, "ensemble graph: sinusoidal mean flow depends on time");
+  }
+  int mask_updates(const Solver<T>& L) const {
+    if (!(L.p.iMoving == 1 && L.p.iPenalization == 1 && (L.p.iMask == "Insect" || L.p.iMask == "insect"))) return 0;
+    return L.p.method == "RK4" ? 4 : 2;
+  }
+
+  void capture() {
+    for (Group& g : groups) {
+      check_capturable(*g.lead);
+      g.nstage = mask_updates(*g.lead);
+      Solver<T>& L = *g.lead;
+      L.timers(0, nullptr);  // no event timing inside the graph
+      if (g.nstage && !L.ins_cap)
+        L.ins_cap = (InsectArgs*)L.dalloc(sizeof(InsectArgs) * Solver<T>::INS_SLOTS * g.idx.size());
+    }
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+    try {
+      HIPCHECK(hipEventRecord(fork, origin));
+      for (Group& g : groups) {
+        Solver<T>& L = *g.lead;
+        HIPCHECK(hipStreamWaitEvent(L.stream, fork, 0));
+        BatchScope bs(L, g.idx.size(), g.stride);
+        // the host state is restored afterwards: recording runs no step
+        const double d0 = L.dt0, d1 = L.dt1, vd = L.vis_dt;
+        const int n0 = L.n0, n1 = L.n1;
+        std::vector<Insect> saved;
+        for (int i : g.idx) saved.push_back(S(i).insect);
+        L.capture = true;
+        L.stage_slot = 0;
+        L.dt0 = L.dt1;
+        try {
+          L.fluid_timestep();
+        } catch (...) {
+          L.capture = false;
+          throw;
+        }
+        L.capture = false;
+        if (L.stage_slot != g.nstage) throw FlusiError(900018, "ensemble graph: unexpected number of mask updates");
+        L.dt0 = d0; L.dt1 = d1; L.vis_dt = vd; L.n0 = n0; L.n1 = n1;
+        for (size_t k = 0; k < g.idx.size(); ++k) S(g.idx[k]).insect = saved[k];
+        HIPCHECK(hipEventRecord(g.join, L.stream));
+        HIPCHECK(hipStreamWaitEvent(origin, g.join, 0));
+      }
+    } catch (...) {
+      hipGraph_t gdrop = nullptr;
+      (void)hipStreamEndCapture(origin, &gdrop);
+      if (gdrop) (void)hipGraphDestroy(gdrop);
+      throw;
+    }
+    HIPCHECK(hipStreamEndCapture(origin, &graph));
+    HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    size_t need = 0;
+    for (Group& g : groups) need += (size_t)g.nstage * g.idx.size();
+    if (need > ring_n) {
+      for (int r = 0; r < RING; ++r) {
+        if (ring_h[r]) { if (ring_ev[r]) HIPCHECK(hipEventSynchronize(ring_ev[r])); HIPCHECK(hipHostFree(ring_h[r])); }
+        HIPCHECK(hipHostMalloc((void**)&ring_h[r], sizeof(InsectArgs) * need, hipHostMallocDefault));
+      }
+      ring_n = need;
+    }
+  }
+
+  void graph_step() {
+    const int r = ring_next;
+    ring_next = (ring_next + 1) % RING;
+    if (ring_ev[r]) HIPCHECK(hipEventSynchronize(ring_ev[r]));
+    else HIPCHECK(hipEventCreateWithFlags(&ring_ev[r], hipEventDisableTiming));
+    size_t off = 0;
+    for (Group& g : groups) {
+      if (!g.nstage) continue;
+      Solver<T>& L = *g.lead;
+      const size_t M = g.idx.size();
+      const double t = L.time, dt = L.p.dt_fixed;
+      // the stage times at which fluid_timestep redraws the mask
+      const double ts[4] = {t, g.nstage == 4 ? t + 0.5 * dt : t, t + 0.5 * dt, t + dt};
+      InsectArgs* h = ring_h[r] + off;
+      for (int s = 0; s < g.nstage; ++s)
+        for (size_t k = 0; k < M; ++k) {
+          Solver<T>& m = S(g.idx[k]);
+          m.insect.update(ts[s]);
+          h[s * M + k] = m.insect_args(true);
+        }
+      HIPCHECK(hipMemcpyAsync(L.ins_cap, h, sizeof(InsectArgs) * g.nstage * M, hipMemcpyHostToDevice, origin));
+      off += g.nstage * M;
+    }
+    HIPCHECK(hipEventRecord(ring_ev[r], origin));
+    HIPCHECK(hipGraphLaunch(exec, origin));
+    for (Group& g : groups) g.lead->dt0 = g.lead->dt1;
+    after_step(false);
+  }
+
+  // mode 0: every member steps on its own; 1: batched launches; 2: batched
+  // launches replayed from a captured graph
+  void advance(int nsteps, int mode) override {
+    for (flusi_ctx* c : ctxs)
+      if (!c->s->fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_ensemble_init_fields)");
+    if (mode == 0) {
+      for (int k = 0; k < nsteps; ++k)
+        for (flusi_ctx* c : ctxs) c->s->step();
+      HIPCHECK(hipDeviceSynchronize());
+      return;
+    }
+    if (mode != 1 && mode != 2) throw FlusiError(900019, "ensemble: mode must be 0, 1 or 2");
+    int k = 0;
+    if (mode == 2) {
+      // the captured step assumes an established integrating factor
+      for (; k < nsteps; ++k) {
+        bool fresh = false;
+        for (flusi_ctx* c : ctxs) fresh = fresh || c->s->it == 0;
+        if (!fresh) break;
+        batched_step();
+      }
+      if (k < nsteps && !exec) capture();
+      for (; k < nsteps; ++k) graph_step();
+    } else {
+      for (; k < nsteps; ++k) batched_step();
+    }
+    HIPCHECK(hipStreamSynchronize(origin));
+    for (Group& g : groups) HIPCHECK(hipStreamSynchronize(g.lead->stream));
+  }
+};
+
+}  // namespace flusi
+
 using flusi::FlusiError;
+
+struct flusi_ensemble {
+  std::unique_ptr<flusi::EnsembleBase> e;
+  std::string err;
+};
+
+template <typename F> static int eguarded(flusi_ensemble* c, F&& f) {
+  if (!c || !c->e) return 900000;
+  try {
+    f();
+    return 0;
+  } catch (const FlusiError& x) {
+    c->err = x.what();
+    return x.code;
+  } catch (const std::exception& x) {
+    c->err = x.what();
+    return 900099;
+  }
+}
 
 template <typename F> static int guarded(flusi_ctx* c, F&& f) {
   if (!c || !c->s) return 900000;
@@ -1097,7 +1674,9 @@ int flusi_init(const char* ini_path, int precision, flusi_ctx** out) {
   return 0;
 }
 
-void flusi_free(flusi_ctx* ctx) { delete ctx; }
+void flusi_free(flusi_ctx* ctx) {
+  if (ctx && !ctx->borrowed) delete ctx;  // ensemble members go with their ensemble
+}
 
 const char* flusi_last_error(const flusi_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
@@ -1162,11 +1741,71 @@ int flusi_energy_k(flusi_ctx* ctx, double* E) { return guarded(ctx, [&] { *E = c
 int flusi_cal_vis(flusi_ctx* ctx, double dt, double* vis) { return guarded(ctx, [&] { ctx->s->cal_vis(dt, vis); }); }
 int flusi_fft(flusi_ctx* ctx, const double* f, double* fk) { return guarded(ctx, [&] { ctx->s->fft1(f, fk); }); }
 int flusi_ifft(flusi_ctx* ctx, const double* fk, double* f) { return guarded(ctx, [&] { ctx->s->ifft1(fk, f); }); }
+int flusi_autotune(flusi_ctx* ctx, int reps, int* variant, double* ms) {
+  return guarded(ctx, [&] { ctx->s->autotune_v(reps, variant, ms); });
+}
 int flusi_timers(flusi_ctx* ctx, int enable, double* ms, const char** names) {
   return guarded(ctx, [&] {
     ctx->s->timers(enable, ms);
     if (names) for (int i = 0; i < FLUSI_TIMER_SLOTS; ++i) names[i] = flusi::kTimerNames[i];
   });
+}
+
+// ensembles
+int flusi_ensemble_create(int n, const char* const* ini_paths, int precision, flusi_ensemble** out) {
+  if (!out || n < 1 || !ini_paths) return 900000;
+  *out = nullptr;
+  auto e = new flusi_ensemble();
+  try {
+    std::vector<std::string> paths(ini_paths, ini_paths + n);
+    if (precision == FLUSI_FP64) {
+      auto x = new flusi::Ensemble<double>();
+      e->e.reset(x);
+      x->build(paths, precision);
+    } else if (precision == FLUSI_FP32_MIXED) {
+      auto x = new flusi::Ensemble<float>();
+      e->e.reset(x);
+      x->build(paths, precision);
+    } else {
+      throw FlusiError(900005, "precision must be FLUSI_FP64 or FLUSI_FP32_MIXED");
+    }
+  } catch (const FlusiError& x) {
+    e->err = x.what();
+    e->e.reset();
+    *out = e;
+    return x.code;
+  } catch (const std::exception& x) {
+    e->err = x.what();
+    e->e.reset();
+    *out = e;
+    return 900099;
+  }
+  *out = e;
+  return 0;
+}
+void flusi_ensemble_free(flusi_ensemble* e) { delete e; }
+const char* flusi_ensemble_last_error(const flusi_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+int flusi_ensemble_size(const flusi_ensemble* e, int* nmembers, int* nbatches) {
+  if (!e || !e->e) return 900000;
+  if (nmembers) *nmembers = (int)e->e->ctxs.size();
+  if (nbatches) *nbatches = e->e->nbatches();
+  return 0;
+}
+int flusi_ensemble_batch(flusi_ensemble* e, int b, int* nmem, int* nx, int* ny, int* nz, int* lead) {
+  return eguarded(e, [&] { e->e->batch_info(b, nmem, nx, ny, nz, lead); });
+}
+int flusi_ensemble_member(flusi_ensemble* e, int i, flusi_ctx** member) {
+  return eguarded(e, [&] {
+    if (i < 0 || i >= (int)e->e->ctxs.size()) throw FlusiError(900015, "ensemble: member index out of range");
+    *member = e->e->ctxs[i];
+  });
+}
+int flusi_ensemble_init_fields(flusi_ensemble* e) { return eguarded(e, [&] { e->e->init_fields(); }); }
+int flusi_ensemble_autotune(flusi_ensemble* e, int reps, int* variants, double* ms) {
+  return eguarded(e, [&] { e->e->autotune(reps, variants, ms); });
+}
+int flusi_ensemble_advance(flusi_ensemble* e, int nsteps, int mode) {
+  return eguarded(e, [&] { e->e->advance(nsteps, mode); });
 }
 
 }  // extern "C"

@@ -58,23 +58,24 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
   cplx<T>* ln = tile + line * LS;
   const int b0 = blockIdx.x * TL;
   const long c = blockIdx.y;
+  const long moff = (long)blockIdx.z * a.bat.mstride;  // batch member's slab
   const T sc = (T)a.scale;
   const cplx<T>* tw;
   if constexpr (Sh::TWL) {
     cplx<T>* twl = tile + TL * LS;
-    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(mshift(a.tw, moff));
     for (int i = tid; i < L; i += NT) twl[i] = twg[i];
     tw = twl;
     __syncthreads();
   } else {
-    tw = reinterpret_cast<const cplx<T>*>(a.tw);
+    tw = reinterpret_cast<const cplx<T>*>(mshift(a.tw, moff));
   }
 
   // operands: CONTIG_IN - the lane's 8 positions of its own line;
   // otherwise - the thread's K tile elements (bb = e % TL, a = e / TL)
   cplx<T> r0[8], r1[8];
   auto load = [&](int jb, int which) {
-    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(which ? a.jobs[jb].in1 : a.jobs[jb].in0);
+    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(mshift(which ? a.jobs[jb].in1 : a.jobs[jb].in0, moff));
     if (CONTIG_IN) {
       const bool ok = b0 + line < a.nb;
       const long base = c * a.sin.c + (long)(b0 + line) * a.sin.b + l;
@@ -100,7 +101,7 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
     if (a.jobs[jb].in1) load(jb, 1);
     for (int o = 0; o < nout; ++o) {
       const int op = o ? a.jobs[jb].op2 : a.jobs[jb].op;
-      cplx<T>* out = reinterpret_cast<cplx<T>*>(o ? a.jobs[jb].out2 : a.jobs[jb].out);
+      cplx<T>* out = reinterpret_cast<cplx<T>*>(mshift(o ? a.jobs[jb].out2 : a.jobs[jb].out, moff));
       const bool pf = (o == nout - 1) && (jb + 1 < a.njobs);
       cplx<T> v[8];
       if (CONTIG_IN) {
@@ -148,7 +149,7 @@ void run_tw(const TilePassArgs& a, hipStream_t s) {
   using Sh = TWShape<T, L>;
   static bool init = false;
   if (!init) { set_smem(tile_wave_k<T, L, S, CI>, Sh::smem); init = true; }
-  dim3 grid((a.nb + Sh::TL - 1) / Sh::TL, a.nc);
+  dim3 grid((a.nb + Sh::TL - 1) / Sh::TL, a.nc, batch_n(a.bat));
   hipLaunchKernelGGL((tile_wave_k<T, L, S, CI>), grid, dim3(Sh::NT), Sh::smem, s, a);
 }
 

@@ -127,6 +127,7 @@ __device__ __forceinline__ void wfft(cplx<T> (&v)[4], const cplx<T>* tw, cplx<T>
 // WPE: minimum waves per SIMD the register allocation must allow (0: none)
 template <typename T, int M, bool SPONGE, bool STATS, int WPE>
 __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1))) void x_rhs_wave_k(XPassArgs a) {
+  to_member(a, blockIdx.y);
   constexpr int G = M / 4, LPW = 64 / G, NXH = M + 1;
   constexpr int LS = line_stride<T>(M + 1);
   constexpr int WPB = XW_NT / 64;
@@ -320,14 +321,13 @@ void run_wave(const XPassArgs& a, hipStream_t s) {
   constexpr int LPB = (XW_NT / 64) * (256 / M);  // lines per block
   const int blocks = (a.nlines + LPB - 1) / LPB;
   if (a.sponge) {
-    if (a.want_stats) hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, true, 0>), dim3(blocks), dim3(XW_NT), 0, s, a);
-    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, false, 0>), dim3(blocks), dim3(XW_NT), 0, s, a);
+    if (a.want_stats) hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, true, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
+    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, false, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
   } else if (a.want_stats) {
-    hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, true, 0>), dim3(blocks), dim3(XW_NT), 0, s, a);
+    hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, true, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
   } else {
-    static const int wpe = [] { const char* e = getenv("FLUSI_XWAVE_WPE"); return e ? atoi(e) : 3; }();
-    if (wpe == 3) hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false, 3>), dim3(blocks), dim3(XW_NT), 0, s, a);
-    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false, 0>), dim3(blocks), dim3(XW_NT), 0, s, a);
+    if (a.wpe == 3) hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false, 3>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
+    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
   }
 }
 

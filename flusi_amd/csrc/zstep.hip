@@ -30,6 +30,16 @@ __host__ __device__ constexpr int mode_group(int mode) {
        : (mode == SM_RK2_1 || mode == SM_RK2_2) ? MG_RK2 : MG_RK4;
 }
 
+// A batch member's slab is reached through index offsets rather than shifted
+// pointers (the kernel argument pointers then stay rematerialisable, which
+// keeps the register budget of the unbatched kernel): mc in complex elements,
+// md in doubles.
+struct MOff { long mc, md; };
+template <typename T> __device__ __forceinline__ MOff member_off(const ZStepArgs& a) {
+  const long b = (long)blockIdx.z * a.bat.mstride;
+  return MOff{b / (long)sizeof(cplx<T>), b / (long)sizeof(double)};
+}
+
 // Epilogue at one spectral point (kx, ky, kz): rv[f] is field f of the
 // z-transformed right-hand side (unscaled).  Sponge curl (sponge.f90:22-83),
 // dynamic mean flow, explicit diffusion (rhs.f90:441-468), projection
@@ -37,7 +47,7 @@ __host__ __device__ constexpr int mode_group(int mode) {
 // (fluid_time_step.f90:519-832), the zero-mode overwrite and the Parseval
 // partial sum (integrals.f90:522-552).
 template <typename T, bool SPONGE, int MG>
-__device__ __forceinline__ void zepilogue(const ZStepArgs& a, int kx, int ky, int kzi, const cplx<T>* rv,
+__device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, int kx, int ky, int kzi, const cplx<T>* rv,
                                           double kxv, double kxt2, double wE,
                                           bool final_stage, double& esum, double& ecomp) {
   cplx<T>* U = reinterpret_cast<cplx<T>*>(a.u);
@@ -47,10 +57,10 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, int kx, int ky, in
   cplx<T>* RG0 = reinterpret_cast<cplx<T>*>(a.r0);
   const cplx<T>* RG1 = reinterpret_cast<const cplx<T>*>(a.r1);
   cplx<T>* RO = reinterpret_cast<cplx<T>*>(a.rhs_out);
-  const long si = ((long)kx * a.ny + ky) * a.nz + kzi;
-  const double kyv = a.ky[ky], kzv = a.kz[kzi];
+  const long si = ((long)kx * a.ny + ky) * a.nz + kzi + mo.mc;
+  const double kyv = a.ky[ky + mo.md], kzv = a.kz[kzi + mo.md];
   const double k2 = (kxv * kxv + kyv * kyv) + kzv * kzv;
-  const bool alias = ((kxt2 + a.kyt2[ky]) + a.kzt2[kzi]) >= 1.0;
+  const bool alias = ((kxt2 + a.kyt2[ky + mo.md]) + a.kzt2[kzi + mo.md]) >= 1.0;
   double nr[3], ni[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) { nr[c] = (double)rv[c].x * a.scale; ni[c] = (double)rv[c].y * a.scale; }
@@ -71,7 +81,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, int kx, int ky, in
   if (kx == 0 && ky == 0 && kzi == 0) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      if (a.meanflow_dynamic[c]) { nr[c] = -a.pen_sum[c] * a.dynamic_scale; ni[c] = 0.0; }
+      if (a.meanflow_dynamic[c]) { nr[c] = -a.pen_sum[c + mo.md] * a.dynamic_scale; ni[c] = 0.0; }
   }
   if (a.explicit_diffusion) {
     // rhs.f90:441-468
@@ -202,6 +212,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, int kx, int ky, in
 
 template <typename T, int L, int TL, int NT, bool SPONGE, int MG>
 __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
+  const MOff mo = member_off<T>(a);
   constexpr int NSTG = plan_count_k(PLAN_TILE, L);
   constexpr int R0 = plan_radix_k(PLAN_TILE, L, 0);
   constexpr int RL = plan_radix_k(PLAN_TILE, L, NSTG - 1);
@@ -216,7 +227,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
   const int tid = threadIdx.x;
   const int b0 = blockIdx.x * TL;
   const int kx = blockIdx.y;
-  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw);
+  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
   const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
   // r[f][p][q]: field f at (ky = b0 + bb, kz = j + q L/RL) of last-stage butterfly p
   cplx<T> r[NF][PERL][RL];
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
         const bool ok = b0 + bb < a.ny;
 #pragma unroll
         for (int q = 0; q < R0; ++q)
-          v[p][q] = ok ? nq[f * a.fs + ((long)kx * a.nz + j + q * (L / R0)) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
+          v[p][q] = ok ? nq[mo.mc + f * a.fs + ((long)kx * a.nz + j + q * (L / R0)) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
         Bfly<R0, -1, T>::run(v[p]);
 #pragma unroll
         for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
@@ -252,8 +263,8 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
     __syncthreads();
   }
 
-  const double kxv = a.kx[kx];
-  const double kxt2 = a.kxt2[kx];
+  const double kxv = a.kx[kx + mo.md];
+  const double kxt2 = a.kxt2[kx + mo.md];
   const double wE = (kx == 0 || kx == a.nx / 2) ? 0.5 : 1.0;
   double esum = 0.0, ecomp = 0.0;
   const bool final_stage = MG != MG_RHS && (a.mode == SM_EULER || a.mode == SM_AB2 || a.mode == SM_RK2_2 || a.mode == SM_RK4_4);
@@ -270,7 +281,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
     cplx<T> rv[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) rv[f] = r[f][p][q];
-    zepilogue<T, SPONGE, MG>(a, kx, ky, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
+    zepilogue<T, SPONGE, MG>(a, mo, kx, ky, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
   }
   if (final_stage && a.energy_partial) {
     double v = wave_sum(esum + ecomp);
@@ -279,7 +290,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w];
-      a.energy_partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = s;
+      a.energy_partial[mo.md + (long)blockIdx.y * gridDim.x + blockIdx.x] = s;
     }
   }
 }
@@ -302,6 +313,7 @@ template <typename T, int L> struct ZWShape {
 
 template <typename T, int L, bool SPONGE, int MG>
 __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a) {
+  const MOff mo = member_off<T>(a);
   using namespace wavefft;
   using Sh = ZWShape<T, L>;
   constexpr int TL = Sh::TL, NT = Sh::NT, LS = Sh::LS, K = Sh::K;
@@ -318,11 +330,11 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
   const cplx<T>* tw;
   if constexpr (Sh::TWL) {
     cplx<T>* twl = tile + TL * LS;
-    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
     for (int i = tid; i < L; i += NT) twl[i] = twg[i];
     tw = twl;
   } else {
-    tw = reinterpret_cast<const cplx<T>*>(a.tw);
+    tw = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
   }
   const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
   cplx<T> x[K];
@@ -331,7 +343,7 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
     for (int k = 0; k < K; ++k) {
       const int e = tid + k * NT, bb = e % TL, aa = e / TL;
       const bool ok = b0 + bb < a.ny;
-      x[k] = ok ? nq[f * a.fs + ((long)kx * a.nz + aa) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
+      x[k] = ok ? nq[mo.mc + f * a.fs + ((long)kx * a.nz + aa) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
     }
   };
   load(0);
@@ -352,8 +364,8 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
     __syncthreads();
   }
 
-  const double kxv = a.kx[kx];
-  const double kxt2 = a.kxt2[kx];
+  const double kxv = a.kx[kx + mo.md];
+  const double kxt2 = a.kxt2[kx + mo.md];
   const double wE = (kx == 0 || kx == a.nx / 2) ? 0.5 : 1.0;
   double esum = 0.0, ecomp = 0.0;
   const bool final_stage = MG != MG_RHS && (a.mode == SM_EULER || a.mode == SM_AB2 || a.mode == SM_RK2_2 || a.mode == SM_RK4_4);
@@ -364,7 +376,7 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
       cplx<T> rv[NF];
 #pragma unroll
       for (int f = 0; f < NF; ++f) rv[f] = r[f][q];
-      zepilogue<T, SPONGE, MG>(a, kx, ky, l + q * G, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
+      zepilogue<T, SPONGE, MG>(a, mo, kx, ky, l + q * G, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
     }
   }
   if (final_stage && a.energy_partial) {
@@ -374,7 +386,7 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w];
-      a.energy_partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = s;
+      a.energy_partial[mo.md + (long)blockIdx.y * gridDim.x + blockIdx.x] = s;
     }
   }
 }
@@ -384,19 +396,14 @@ int zstep_blocks(int nz, int ny, int nxh, bool fp64) {
   return ((ny + TL - 1) / TL) * nxh;
 }
 
-static bool zwave_enabled() {
-  static int v = [] { const char* e = getenv("FLUSI_ZWAVE"); return e ? atoi(e) : 0; }();
-  return v != 0;
-}
-
 template <typename T, int L, bool SP, int MG>
 static void run_z(const ZStepArgs& a, hipStream_t s) {
   if constexpr (L == 64 || L == 128 || L == 256 || L == 512) {
-    if (zwave_enabled()) {
+    if (a.wave) {
       using Sh = ZWShape<T, L>;
       static bool winit = false;
       if (!winit) { set_smem(zstep_wave_k<T, L, SP, MG>, Sh::smem); winit = true; }
-      dim3 grid((a.ny + Sh::TL - 1) / Sh::TL, a.nxh);
+      dim3 grid((a.ny + Sh::TL - 1) / Sh::TL, a.nxh, batch_n(a.bat));
       hipLaunchKernelGGL((zstep_wave_k<T, L, SP, MG>), grid, dim3(Sh::NT), Sh::smem, s, a);
       return;
     }
@@ -406,7 +413,7 @@ static void run_z(const ZStepArgs& a, hipStream_t s) {
   const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
   static bool init = false;
   if (!init) { set_smem(zstep_k<T, L, TL, NT, SP, MG>, smem); init = true; }
-  dim3 grid((a.ny + TL - 1) / TL, a.nxh);
+  dim3 grid((a.ny + TL - 1) / TL, a.nxh, batch_n(a.bat));
   hipLaunchKernelGGL((zstep_k<T, L, TL, NT, SP, MG>), grid, dim3(NT), smem, s, a);
 }
 template <typename T, int L, bool SP>

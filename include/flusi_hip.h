@@ -104,6 +104,44 @@ int flusi_ifft(flusi_ctx* ctx, const double* fk, double* f);
 #define FLUSI_TIMER_SLOTS 8
 int flusi_timers(flusi_ctx* ctx, int enable, double* ms, const char** names);
 
+/* Kernel-variant autotuner (no counterpart in the reference, which has one
+ * FFT library plan per transform, cof_p3dfft.f90:160-180 p3dfft_setup).
+ * Runs `reps` right-hand sides of the current state for every variant the
+ * grid allows and keeps the fastest for this context; results are cached per
+ * grid, precision and batch size for the life of the process.
+ * variant[5] (may be NULL): in-wave y/z passes, block-pass prefetch, one-wave
+ * x pass, its waves-per-SIMD register budget, in-wave z step.  ms: time of
+ * one right-hand side with the chosen variant. */
+int flusi_autotune(flusi_ctx* ctx, int reps, int* variant, double* ms);
+
+/* Batched ensembles: many independent simulations, each from its own .ini,
+ * advanced together (no counterpart in the reference, which runs one
+ * simulation per MPI job).  Members whose grids and solver parameters agree
+ * form a batch whose passes run as one launch; with mode
+ * FLUSI_ENSEMBLE_GRAPH one step of every batch is captured into a hipGraph
+ * and replayed.  Members are ordinary contexts owned by the ensemble
+ * (flusi_ensemble_member; flusi_free on them does nothing), so every
+ * per-context call above works on them between advances.  Batched modes need
+ * dt_fixed > 0; the graph mode also needs RK2 or RK4 and no time-dependent
+ * forcing (sinusoidal mean flow, startup conditioner). */
+typedef struct flusi_ensemble flusi_ensemble;
+#define FLUSI_ENSEMBLE_MEMBERS 0  /* every member steps on its own */
+#define FLUSI_ENSEMBLE_BATCHED 1  /* one launch per pass and batch */
+#define FLUSI_ENSEMBLE_GRAPH 2    /* batched step captured once, replayed */
+int flusi_ensemble_create(int n, const char* const* ini_paths, int precision, flusi_ensemble** ens);
+void flusi_ensemble_free(flusi_ensemble* ens);
+const char* flusi_ensemble_last_error(const flusi_ensemble* ens);
+int flusi_ensemble_size(const flusi_ensemble* ens, int* nmembers, int* nbatches);
+/* batch b: its member count, grid, and the index of the member that launches
+ * for it (its flusi_timers count the batched launches) */
+int flusi_ensemble_batch(flusi_ensemble* ens, int b, int* nmem, int* nx, int* ny, int* nz, int* lead);
+int flusi_ensemble_member(flusi_ensemble* ens, int i, flusi_ctx** member);
+int flusi_ensemble_init_fields(flusi_ensemble* ens);
+/* variants: 5 ints per batch, ms: one per batch (either may be NULL) */
+int flusi_ensemble_autotune(flusi_ensemble* ens, int reps, int* variants, double* ms);
+/* advances every member by nsteps time steps (time_step.f90:57-167 each) */
+int flusi_ensemble_advance(flusi_ensemble* ens, int nsteps, int mode);
+
 #ifdef __cplusplus
 }
 #endif
