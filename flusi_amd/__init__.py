@@ -64,6 +64,10 @@ EXPORTED: Dict[str, tuple] = {
     "flusi_ifft": (_I, [_P, _D, _D]),
     "flusi_timers": (_I, [_P, _I, _D, ctypes.POINTER(ctypes.c_char_p)]),
     "flusi_autotune": (_I, [_P, _I, ctypes.POINTER(_I), _D]),
+    "flusi_adjoint_ncontrols": (_I, [_P, ctypes.POINTER(_I)]),
+    "flusi_adjoint_get_controls": (_I, [_P, _D]),
+    "flusi_adjoint_forward": (_I, [_P, _D, _I, _I, _D]),
+    "flusi_adjoint_backward": (_I, [_P, _D]),
     "flusi_ensemble_create": (_I, [_I, ctypes.POINTER(ctypes.c_char_p), _I, ctypes.POINTER(_P)]),
     "flusi_ensemble_free": (None, [_P]),
     "flusi_ensemble_last_error": (ctypes.c_char_p, [_P]),
@@ -268,6 +272,31 @@ class Flusi:
         names = (ctypes.c_char_p * TIMER_SLOTS)()
         self._check(self._lib.flusi_timers(self._ctx, int(enable), _dp(ms), names))
         return {names[i].decode(): float(ms[i]) for i in range(TIMER_SLOTS)}
+
+    # -- discrete adjoint ---------------------------------------------------
+    def adjoint_controls(self) -> np.ndarray:
+        """Current values of the wing kinematics' Fourier coefficients (the
+        adjoint's controls)."""
+        n = _I()
+        self._check(self._lib.flusi_adjoint_ncontrols(self._ctx, ctypes.byref(n)))
+        th = np.zeros(n.value)
+        self._check(self._lib.flusi_adjoint_get_controls(self._ctx, _dp(th)))
+        return th
+
+    def adjoint_forward(self, theta, nsteps: int, every: int = 20) -> float:
+        """Mean lift over nsteps RK4 steps from the initial condition with the
+        given controls; keeps checkpoints for adjoint_backward."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        J = ctypes.c_double()
+        self._check(self._lib.flusi_adjoint_forward(self._ctx, _dp(th), int(nsteps), int(every), ctypes.byref(J)))
+        self._nctl = th.size
+        return J.value
+
+    def adjoint_backward(self) -> np.ndarray:
+        """dJ/dtheta of the last adjoint_forward."""
+        g = np.zeros(getattr(self, "_nctl", 0) or self.adjoint_controls().size)
+        self._check(self._lib.flusi_adjoint_backward(self._ctx, _dp(g)))
+        return g
 
     def autotune(self, reps: int = 3):
         """Picks the fastest kernel variant for this grid (cached per grid,

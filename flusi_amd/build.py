@@ -18,7 +18,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libflusi_hip.so")
-SOURCES = ["passes.hip", "tiles_wave.hip", "zstep.hip", "xrhs_wave.hip", "fields.hip", "solver.hip", "host_params.cpp"]
+SOURCES = ["passes.hip", "tiles_wave.hip", "zstep.hip", "xrhs_wave.hip", "fields.hip", "adjoint.hip", "solver.hip",
+           "host_params.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FLUSI_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

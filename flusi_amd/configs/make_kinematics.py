@@ -1,7 +1,9 @@
-"""Writes kin_cfg3.ini: seeded 5-term Fourier wing kinematics of the tethered
-flapping-insect workload (stroke phi: a1 = 70 deg, pitch alpha: b1 = 45 deg,
-deviation theta small), in FLUSI's from_file kinematics format
+"""Writes kin_cfg3.ini and kin_cfg4.ini: seeded Fourier wing kinematics of
+the tethered flapping-insect workload (stroke phi: a1 = 70 deg, pitch alpha:
+b1 = 45 deg, deviation theta small), in FLUSI's from_file kinematics format
 (wings_motion.f90:98-171: [kinematics] type=fourier, units=degree).
+kin_cfg3.ini: 5 terms from default_rng(3); kin_cfg4.ini: 2 terms from
+default_rng(4), whose 12 coefficients are the adjoint workload's controls.
 
     python flusi_amd/configs/make_kinematics.py
 """
@@ -25,8 +27,8 @@ def make(seed: int = 3, nfft: int = 5):
                 ai_theta=ai_theta, bi_theta=bi_theta, nfft=nfft)
 
 
-def write(path: str, seed: int = 3):
-    k = make(seed)
+def write(path: str, seed: int = 3, nfft: int = 5):
+    k = make(seed, nfft)
     fmt = lambda a: " ".join("%.15e" % v for v in a)
     with open(path, "w") as fh:
         fh.write("; seeded wing kinematics (numpy default_rng(%d)), FLUSI from_file format\n" % seed)
@@ -39,4 +41,6 @@ def write(path: str, seed: int = 3):
 
 
 if __name__ == "__main__":
-    write(os.path.join(os.path.dirname(os.path.abspath(__file__)), "kin_cfg3.ini"))
+    here = os.path.dirname(os.path.abspath(__file__))
+    write(os.path.join(here, "kin_cfg3.ini"))
+    write(os.path.join(here, "kin_cfg4.ini"), seed=4, nfft=2)

@@ -37,6 +37,8 @@ struct InsectArgs {
   int ncaps;
   const double* caps;
   int wing_shape;           // 1 = TwoEllipses, 2 = rectangular, 3 = Fourier outline
+  int smooth_mode;          // differentiable variant (adjoint runs): continuous wing-root taper
+                            // and us blended by part weights where parts overlap
   const double* fw_table;   // Fourier outline: radius table on nodes j*fw_dphi
   double fw_xc, fw_yc, fw_dphi, fw_bbox[6];
   double L_span, b_top, b_bot, thickness;

@@ -88,6 +88,28 @@ void launch_cavity(const GridGeom& g, const CavityArgs& a, void* mask, void* us,
 // draw_body_jerry (body_geometry.f90:511-595), drawsphere (:1657-1724),
 // draw_wing_twoellipses / _rectangular (wings_geometry.f90:527-712) and the
 // stage II body-motion transform (wings_geometry.f90:71-108).
+// One wing's value mt at a point (wing frame xw): the mask keeps the largest
+// part, whose colour and velocity win (module_insects.f90 draw_insect_wings);
+// the differentiable variant also accumulates every wing's weighted velocity.
+__device__ __forceinline__ void wing_point(const InsectArgs& a, int w, const double* xw, double mt, double& m,
+                                           uint8_t& c, bool& changed, double* u_s, double& wsum, double* usw) {
+  if (!(mt > 0.0)) return;
+  const double* r = a.rot[w];
+  const double v[3] = {r[1] * xw[2] - r[2] * xw[1], r[2] * xw[0] - r[0] * xw[2], r[0] * xw[1] - r[1] * xw[0]};
+  double vw[3];
+  matTvec(a.Mw[w], v, vw);
+  if (a.smooth_mode) {
+    wsum += mt;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) usw[i] += mt * vw[i];
+  }
+  if (m < mt) {
+    m = mt; c = (uint8_t)a.color_w[w]; changed = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u_s[i] = vw[i];
+  }
+}
+
 template <typename T>
 __global__ void insect_k(GridGeom g, const InsectArgs* __restrict__ ap, T* mask, T* us, uint8_t* color, int ez,
                          long mstride) {
@@ -255,6 +277,9 @@ __global__ void insect_k(GridGeom g, const InsectArgs* __restrict__ ap, T* mask,
     }
   }
 
+  // differentiable variant: the body's value and the wings' weighted velocities
+  const double mbody = m;
+  double wsum = 0.0, usw[3] = {0.0, 0.0, 0.0};
   for (int w = 0; w < 2; ++w) {
     if (!a.draw_wing[w] || !in_box(1 + w)) continue;
     const double d[3] = {xb[0] - a.piv[w][0], xb[1] - a.piv[w][1], xb[2] - a.piv[w][2]};
@@ -273,12 +298,7 @@ __global__ void insect_k(GridGeom g, const InsectArgs* __restrict__ ap, T* mask,
       const double R = sqrt((xw[0] - a.fw_xc) * (xw[0] - a.fw_xc) + (xw[1] - a.fw_yc) * (xw[1] - a.fw_yc));
       const double mt = steps_d(fabs(xw[2] - 0.0), 0.5 * a.thickness, a.smooth) * steps_d(R, R0, a.smooth);
       if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
-      if (m < mt && mt > 0.0) {
-        m = mt; c = (uint8_t)a.color_w[w]; changed = true;
-        const double* r = a.rot[w];
-        const double v[3] = {r[1] * xw[2] - r[2] * xw[1], r[2] * xw[0] - r[0] * xw[2], r[0] * xw[1] - r[1] * xw[0]};
-        matTvec(a.Mw[w], v, u_s);
-      }
+      wing_point(a, w, xw, mt, m, c, changed, u_s, wsum, usw);
       continue;
     }
     if (!(xw[1] >= -a.safety && xw[1] <= a.L_span + a.safety)) continue;
@@ -293,20 +313,25 @@ __global__ void insect_k(GridGeom g, const InsectArgs* __restrict__ ap, T* mask,
       x_top = a.b_top; x_bot = -a.b_bot;
     }
     if (!(xw[0] > x_bot - a.safety && xw[0] < x_top + a.safety)) continue;
-    const double yt = (xw[1] < 0.0) ? steps_d(-xw[1], 0.0, a.smooth) : steps_d(xw[1], a.L_span, a.smooth);
+    // the reference's taper jumps from 1/2 to 1 at the root plane y = 0; the
+    // differentiable variant takes the smaller of the two end tapers
+    const double yt = a.smooth_mode ? fmin(steps_d(-xw[1], 0.0, a.smooth), steps_d(xw[1], a.L_span, a.smooth))
+                    : (xw[1] < 0.0) ? steps_d(-xw[1], 0.0, a.smooth) : steps_d(xw[1], a.L_span, a.smooth);
     const double zt = steps_d(fabs(xw[2]), 0.5 * a.thickness, a.smooth);
     const double xt = (xw[0] < 0.0) ? steps_d(-xw[0], -x_bot, a.smooth) : steps_d(xw[0], x_top, a.smooth);
     const double mt = zt * yt * xt;
     if (!mask_loaded) { m = (double)mask[p]; mask_loaded = true; }
-    if (m < mt && mt > 0.0) {
-      m = mt; c = (uint8_t)a.color_w[w]; changed = true;
-      const double* r = a.rot[w];
-      const double v[3] = {r[1] * xw[2] - r[2] * xw[1], r[2] * xw[0] - r[0] * xw[2], r[0] * xw[1] - r[1] * xw[0]};
-      matTvec(a.Mw[w], v, u_s);
-    }
+    wing_point(a, w, xw, mt, m, c, changed, u_s, wsum, usw);
   }
   const bool new_wing = (c == 2 || c == 3);
-  if (new_wing && m > 0.0) {
+  const bool add_body_velocity = a.smooth_mode ? wsum > 0.0 : (new_wing && m > 0.0);
+  if (add_body_velocity) {
+    if (a.smooth_mode) {
+      // us = sum_w m_w us_w / (m_body + sum_w m_w): continuous where parts overlap
+      const double wn = 1.0 / (mbody + wsum);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) u_s[i] = usw[i] * wn;
+    }
     const double v[3] = {a.vb[0] + a.rot_body_b[1] * xb[2] - a.rot_body_b[2] * xb[1],
                          a.vb[1] + a.rot_body_b[2] * xb[0] - a.rot_body_b[0] * xb[2],
                          a.vb[2] + a.rot_body_b[0] * xb[1] - a.rot_body_b[1] * xb[0]};

@@ -296,6 +296,35 @@ std::array<double, 6> WingKinematics::eval(double t, const std::string& base_dir
   throw FlusiError(10624, "flapping motion protocol not supported: " + protocol);
 }
 
+int WingKinematics::ncontrols(const std::string& base_dir) {
+  if (protocol != "from_file") return 0;
+  if (!initialized) (void)eval(0.0, base_dir);
+  const size_t n = ai[0].size();
+  for (int a = 0; a < 3; ++a)
+    if (ai[a].size() != n || bi[a].size() != n)
+      throw FlusiError(900020, "kinematics controls need one nfft for phi, alpha and theta");
+  return (int)(6 * n);
+}
+double& WingKinematics::control(int p, const std::string& base_dir) {
+  const int nc = ncontrols(base_dir);
+  if (p < 0 || p >= nc) throw FlusiError(900021, "kinematics control index out of range");
+  const int n = nc / 6, a = p / (2 * n), r = p % (2 * n);
+  return r < n ? ai[a][r] : bi[a][r - n];
+}
+int Insect::ncontrols() {
+  const int r = kine_r.ncontrols(base_dir), l = kine_l.ncontrols(base_dir);
+  if (r && l && r != l) throw FlusiError(900020, "kinematics controls: the wings' series differ in length");
+  return std::max(r, l);
+}
+double Insect::get_control(int p) {
+  if (kine_r.ncontrols(base_dir)) return kine_r.control(p, base_dir);
+  return kine_l.control(p, base_dir);
+}
+void Insect::set_control(int p, double v) {
+  if (kine_r.ncontrols(base_dir)) kine_r.control(p, base_dir) = v;
+  if (kine_l.ncontrols(base_dir)) kine_l.control(p, base_dir) = v;
+}
+
 // insect_init_clean.f90:1-250 (tethered insect)
 void Insect::init(const IniFile& ini_c, const Params& p, const std::string& bdir) {
   IniFile& ini = const_cast<IniFile&>(ini_c);

@@ -84,6 +84,11 @@ struct WingKinematics {
   std::vector<double> ai[3], bi[3];
   // returns phi, alpha, theta, phi_dt, alpha_dt, theta_dt (radians)
   std::array<double, 6> eval(double t, const std::string& base_dir);
+  // Fourier coefficients of a from_file kinematics as controls: for angle a
+  // (phi, alpha, theta) and n = nfft, control a*2n + i is ai_a(i) and
+  // a*2n + n + i is bi_a(i), in the file's units; all angles need one nfft
+  int ncontrols(const std::string& base_dir);
+  double& control(int p, const std::string& base_dir);
 };
 
 struct InsectMotion {
@@ -127,6 +132,11 @@ struct Insect {
   double time = 0.0;
   void init(const IniFile& ini, const Params& p, const std::string& base_dir);
   void update(double t);                       // Update_Insect (module_insects.f90:281-333)
+  // the wings' from_file Fourier coefficients (shared by both wings when both
+  // read the same series): count, value, and setting it on every such wing
+  int ncontrols();
+  double get_control(int p);
+  void set_control(int p, double v);
   std::vector<double> kinematics_row(double t) const;  // write_kinematics_file columns
  private:
   InsectMotion motion(double t);

@@ -1,6 +1,7 @@
 // Solver context: device state, the RHS pass pipeline, the AB2/RK2/RK4
 // steppers, create_mask, write_integrals and the C ABI of include/flusi_hip.h.
 #include "../../include/flusi_hip.h"
+#include "adjoint.h"
 #include "fft_core.h"
 #include "fields.h"
 #include "flusi_kernels.h"
@@ -99,6 +100,10 @@ class SolverBase {
   virtual void ifft1(const double* fk, double* f) = 0;
   virtual void timers(int enable, double* ms) = 0;
   virtual void autotune_v(int reps, int* variant, double* ms) = 0;
+  virtual int adjoint_ncontrols() = 0;
+  virtual void adjoint_get_controls(double* theta) = 0;
+  virtual double adjoint_forward(const double* theta, int nsteps, int every) = 0;
+  virtual void adjoint_backward(double* grad) = 0;
 
   // time_step.f90:57-167 loop body
   void step() {
@@ -708,6 +713,277 @@ class Solver : public SolverBase {
     if (u) get_u(u, false);
   }
 
+  // ------------------------------------------------------------ adjoint
+  // Gradient of the mean lift J = (1/K) sum_{n=1..K} F_z(t_n) over K RK4 steps
+  // (F_z = dV/eta sum chi (u_z - us_z), the force of integrals.f90:87-100 /
+  // drag.f90:17-454 along z) with respect to the wing kinematics' Fourier
+  // coefficients.  The forward run keeps the state every `every` steps; the
+  // backward sweep recomputes one segment at a time, then each step's stage
+  // states, and applies the transposed stages:
+  //   u_{n+1} = u_n + dt/6 (k1 + 2 k2 + 2 k3 + k4),  k_s = N(u_s, t_s),
+  //   kb4 = dt/6 ub;              W4 = N_u(u_4)^T kb4;  un = ub + W4
+  //   kb3 = dt/3 ub + dt W4;      W3 = N_u(u_3)^T kb3;  un += W3
+  //   kb2 = dt/3 ub + dt/2 W3;    W2 = N_u(u_2)^T kb2;  un += W2
+  //   kb1 = dt/6 ub + dt/2 W2;    W1 = N_u(u_n)^T kb1;  un += W1
+  // with theta-gradient terms <phi_s, dg/dtheta> at every stage and the
+  // objective's own terms at every t_n.  dchi/dtheta and dus/dtheta come from
+  // central differences of the mask drawing (FP64 scratch fields).
+  struct AdjointState {
+    int K = 0, every = 0;
+    std::vector<double> theta, tn;  // controls and the step times t_0..t_K
+    void* ck = nullptr;             // states at steps 0, every, 2 every, ...
+    void* seg = nullptr;            // states of one segment
+    void* st[3] = {nullptr, nullptr, nullptr};  // stage states u_2, u_3, u_4
+    void *ub = nullptr, *un = nullptr, *kb = nullptr, *wv = nullptr, *mw = nullptr, *A = nullptr, *B = nullptr;
+    void* ph = nullptr;             // physical: phi (3), a_w (3)
+    double *mP = nullptr, *usP = nullptr, *mM = nullptr, *usM = nullptr;
+    uint8_t* colS = nullptr;
+    double* part = nullptr;
+    double J = 0.0;
+    int nck = 0;
+    bool have_forward = false;
+  } adj;
+  double mask_h = 1.0e-6;  // control step of the mask differences (the controls' units)
+  // Adjoint runs redraw the whole insect at every mask update, so that the
+  // mask is a function of time and controls only.  (The reference keeps the
+  // body from its first drawing and clears only the previous wings' points;
+  // where wing and body smoothing layers overlap that history makes the mask
+  // a non-differentiable function of the kinematics.)
+  bool mask_fresh = false;
+
+  SpecGeom sgeom() const {
+    return SpecGeom{nx, ny, nz, nxh, nc, kx_d, ky_d, kz_d, kxt2_d, kyt2_d, kzt2_d, p.iDealias == 1 ? 1 : 0};
+  }
+  size_t state_bytes() const { return 3 * sizeof(C) * nc; }
+  void* state_at(void* base, int i) { return (char*)base + (size_t)i * state_bytes(); }
+  void copy_state(void* dst, const void* src) {
+    HIPCHECK(hipMemcpyAsync(dst, src, state_bytes(), hipMemcpyDeviceToDevice, stream));
+  }
+  double reduce_partials(double* part, int nb) {
+    launch_reduce_rows(part, nb, 1, kinds + 4, rmid, red + 100, stream);  // red[16..77]: diagnostics
+    double v = 0.0;
+    HIPCHECK(hipMemcpyAsync(&v, red + 100, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    return v;
+  }
+  void adjoint_check() {
+    need_fft();
+    if (!(p.iMask == "Insect" || p.iMask == "insect") || p.iMoving != 1 || p.iPenalization != 1)
+      throw FlusiError(900023, "adjoint: a moving insect mask is required");
+    if (p.method != "RK4") throw FlusiError(900023, "adjoint: RK4 only");
+    if (p.dt_fixed <= 0.0) throw FlusiError(900023, "adjoint: dt_fixed > 0 required");
+    if (sponge_code()) throw FlusiError(900023, "adjoint: no vorticity sponge");
+    for (int c = 0; c < 3; ++c)
+      if (p.iMeanFlow[c] != "free") throw FlusiError(900023, "adjoint: iMeanFlow must be free");
+    if (p.startup_conditioner == "yes") throw FlusiError(900023, "adjoint: no startup conditioner");
+  }
+  int adjoint_ncontrols() override {
+    if (!fields_initialized) init_fields();
+    adjoint_check();
+    return insect.ncontrols();
+  }
+  void adjoint_get_controls(double* th) override {
+    const int n = adjoint_ncontrols();
+    for (int i = 0; i < n; ++i) th[i] = insect.get_control(i);
+  }
+  void adjoint_alloc(int K, int every) {
+    AdjointState& a = adj;
+    const int nck = K / every + 1;
+    if (!a.ub) {
+      for (void** q : {&a.ub, &a.un, &a.kb, &a.wv, &a.mw, &a.A, &a.B}) *q = dalloc(state_bytes());
+      for (int s = 0; s < 3; ++s) a.st[s] = dalloc(state_bytes());
+      a.ph = dalloc(6 * sizeof(T) * npts);
+      a.mP = (double*)dalloc(sizeof(double) * npts);
+      a.mM = (double*)dalloc(sizeof(double) * npts);
+      a.usP = (double*)dalloc(3 * sizeof(double) * npts);
+      a.usM = (double*)dalloc(3 * sizeof(double) * npts);
+      a.colS = (uint8_t*)dalloc(npts);
+      a.part = (double*)dalloc(sizeof(double) * adjoint_reduce_blocks(npts));
+    }
+    if (a.nck < nck || a.every != every) {
+      dfree(a.ck); dfree(a.seg);
+      a.ck = dalloc((size_t)nck * state_bytes());
+      a.seg = dalloc((size_t)(every + 1) * state_bytes());
+      a.nck = nck;
+    }
+    a.K = K; a.every = every;
+  }
+  // z-force of the current state (its physical velocity in pf(0..2)) with the current mask
+  double lift_now() {
+    launch_lift<T>(npts, pf(0), us, mask, adj.part, stream);
+    return reduce_partials(adj.part, adjoint_reduce_blocks(npts)) * p.dx() * p.dy() * p.dz() / p.eps;
+  }
+  void step_plain() {  // time_step.f90:57-167 without the output
+    dt0 = dt1;
+    fluid_timestep();
+    time = time + dt1;
+    it += 1;
+  }
+  // J for the controls th over K steps from the initial condition; keeps the
+  // checkpoints for adjoint_backward
+  double adjoint_forward(const double* th, int K, int every) override {
+    if (K < 1 || every < 1) throw FlusiError(900024, "adjoint: nsteps and the checkpoint interval must be >= 1");
+    mask_fresh = true;
+    init_fields();
+    adjoint_check();
+    const int ncon = insect.ncontrols();
+    if (ncon == 0) throw FlusiError(900023, "adjoint: the wing kinematics must be from_file Fourier series");
+    adj.theta.assign(th, th + ncon);
+    for (int i = 0; i < ncon; ++i) insect.set_control(i, th[i]);
+    create_mask(time);  // the initial mask with these controls
+    adjoint_alloc(K, every);
+    ensure_phys();
+    adj.tn.assign(K + 1, 0.0);
+    adj.tn[0] = time;
+    copy_state(state_at(adj.ck, 0), uk);
+    double J = 0.0;
+    for (int n = 1; n <= K; ++n) {
+      step_plain();
+      adj.tn[n] = time;
+      ifft_fields(uk, 3, pf(0));
+      J += lift_now() / K;  // the last stage left the mask at t_n
+      if (n % every == 0) copy_state(state_at(adj.ck, n / every), uk);
+    }
+    adj.J = J;
+    adj.have_forward = true;
+    return J;
+  }
+  // central-difference drawings of the insect at t with control i moved by +-h
+  void perturbed_masks(double t, int i) {
+    const double v = insect.get_control(i);
+    for (int sgn : {+1, -1}) {
+      insect.set_control(i, v + sgn * mask_h);
+      insect.update(t);
+      InsectArgs a = insect_args(true, true);
+      if (!ins_h) {  // the eager argument ring
+        HIPCHECK(hipStreamSynchronize(stream));
+        ins_per_slot = std::max(ins_per_slot, 1);
+        if (!ins_d) ins_d = (InsectArgs*)dalloc(sizeof(InsectArgs) * INS_SLOTS * ins_per_slot);
+        HIPCHECK(hipHostMalloc((void**)&ins_h, sizeof(InsectArgs) * INS_SLOTS * ins_per_slot, hipHostMallocDefault));
+      }
+      const int slot = ins_next;
+      ins_next = (ins_next + 1) % INS_SLOTS;
+      if (ins_ev[slot]) HIPCHECK(hipEventSynchronize(ins_ev[slot]));
+      else HIPCHECK(hipEventCreateWithFlags(&ins_ev[slot], hipEventDisableTiming));
+      InsectArgs* h = ins_h + (size_t)slot * ins_per_slot;
+      *h = a;
+      InsectArgs* d = ins_d + (size_t)slot * ins_per_slot;
+      HIPCHECK(hipMemcpyAsync(d, h, sizeof(InsectArgs), hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipEventRecord(ins_ev[slot], stream));
+      launch_insect<double>(geom, a, d, sgn > 0 ? adj.mP : adj.mM, sgn > 0 ? adj.usP : adj.usM, adj.colS, stream,
+                            Batch{1, 0});
+    }
+    insect.set_control(i, v);
+    kcheck("perturbed masks");
+  }
+  // grad_i += scale * sum f . (dchi_i (u - us) - chi dus_i) / (2h) at time t, with
+  // f = phi or e_z (phi null), u in pf(0..2), mask and us at t
+  void theta_terms(double t, const void* phi, double scale, double* grad) {
+    const int ncon = (int)adj.theta.size();
+    for (int i = 0; i < ncon; ++i) {
+      perturbed_masks(t, i);
+      launch_theta<T>(npts, phi, pf(0), us, mask, adj.mP, adj.usP, adj.mM, adj.usM, adj.part, stream);
+      grad[i] += scale * reduce_partials(adj.part, adjoint_reduce_blocks(npts)) / (2.0 * mask_h);
+    }
+    insect.update(t);
+  }
+  // W = N_u(u_s)^T lam at time t_s; grad += <phi, dg/dtheta>
+  void adjoint_rhs(const void* u_s, double t_s, const void* lam, void* W, double* grad) {
+    const SpecGeom g = sgeom();
+    const double N = double(nx) * double(ny) * double(nz);
+    create_mask(t_s);
+    inv_uw_phys(u_s);  // pf(0..2) u, pf(3..5) w
+    launch_adj_pre<T>(g, p.nu, 1.0 / N, lam, W, adj.mw, stream);
+    void* phi = adj.ph;
+    void* aw = (char*)adj.ph + 3 * sizeof(T) * npts;
+    ifft_fields(adj.mw, 3, phi);
+    // g = u x w - chi/eta (u - us): dg/dtheta = -(1/eta)(dchi (u - us) - chi dus)
+    theta_terms(t_s, phi, -1.0 / p.eps, grad);
+    launch_adj_point<T>(npts, 1.0 / p.eps, phi, pf(0), pf(3), mask, aw, stream);
+    fft_fields(phi, 3, adj.A);
+    fft_fields(aw, 3, adj.B);
+    launch_adj_post<T>(g, N, adj.A, adj.B, W, stream);
+    kcheck("adjoint rhs");
+  }
+  // ub += dJ/du at step n (its state u_n), grad += dJ/dtheta there
+  void objective_terms(int n, const void* u_n, double* grad) {
+    const double dV = p.dx() * p.dy() * p.dz();
+    const double N = double(nx) * double(ny) * double(nz);
+    const double s = dV / (p.eps * adj.K);
+    create_mask(adj.tn[n]);
+    ifft_fields(u_n, 3, pf(0));
+    theta_terms(adj.tn[n], nullptr, s, grad);
+    void* src = adj.ph;
+    launch_ez_mask<T>(npts, s, mask, src, stream);
+    fft_fields(src, 3, adj.A);
+    launch_adj_post<T>(sgeom(), N, adj.A, nullptr, adj.ub, stream);
+    kcheck("objective terms");
+  }
+  // stage states u_2, u_3, u_4 of the step from u_n at t_n
+  void stage_states(const void* u_n, double t_n) {
+    copy_state(uk, u_n);
+    const double dt = p.dt_fixed;
+    const int modes[3] = {SM_RK4_1, SM_RK4_2, SM_RK4_3};
+    const double ts[3] = {t_n, t_n + 0.5 * dt, t_n + 0.5 * dt};
+    for (int s = 0; s < 3; ++s) {
+      pre_rhs(s == 0 ? uk : reg[0], ts[s]);
+      ZStepArgs z = zargs(modes[s]);
+      z.u = uk; z.udiff = (s == 0) ? uk : reg[0]; z.ustage = reg[0]; z.acc = reg[1]; z.dt = dt;
+      post_rhs(z);
+      copy_state(adj.st[s], reg[0]);
+    }
+  }
+  void adjoint_backward(double* grad_out) override {
+    if (!adj.have_forward) throw FlusiError(900025, "adjoint: run adjoint_forward first");
+    AdjointState& a = adj;
+    const int K = a.K, E = a.every, ncon = (int)a.theta.size();
+    const double dt = p.dt_fixed;
+    const size_t ns = 3 * nc * 2;  // reals of a state
+    std::vector<double> grad(ncon, 0.0);
+    launch_zero(a.ub, state_bytes(), stream);
+    const int nseg = (K + E - 1) / E;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+      const int n0 = sg * E, n1 = std::min(K, n0 + E);
+      // the segment's states u_{n0} .. u_{n1}
+      copy_state(uk, state_at(a.ck, sg));
+      time = a.tn[n0]; it = n0; dt1 = dt;
+      copy_state(state_at(a.seg, 0), uk);
+      for (int n = n0; n < n1; ++n) {
+        step_plain();
+        copy_state(state_at(a.seg, n + 1 - n0), uk);
+      }
+      for (int n = n1 - 1; n >= n0; --n) {
+        objective_terms(n + 1, state_at(a.seg, n + 1 - n0), grad.data());
+        const void* u_n = state_at(a.seg, n - n0);
+        stage_states(u_n, a.tn[n]);
+        const double t = a.tn[n];
+        // stage 4 (u_4 at t + dt)
+        launch_axpby<T>(ns, dt / 6.0, a.ub, 0.0, nullptr, a.kb, stream);
+        adjoint_rhs(a.st[2], t + dt, a.kb, a.wv, grad.data());
+        launch_axpby<T>(ns, 1.0, a.ub, 1.0, a.wv, a.un, stream);
+        // stage 3 (u_3 at t + dt/2)
+        launch_axpby<T>(ns, dt / 3.0, a.ub, dt, a.wv, a.kb, stream);
+        adjoint_rhs(a.st[1], t + 0.5 * dt, a.kb, a.wv, grad.data());
+        launch_axpby<T>(ns, 1.0, a.un, 1.0, a.wv, a.un, stream);
+        // stage 2 (u_2 at t + dt/2)
+        launch_axpby<T>(ns, dt / 3.0, a.ub, 0.5 * dt, a.wv, a.kb, stream);
+        adjoint_rhs(a.st[0], t + 0.5 * dt, a.kb, a.wv, grad.data());
+        launch_axpby<T>(ns, 1.0, a.un, 1.0, a.wv, a.un, stream);
+        // stage 1 (u_n at t)
+        launch_axpby<T>(ns, dt / 6.0, a.ub, 0.5 * dt, a.wv, a.kb, stream);
+        adjoint_rhs(u_n, t, a.kb, a.wv, grad.data());
+        launch_axpby<T>(ns, 1.0, a.un, 1.0, a.wv, a.ub, stream);
+      }
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+    // the solver is left at the initial state
+    copy_state(uk, state_at(a.ck, 0));
+    time = a.tn[0]; it = 0;
+    create_mask(time);
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (int i = 0; i < ncon; ++i) grad_out[i] = grad[i];
+  }
+
   // ------------------------------------------------------------ masks
   void create_mask(double t) override {
     if (p.iMask == "Insect" && (p.iMoving != 1 || p.iPenalization != 1))
@@ -783,8 +1059,10 @@ class Solver : public SolverBase {
   }
 
   // full: visit the whole grid (a batched or captured launch has one fixed
-  // region for every member and stage)
-  InsectArgs insect_args(bool full = false) {
+  // region for every member and stage).  scratch: a complete drawing (body
+  // and wings, everything reset) into other buffers, leaving the record of
+  // what the solver's own mask holds untouched.
+  InsectArgs insect_args(bool full = false, bool scratch = false) {
     Insect& I = insect;
     InsectArgs a{};
     const double h = std::max(p.dx(), std::max(p.dy(), p.dz()));
@@ -792,9 +1070,13 @@ class Solver : public SolverBase {
     I.safety = 3.5 * I.smooth;
     // delete_old_mask: a resting body keeps its mask after the first call
     a.full_reset = I.cleaned_already_once ? 0 : 1;
-    I.cleaned_already_once = true;
+    if (!scratch) I.cleaned_already_once = true;
     a.draw_body = 0;
-    if (!I.body_already_drawn) {
+    if (scratch || (mask_fresh && I.body_already_drawn)) {
+      if (!I.body_already_drawn) throw FlusiError(900022, "scratch insect drawing before the first mask");
+      a.full_reset = 1;
+      a.draw_body = (I.BodyType != "nobody") ? 1 : 0;
+    } else if (!I.body_already_drawn) {
       a.draw_body = (I.BodyType != "nobody") ? 1 : 0;
       I.body_already_drawn = true;
       if (I.BodyType == "jerry" || I.BodyType == "Jerry") {
@@ -838,7 +1120,7 @@ class Solver : public SolverBase {
       }
       a.nspheres = 3;
     }
-    if (a.draw_body && a.body_type == 3 &&
+    if (a.draw_body && a.body_type == 3 && !((scratch || mask_fresh) && caps_d) &&
         (I.HasDetails == "all" || I.HasDetails == "legs" || I.HasDetails == "antennae_proboscis")) {
       // legs, antennae, proboscis (body_geometry.f90:276-345): capsules in the
       // global frame, end spheres periodized with drawsphere's index boxes
@@ -873,6 +1155,9 @@ class Solver : public SolverBase {
       HIPCHECK(hipStreamSynchronize(stream));
       a.ncaps = (int)cl.size();
       a.caps = caps_d;
+    } else if (a.draw_body && a.body_type == 3 && caps_d) {  // a tethered body's capsules do not move
+      a.ncaps = (int)bumblebee_capsules().size();
+      a.caps = caps_d;
     }
     if (I.WingShape == "bumblebee") {
       if (!fw_d) {
@@ -884,6 +1169,7 @@ class Solver : public SolverBase {
       for (int i = 0; i < 6; ++i) a.fw_bbox[i] = I.fwing.bbox[i];
     }
     a.wing_shape = (I.WingShape == "TwoEllipses") ? 1 : (I.WingShape == "bumblebee" ? 3 : 2);
+    a.smooth_mode = mask_fresh ? 1 : 0;
     a.L_span = I.L_span; a.b_top = I.b_top; a.b_bot = I.b_bot; a.thickness = I.WingThickness;
     a.draw_wing[0] = I.RightWing == "yes"; a.draw_wing[1] = I.LeftWing == "yes";
     for (int i = 0; i < 9; ++i) { a.Mw[0][i] = s.Mwr[i]; a.Mw[1][i] = s.Mwl[i]; }
@@ -933,7 +1219,7 @@ class Solver : public SolverBase {
     // (the only places where wing colours can be left); everything on the
     // first call and for periodic insects
     const int nn[3] = {nx, ny, nz};
-    if (!a.use_box || a.full_reset || full) {
+    if (!a.use_box || a.full_reset || full || scratch) {
       for (int i = 0; i < 3; ++i) { a.reg_lo[i] = 0; a.reg_hi[i] = nn[i] - 1; }
     } else {
       for (int i = 0; i < 3; ++i) { a.reg_lo[i] = nn[i]; a.reg_hi[i] = -1; }
@@ -950,10 +1236,11 @@ class Solver : public SolverBase {
         if (prev_wing_valid[w]) include(prev_wing_lo[w], prev_wing_hi[w]);
       }
     }
-    for (int w = 0; w < 2; ++w) {
-      prev_wing_valid[w] = a.draw_wing[w] != 0;
-      for (int i = 0; i < 3; ++i) { prev_wing_lo[w][i] = a.box_lo[1 + w][i]; prev_wing_hi[w][i] = a.box_hi[1 + w][i]; }
-    }
+    if (!scratch)
+      for (int w = 0; w < 2; ++w) {
+        prev_wing_valid[w] = a.draw_wing[w] != 0;
+        for (int i = 0; i < 3; ++i) { prev_wing_lo[w][i] = a.box_lo[1 + w][i]; prev_wing_hi[w][i] = a.box_hi[1 + w][i]; }
+      }
     return a;
   }
   // index boxes of the wings drawn by the previous insect mask update
@@ -1743,6 +2030,21 @@ int flusi_fft(flusi_ctx* ctx, const double* f, double* fk) { return guarded(ctx,
 int flusi_ifft(flusi_ctx* ctx, const double* fk, double* f) { return guarded(ctx, [&] { ctx->s->ifft1(fk, f); }); }
 int flusi_autotune(flusi_ctx* ctx, int reps, int* variant, double* ms) {
   return guarded(ctx, [&] { ctx->s->autotune_v(reps, variant, ms); });
+}
+int flusi_adjoint_ncontrols(flusi_ctx* ctx, int* n) {
+  return guarded(ctx, [&] { *n = ctx->s->adjoint_ncontrols(); });
+}
+int flusi_adjoint_get_controls(flusi_ctx* ctx, double* theta) {
+  return guarded(ctx, [&] { ctx->s->adjoint_get_controls(theta); });
+}
+int flusi_adjoint_forward(flusi_ctx* ctx, const double* theta, int nsteps, int every, double* J) {
+  return guarded(ctx, [&] {
+    const double j = ctx->s->adjoint_forward(theta, nsteps, every);
+    if (J) *J = j;
+  });
+}
+int flusi_adjoint_backward(flusi_ctx* ctx, double* grad) {
+  return guarded(ctx, [&] { ctx->s->adjoint_backward(grad); });
 }
 int flusi_timers(flusi_ctx* ctx, int enable, double* ms, const char** names) {
   return guarded(ctx, [&] {

@@ -114,6 +114,27 @@ int flusi_timers(flusi_ctx* ctx, int enable, double* ms, const char** names);
  * one right-hand side with the chosen variant. */
 int flusi_autotune(flusi_ctx* ctx, int reps, int* variant, double* ms);
 
+/* Discrete adjoint (no counterpart in the reference).  The objective is the
+ * mean lift J = (1/K) sum_{n=1..K} F_z(t_n) over K RK4 steps from the initial
+ * condition, F_z the z-force dV/eta sum chi (u_z - us_z) of the insect
+ * (drag.f90:17-454 with the mask regenerated at t_n, integrals.f90:87-100).
+ * The controls are the Fourier coefficients of the wings' from_file
+ * kinematics (wings_motion.f90:98-222), shared by both wings when they read
+ * one series: for angle a in (phi, alpha, theta) and n = nfft, control
+ * a*2n + i is ai(i) and a*2n + n + i is bi(i), in the file's units.
+ * Requires a moving insect mask, RK4, dt_fixed > 0, free mean flow and no
+ * sponge.  flusi_adjoint_forward re-initialises the fields, runs K steps with
+ * the given controls and keeps a checkpoint every `every` steps;
+ * flusi_adjoint_backward then returns dJ/dtheta (the solver is left at the
+ * initial state).  From the first adjoint run on, the context redraws the
+ * whole insect at every mask update (the reference keeps the body from its
+ * first drawing and clears only the previous wings, a history that makes the
+ * mask non-differentiable in the kinematics where wing and body overlap). */
+int flusi_adjoint_ncontrols(flusi_ctx* ctx, int* n);
+int flusi_adjoint_get_controls(flusi_ctx* ctx, double* theta);
+int flusi_adjoint_forward(flusi_ctx* ctx, const double* theta, int nsteps, int every, double* J);
+int flusi_adjoint_backward(flusi_ctx* ctx, double* grad);
+
 /* Batched ensembles: many independent simulations, each from its own .ini,
  * advanced together (no counterpart in the reference, which runs one
  * simulation per MPI job).  Members whose grids and solver parameters agree
