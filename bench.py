@@ -5,10 +5,15 @@ beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512] [--no-cpu] [--no-fp64]
     python bench.py --workload ensemble [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --workload adjoint [--adj-steps 400] [--fd 2]
 
 The ensemble workload is 48 independent penalized simulations (16 each at
 64^3, 96^3 and 128^3, half spheres, half flapping plates, RK2, FP32-mixed)
-advanced as batched launches replayed from one captured hipGraph.
+advanced as batched launches replayed from one captured hipGraph.  The
+adjoint workload is d(mean lift over 400 RK4 steps)/d(12 wing kinematics
+coefficients) of the flapping insect at 128^3 (checkpoint every 20 steps), in
+FP64 and FP32-mixed, with central differences of the FP64 mean lift for the
+largest gradient components beside it.
 
 Prints one JSON line (rank 0).  Data are synthetic: the flow starts from rest
 and is driven by the flapping wings (quiescent initial condition), with the
@@ -18,6 +23,8 @@ import argparse
 import json
 import math
 import os
+
+import numpy as np
 import subprocess
 import sys
 import tempfile
@@ -220,6 +227,59 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint(args):
+    import shutil
+    import flusi_amd
+    d = tempfile.mkdtemp()
+    for f in ("cfg4_insect_adjoint_128.ini", "kin_cfg4.ini"):
+        shutil.copy(os.path.join(ROOT, "flusi_amd", "configs", f), d)
+    ini = os.path.join(d, "cfg4_insect_adjoint_128.ini")
+    K, every = args.adj_steps, 20
+    out = {}
+    for name, prec in (("fp64", flusi_amd.FP64), ("fp32", flusi_amd.FP32_MIXED)):
+        s = flusi_amd.Flusi(ini, prec)
+        th = s.adjoint_controls()
+        t0 = time.perf_counter()
+        J = s.adjoint_forward(th, K, every)
+        t1 = time.perf_counter()
+        g = s.adjoint_backward()
+        t2 = time.perf_counter()
+        out[name] = {"J": J, "grad": list(g), "forward_s": t1 - t0, "backward_s": t2 - t1}
+        print(f"adjoint {name}: J={J:.10g} forward {t1 - t0:.2f}s backward {t2 - t1:.2f}s", file=sys.stderr, flush=True)
+        if name == "fp64" and args.fd > 0:
+            h = 1.0e-4
+            fd = {}
+            for i in np.argsort(-np.abs(g))[:args.fd]:
+                tp, tm = th.copy(), th.copy()
+                tp[i] += h
+                tm[i] -= h
+                fd[int(i)] = (s.adjoint_forward(tp, K, every) - s.adjoint_forward(tm, K, every)) / (2 * h)
+            out["fd"] = {"h": h, "components": fd,
+                         "max_rel_err": max(abs(out["fp64"]["grad"][i] - v) for i, v in fd.items())
+                         / max(abs(v) for v in fd.values())}
+            print(f"adjoint fd check {out['fd']}", file=sys.stderr, flush=True)
+        s.close()
+    n = 128
+    g64, g32 = np.array(out["fp64"]["grad"]), np.array(out["fp32"]["grad"])
+    t64 = out["fp64"]["forward_s"] + out["fp64"]["backward_s"]
+    res = {
+        "metric": "adjoint gradient: grid-pt·steps/s of forward + backward sweep, 128^3 insect, 12 controls",
+        "value": n ** 3 * K / t64, "unit": "grid-pt·steps/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": 1e3 * t64 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "d(mean lift)/d(12 Fourier coefficients of the wing kinematics), flapping insect "
+                               "128^3, RK4 dt=9e-5, eta=1e-4, checkpoint every 20 steps", "grid": [n, n, n],
+                   "controls": len(g64), "checkpoint_every": every},
+        "roofline": None,
+        "J_fp64": out["fp64"]["J"], "J_fp32": out["fp32"]["J"],
+        "grad_fp64": out["fp64"]["grad"], "grad_fp32_rel_l2_vs_fp64": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+        "fd_check": out.get("fd"),
+        "fp32": {"forward_s": out["fp32"]["forward_s"], "backward_s": out["fp32"]["backward_s"]},
+        "fp64": {"forward_s": out["fp64"]["forward_s"], "backward_s": out["fp64"]["backward_s"]},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,7 +288,9 @@ def main():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", choices=["insect", "ensemble"], default="insect")
+    ap.add_argument("--workload", choices=["insect", "ensemble", "adjoint"], default="insect")
+    ap.add_argument("--adj-steps", type=int, default=400)
+    ap.add_argument("--fd", type=int, default=2, help="central-difference checks of the largest gradient components")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for HBM bytes)")
     args = ap.parse_args()
@@ -247,6 +309,9 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         run_ensemble(args)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args)
         return
     import flusi_amd
     n = args.grid

@@ -121,7 +121,8 @@ __global__ void adj_point_k(long npts, double eps_inv, T* __restrict__ phi, cons
 
 constexpr int RED_NT = 256;
 
-__device__ __forceinline__ void block_partial(double v, double* partial) {
+// the block's sum into partial[blockIdx.x * nv]
+__device__ __forceinline__ void block_partial(double v, double* partial, int nv = 1) {
   __shared__ double red[RED_NT / 64];
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -129,7 +130,7 @@ __device__ __forceinline__ void block_partial(double v, double* partial) {
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int k = 0; k < RED_NT / 64; ++k) s += red[k];
-    partial[blockIdx.x] = s;
+    partial[(long)blockIdx.x * nv] = s;
   }
 }
 
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(RED_NT) void theta_k(long npts, const T* __restrict
                                                   const T* __restrict__ us, const T* __restrict__ mask,
                                                   const double* __restrict__ mP, const double* __restrict__ usP,
                                                   const double* __restrict__ mM, const double* __restrict__ usM,
-                                                  double* __restrict__ partial) {
+                                                  double* __restrict__ partial, int nv) {
   double acc = 0.0;
   for (long p = (long)blockIdx.x * RED_NT + threadIdx.x; p < npts; p += (long)gridDim.x * RED_NT) {
     const double dm = mP[p] - mM[p];
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(RED_NT) void theta_k(long npts, const T* __restrict
       acc += f * (dm * ((double)u[c * npts + p] - (double)us[c * npts + p]) - m * dus);
     }
   }
-  block_partial(acc, partial);
+  block_partial(acc, partial, nv);
 }
 
 // sum over points of m (u_z - us_z)
@@ -209,9 +210,9 @@ void launch_adj_point(long npts, double eps_inv, void* phi, const void* u, const
 }
 template <typename T>
 void launch_theta(long npts, const void* phi, const void* u, const void* us, const void* mask, const double* mP,
-                  const double* usP, const double* mM, const double* usM, double* partial, hipStream_t s) {
+                  const double* usP, const double* mM, const double* usM, double* partial, int nv, hipStream_t s) {
   hipLaunchKernelGGL(theta_k<T>, dim3(adjoint_reduce_blocks(npts)), dim3(RED_NT), 0, s, npts, (const T*)phi,
-                     (const T*)u, (const T*)us, (const T*)mask, mP, usP, mM, usM, partial);
+                     (const T*)u, (const T*)us, (const T*)mask, mP, usP, mM, usM, partial, nv);
 }
 template <typename T>
 void launch_lift(long npts, const void* u, const void* us, const void* mask, double* partial, hipStream_t s) {
@@ -230,10 +231,11 @@ void launch_ez_mask(long npts, double scale, const void* mask, void* out, hipStr
   template void launch_adj_point<T>(long, double, void*, const void*, const void*, const void*, void*,            \
                                     hipStream_t);                                                                 \
   template void launch_theta<T>(long, const void*, const void*, const void*, const void*, const double*,          \
-                                const double*, const double*, const double*, double*, hipStream_t);               \
+                                const double*, const double*, const double*, double*, int, hipStream_t);          \
   template void launch_lift<T>(long, const void*, const void*, const void*, double*, hipStream_t);                \
   template void launch_ez_mask<T>(long, double, const void*, void*, hipStream_t);
 FLUSI_ADJ_INST(float)
 FLUSI_ADJ_INST(double)
+
 
 }  // namespace flusi

@@ -738,12 +738,16 @@ class Solver : public SolverBase {
     void* ph = nullptr;             // physical: phi (3), a_w (3)
     double *mP = nullptr, *usP = nullptr, *mM = nullptr, *usM = nullptr;
     uint8_t* colS = nullptr;
-    double* part = nullptr;
+    double* part = nullptr;         // reduction partials (blocks x controls)
+    double* gstage = nullptr;       // one call's per-control sums
+    double* gacc = nullptr;         // the gradient, accumulated on the device
+    int* kzero = nullptr;           // reduction kinds: all sums
     double J = 0.0;
     int nck = 0;
     bool have_forward = false;
   } adj;
   double mask_h = 1.0e-6;  // control step of the mask differences (the controls' units)
+  static constexpr int kMaxControls = 64;
   // Adjoint runs redraw the whole insect at every mask update, so that the
   // mask is a function of time and controls only.  (The reference keeps the
   // body from its first drawing and clears only the previous wings' points;
@@ -798,7 +802,16 @@ class Solver : public SolverBase {
       a.usP = (double*)dalloc(3 * sizeof(double) * npts);
       a.usM = (double*)dalloc(3 * sizeof(double) * npts);
       a.colS = (uint8_t*)dalloc(npts);
-      a.part = (double*)dalloc(sizeof(double) * adjoint_reduce_blocks(npts));
+    }
+    const int ncon = (int)a.theta.size();
+    if (!a.part || a.gstage == nullptr || ncon > kMaxControls) {
+      if (ncon > kMaxControls) throw FlusiError(900026, "adjoint: at most 64 controls");
+      dfree(a.part); dfree(a.gstage); dfree(a.gacc); dfree(a.kzero);
+      a.part = (double*)dalloc(sizeof(double) * adjoint_reduce_blocks(npts) * kMaxControls);
+      a.gstage = (double*)dalloc(sizeof(double) * kMaxControls);
+      a.gacc = (double*)dalloc(sizeof(double) * kMaxControls);
+      a.kzero = (int*)dalloc(sizeof(int) * kMaxControls);
+      launch_zero(a.kzero, sizeof(int) * kMaxControls, stream);
     }
     if (a.nck < nck || a.every != every) {
       dfree(a.ck); dfree(a.seg);
@@ -876,19 +889,23 @@ class Solver : public SolverBase {
     insect.set_control(i, v);
     kcheck("perturbed masks");
   }
-  // grad_i += scale * sum f . (dchi_i (u - us) - chi dus_i) / (2h) at time t, with
-  // f = phi or e_z (phi null), u in pf(0..2), mask and us at t
-  void theta_terms(double t, const void* phi, double scale, double* grad) {
+  // gacc_i += scale * sum f . (dchi_i (u - us) - chi dus_i) / (2h) at time t,
+  // with f = phi or e_z (phi null), u in pf(0..2), mask and us at t; all on
+  // the device
+  void theta_terms(double t, const void* phi, double scale) {
     const int ncon = (int)adj.theta.size();
+    const int nb = adjoint_reduce_blocks(npts);
     for (int i = 0; i < ncon; ++i) {
       perturbed_masks(t, i);
-      launch_theta<T>(npts, phi, pf(0), us, mask, adj.mP, adj.usP, adj.mM, adj.usM, adj.part, stream);
-      grad[i] += scale * reduce_partials(adj.part, adjoint_reduce_blocks(npts)) / (2.0 * mask_h);
+      launch_theta<T>(npts, phi, pf(0), us, mask, adj.mP, adj.usP, adj.mM, adj.usM, adj.part + i, ncon, stream);
     }
+    launch_reduce_rows(adj.part, nb, ncon, adj.kzero, rmid, adj.gstage, stream);
+    launch_axpby<double>(ncon, 1.0, adj.gacc, scale / (2.0 * mask_h), adj.gstage, adj.gacc, stream);
+    kcheck("theta terms");
     insect.update(t);
   }
   // W = N_u(u_s)^T lam at time t_s; grad += <phi, dg/dtheta>
-  void adjoint_rhs(const void* u_s, double t_s, const void* lam, void* W, double* grad) {
+  void adjoint_rhs(const void* u_s, double t_s, const void* lam, void* W) {
     const SpecGeom g = sgeom();
     const double N = double(nx) * double(ny) * double(nz);
     create_mask(t_s);
@@ -898,7 +915,7 @@ class Solver : public SolverBase {
     void* aw = (char*)adj.ph + 3 * sizeof(T) * npts;
     ifft_fields(adj.mw, 3, phi);
     // g = u x w - chi/eta (u - us): dg/dtheta = -(1/eta)(dchi (u - us) - chi dus)
-    theta_terms(t_s, phi, -1.0 / p.eps, grad);
+    theta_terms(t_s, phi, -1.0 / p.eps);
     launch_adj_point<T>(npts, 1.0 / p.eps, phi, pf(0), pf(3), mask, aw, stream);
     fft_fields(phi, 3, adj.A);
     fft_fields(aw, 3, adj.B);
@@ -906,13 +923,13 @@ class Solver : public SolverBase {
     kcheck("adjoint rhs");
   }
   // ub += dJ/du at step n (its state u_n), grad += dJ/dtheta there
-  void objective_terms(int n, const void* u_n, double* grad) {
+  void objective_terms(int n, const void* u_n) {
     const double dV = p.dx() * p.dy() * p.dz();
     const double N = double(nx) * double(ny) * double(nz);
     const double s = dV / (p.eps * adj.K);
     create_mask(adj.tn[n]);
     ifft_fields(u_n, 3, pf(0));
-    theta_terms(adj.tn[n], nullptr, s, grad);
+    theta_terms(adj.tn[n], nullptr, s);
     void* src = adj.ph;
     launch_ez_mask<T>(npts, s, mask, src, stream);
     fft_fields(src, 3, adj.A);
@@ -939,8 +956,8 @@ class Solver : public SolverBase {
     const int K = a.K, E = a.every, ncon = (int)a.theta.size();
     const double dt = p.dt_fixed;
     const size_t ns = 3 * nc * 2;  // reals of a state
-    std::vector<double> grad(ncon, 0.0);
     launch_zero(a.ub, state_bytes(), stream);
+    launch_zero(a.gacc, sizeof(double) * kMaxControls, stream);
     const int nseg = (K + E - 1) / E;
     for (int sg = nseg - 1; sg >= 0; --sg) {
       const int n0 = sg * E, n1 = std::min(K, n0 + E);
@@ -953,35 +970,34 @@ class Solver : public SolverBase {
         copy_state(state_at(a.seg, n + 1 - n0), uk);
       }
       for (int n = n1 - 1; n >= n0; --n) {
-        objective_terms(n + 1, state_at(a.seg, n + 1 - n0), grad.data());
+        objective_terms(n + 1, state_at(a.seg, n + 1 - n0));
         const void* u_n = state_at(a.seg, n - n0);
         stage_states(u_n, a.tn[n]);
         const double t = a.tn[n];
         // stage 4 (u_4 at t + dt)
         launch_axpby<T>(ns, dt / 6.0, a.ub, 0.0, nullptr, a.kb, stream);
-        adjoint_rhs(a.st[2], t + dt, a.kb, a.wv, grad.data());
+        adjoint_rhs(a.st[2], t + dt, a.kb, a.wv);
         launch_axpby<T>(ns, 1.0, a.ub, 1.0, a.wv, a.un, stream);
         // stage 3 (u_3 at t + dt/2)
         launch_axpby<T>(ns, dt / 3.0, a.ub, dt, a.wv, a.kb, stream);
-        adjoint_rhs(a.st[1], t + 0.5 * dt, a.kb, a.wv, grad.data());
+        adjoint_rhs(a.st[1], t + 0.5 * dt, a.kb, a.wv);
         launch_axpby<T>(ns, 1.0, a.un, 1.0, a.wv, a.un, stream);
         // stage 2 (u_2 at t + dt/2)
         launch_axpby<T>(ns, dt / 3.0, a.ub, 0.5 * dt, a.wv, a.kb, stream);
-        adjoint_rhs(a.st[0], t + 0.5 * dt, a.kb, a.wv, grad.data());
+        adjoint_rhs(a.st[0], t + 0.5 * dt, a.kb, a.wv);
         launch_axpby<T>(ns, 1.0, a.un, 1.0, a.wv, a.un, stream);
         // stage 1 (u_n at t)
         launch_axpby<T>(ns, dt / 6.0, a.ub, 0.5 * dt, a.wv, a.kb, stream);
-        adjoint_rhs(u_n, t, a.kb, a.wv, grad.data());
+        adjoint_rhs(u_n, t, a.kb, a.wv);
         launch_axpby<T>(ns, 1.0, a.un, 1.0, a.wv, a.ub, stream);
       }
     }
-    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipMemcpyAsync(grad_out, a.gacc, sizeof(double) * ncon, hipMemcpyDeviceToHost, stream));
     // the solver is left at the initial state
     copy_state(uk, state_at(a.ck, 0));
     time = a.tn[0]; it = 0;
     create_mask(time);
     HIPCHECK(hipStreamSynchronize(stream));
-    for (int i = 0; i < ncon; ++i) grad_out[i] = grad[i];
   }
 
   // ------------------------------------------------------------ masks
