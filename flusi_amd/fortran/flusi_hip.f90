@@ -175,6 +175,41 @@ module flusi_hip
       type(c_ptr), value :: names
     end function
 
+    integer(c_int) function flusi_autotune(ctx, reps, variant, ms) bind(C, name="flusi_autotune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reps
+      integer(c_int), dimension(5), intent(out) :: variant
+      real(c_double), intent(out) :: ms
+    end function
+
+    ! discrete adjoint: d(mean lift)/d(wing kinematics Fourier coefficients)
+    integer(c_int) function flusi_adjoint_ncontrols(ctx, n) bind(C, name="flusi_adjoint_ncontrols")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: n
+    end function
+
+    integer(c_int) function flusi_adjoint_get_controls(ctx, theta) bind(C, name="flusi_adjoint_get_controls")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(out) :: theta
+    end function
+
+    integer(c_int) function flusi_adjoint_forward(ctx, theta, nsteps, every, J) bind(C, name="flusi_adjoint_forward")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(in) :: theta
+      integer(c_int), value :: nsteps, every
+      real(c_double), intent(out) :: J
+    end function
+
+    integer(c_int) function flusi_adjoint_backward(ctx, grad) bind(C, name="flusi_adjoint_backward")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(out) :: grad
+    end function
+
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s
