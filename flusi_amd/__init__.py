@@ -49,6 +49,7 @@ EXPORTED: Dict[str, tuple] = {
     "flusi_get_uk": (_I, [_P, _D]),
     "flusi_get_u": (_I, [_P, _D]),
     "flusi_get_vorticity": (_I, [_P, _D]),
+    "flusi_get_pressure": (_I, [_P, ctypes.c_double, _D]),
     "flusi_cal_nlk": (_I, [_P, ctypes.c_double, _I, _D, _D]),
     "flusi_fluid_timestep": (_I, [_P, _D, ctypes.POINTER(_I), _D, _D]),
     "flusi_step": (_I, [_P]),
@@ -199,6 +200,13 @@ class Flusi:
         self._check(self._lib.flusi_get_vorticity(self._ctx, _dp(a)))
         return a
 
+    def get_pressure(self, time: float) -> np.ndarray:
+        """save_fields' pressure at `time` (mask regenerated when moving):
+        (nz, ny, nx) float64."""
+        out = np.empty((self.nz, self.ny, self.nx))
+        self._check(self._lib.flusi_get_pressure(self._ctx, float(time), _dp(out)))
+        return out
+
     def cal_nlk(self, time: float, it: int = 0, want_u: bool = False):
         nlk = np.empty(self.spectral_shape, dtype=np.complex128)
         u = np.empty(self.physical_shape) if want_u else None
@@ -300,8 +308,8 @@ class Flusi:
 
     def autotune(self, reps: int = 3):
         """Picks the fastest kernel variant for this grid (cached per grid,
-        precision and batch size); returns (variant[5], ms per right-hand side)."""
-        v = (_I * 5)()
+        precision and batch size); returns (variant[6], ms per right-hand side)."""
+        v = (_I * 6)()
         ms = ctypes.c_double()
         self._check(self._lib.flusi_autotune(self._ctx, int(reps), v, ctypes.byref(ms)))
         return list(v), ms.value
@@ -375,10 +383,10 @@ class Ensemble:
         self._check(self._lib.flusi_ensemble_init_fields(self._e))
 
     def autotune(self, reps: int = 3):
-        v = (_I * (5 * self.nbatches))()
+        v = (_I * (6 * self.nbatches))()
         ms = (ctypes.c_double * self.nbatches)()
         self._check(self._lib.flusi_ensemble_autotune(self._e, int(reps), v, ms))
-        return [(list(v[5 * b:5 * b + 5]), ms[b]) for b in range(self.nbatches)]
+        return [(list(v[6 * b:6 * b + 6]), ms[b]) for b in range(self.nbatches)]
 
     def advance(self, nsteps: int, mode: int = ENSEMBLE_GRAPH):
         self._check(self._lib.flusi_ensemble_advance(self._e, int(nsteps), int(mode)))

@@ -58,9 +58,8 @@ struct InsectArgs {
 };
 
 // capsule record: segment end points (global, not periodized), radius, and
-// the end spheres' centres (periodized) with their index boxes
-enum CapsuleField { CAP_A = 0, CAP_B = 3, CAP_R = 6, CAP_SA = 7, CAP_SB = 10,
-                    CAP_SA_LO = 13, CAP_SA_HI = 16, CAP_SB_LO = 19, CAP_SB_HI = 22, CAPSULE_W = 25 };
+// the end spheres' centres wrapped into the box
+enum CapsuleField { CAP_A = 0, CAP_B = 3, CAP_R = 6, CAP_SA = 7, CAP_SB = 10, CAPSULE_W = 13 };
 
 // drag.f90:17-454 + integrals.f90:48-238 reductions over physical u, w, div
 struct DiagArgs {
@@ -96,6 +95,8 @@ template <typename T> void launch_cavity(const GridGeom& g, const CavityArgs& a,
 // batch member (bat.nmem > 1: every member's region must equal a's)
 template <typename T> void launch_insect(const GridGeom& g, const InsectArgs& a, const InsectArgs* a_dev, void* mask,
                                          void* us, uint8_t* color, hipStream_t s, const Batch& bat);
+// p -= |u|^2 / 2 (u: 3 physical fields, p: 1)
+template <typename T> void launch_pressure_phys(long npts, const void* u, void* p, hipStream_t s);
 template <typename T> void launch_vortex_ring(const GridGeom& g, double x0, double y0, double z0, double r00, double a, double gamma0, void* vort, hipStream_t s);
 // spectral helpers on S-layout fields (3 components, component stride ncplx)
 template <typename T> void launch_biot_savart(const GridGeom& g, const double* kx, const double* ky, const double* kz, const void* vk, void* uk, hipStream_t s);

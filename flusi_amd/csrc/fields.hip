@@ -230,19 +230,26 @@ __global__ void insect_k(GridGeom g, const InsectArgs* __restrict__ ap, T* mask,
 
   if (a.draw_body && a.ncaps > 0) {
     // body_geometry.f90:1531-1647 draw_cylinder_new and its end spheres
-    // (:1657-1724 drawsphere), in the reference's drawing order
+    // (:1657-1724 drawsphere), in the reference's drawing order.  With a
+    // periodic insect every capsule is drawn whole across the box boundary
+    // (distances to the segment midpoint and to the end points periodized),
+    // as the reference's insects_flow_BB_tethered_AB2_periodic fixture shows
+    // (its mask volume equals the centred insect's to 1e-9).
     const double xa[3] = {ix * g.dx, iy * g.dy, iz * g.dz};
-    const int ii[3] = {ix, iy, iz};
     for (int k = 0; k < a.ncaps; ++k) {
       const double* cp = a.caps + (long)k * CAPSULE_W;
       const double r = cp[CAP_R], rr0 = r + a.safety;
+      double d[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) d[i] = xa[i] - 0.5 * (cp[CAP_A + i] + cp[CAP_B + i]);
+      periodize_d(d, g);
       double cb2 = 0.0, rb2 = 0.0, u2 = 0.0, vp[3], ab[3], u[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const double cb = 0.5 * (cp[CAP_A + i] + cp[CAP_B + i]) - xa[i];
+        const double cb = -d[i];
         const double rb = cp[CAP_A + i] - cp[CAP_B + i];
         cb2 += cb * cb; rb2 += rb * rb;
-        ab[i] = xa[i] - cp[CAP_A + i];
+        ab[i] = g.periodic ? (d[i] + 0.5 * (cp[CAP_A + i] + cp[CAP_B + i])) - cp[CAP_A + i] : xa[i] - cp[CAP_A + i];
         u[i] = cp[CAP_B + i] - cp[CAP_A + i];
         u2 += u[i] * u[i];
       }
@@ -258,13 +265,7 @@ __global__ void insect_k(GridGeom g, const InsectArgs* __restrict__ ap, T* mask,
         }
       }
       for (int e = 0; e < 2; ++e) {
-        const double* sc = cp + (e ? CAP_SB : CAP_SA);
-        const double* lo = cp + (e ? CAP_SB_LO : CAP_SA_LO);
-        const double* hi = cp + (e ? CAP_SB_HI : CAP_SA_HI);
-        bool inb = true;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) inb = inb && ii[i] >= (int)lo[i] && ii[i] <= (int)hi[i];
-        if (!inb) continue;
+        const double* sc = cp + (e ? CAP_SB : CAP_SA);  // end point, wrapped into the box
         double xs[3] = {xa[0] - sc[0], xa[1] - sc[1], xa[2] - sc[2]};
         periodize_d(xs, g);
         const double R = sqrt(xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2]);
@@ -661,6 +662,22 @@ template <typename T> void launch_to_f64(const void* src, double* dst, long n, h
   const int nt = 256;
   hipLaunchKernelGGL(to_f64_k<T>, dim3((n + nt - 1) / nt), dim3(nt), 0, s, (const T*)src, dst, n);
 }
+
+// save_fields.f90:129-133: p = Pi - |u|^2 / 2 (rotational form), in place
+template <typename T>
+__global__ void pressure_phys_k(long npts, const T* __restrict__ u, T* __restrict__ p) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  const double a = u[i], b = u[npts + i], c = u[2 * npts + i];
+  p[i] = (T)((double)p[i] - 0.5 * ((a * a + b * b) + c * c));
+}
+template <typename T>
+void launch_pressure_phys(long npts, const void* u, void* p, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(pressure_phys_k<T>, dim3((npts + nt - 1) / nt), dim3(nt), 0, s, npts, (const T*)u, (T*)p);
+}
+template void launch_pressure_phys<float>(long, const void*, void*, hipStream_t);
+template void launch_pressure_phys<double>(long, const void*, void*, hipStream_t);
 
 __global__ void zero_k(uint32_t* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -104,6 +104,7 @@ enum StepMode {
   SM_RK4_2 = 6,
   SM_RK4_3 = 7,
   SM_RK4_4 = 8,
+  SM_PRESSURE = 9,   // out0 = -i k.nlk / k^2 of the unprojected right-hand side (1 field)
 };
 
 struct ZStepArgs {
@@ -138,6 +139,7 @@ struct ZStepArgs {
   double mean_value[3];
   double* energy_partial;  // per block Parseval partial sum (final stages only)
   int wave;              // 1: transform each z line inside one wavefront (nz = 64..512)
+  int eu;                // block kernel, nz >= 256: epilogue positions scheduled together (0 = all)
   Batch bat;             // grid z: member
 };
 

@@ -89,6 +89,7 @@ class SolverBase {
   virtual void set_uk(const double* uk) = 0;
   virtual void get_uk(double* uk) = 0;
   virtual void get_u(double* u, bool vort) = 0;
+  virtual void get_pressure(double time, double* p) = 0;
   virtual void cal_nlk(double time, double* nlk, double* u) = 0;
   virtual void fluid_timestep() = 0;
   virtual void create_mask(double time) = 0;
@@ -172,6 +173,7 @@ class Solver : public SolverBase {
   bool in_setup = false;
   std::vector<void*> owned;    // device allocations freed by this solver
   bool zwave = false;          // z-forward step pass with in-wave line transforms
+  int zeu = 0;                 // block z step, nz >= 256: epilogue positions scheduled together (0 = all)
   double* fw_d = nullptr;     // Fourier wing radius table
   bool xwave = false;       // right-hand-side x pass with one wavefront per line
   double* zpart = nullptr; int nzblocks = 0;
@@ -289,6 +291,7 @@ class Solver : public SolverBase {
       tw_z = make_tw(nz, nz); tw_y = make_tw(ny, ny); tw_M = make_tw(nx / 2, nx / 2); tw_N = make_tw(nxh, nx);
       if (const char* e = getenv("FLUSI_TILE_WAVE")) twave = atoi(e) != 0;
       if (const char* e = getenv("FLUSI_ZWAVE")) zwave = atoi(e) != 0;
+      if (const char* e = getenv("FLUSI_ZSTEP_EU")) zeu = atoi(e);
       if (const char* e = getenv("FLUSI_TILE_PREFETCH")) tpf = atoi(e) != 0;
       if (const char* e = getenv("FLUSI_XWAVE_WPE")) xwpe = atoi(e);
       xwave = x_rhs_wave_supported(nx);
@@ -485,6 +488,7 @@ class Solver : public SolverBase {
     z.dynamic_scale = sc / p.m_fluid;
     for (int c = 0; c < 3; ++c) z.meanflow_dynamic[c] = p.iMeanFlow[c] == "dynamic";
     z.wave = zwave ? 1 : 0;
+    z.eu = zeu;
     z.bat = bat;
     return z;
   }
@@ -561,10 +565,10 @@ class Solver : public SolverBase {
   // whole batch when one is active) for every variant the grid allows and
   // keeps the fastest; results are cached per grid, precision, sponge and
   // batch size for the life of the process.
-  struct Variant { int twave, tpf, xwave, xwpe, zwave; };
-  Variant variant() const { return Variant{twave, tpf, xwave, xwpe, zwave}; }
+  struct Variant { int twave, tpf, xwave, xwpe, zwave, zeu; };
+  Variant variant() const { return Variant{twave, tpf, xwave, xwpe, zwave, zeu}; }
   void apply(const Variant& v) {
-    twave = v.twave != 0; tpf = v.tpf != 0; xwave = v.xwave != 0; xwpe = v.xwpe; zwave = v.zwave != 0;
+    twave = v.twave != 0; tpf = v.tpf != 0; xwave = v.xwave != 0; xwpe = v.xwpe; zwave = v.zwave != 0; zeu = v.zeu;
   }
   double time_rhs(int reps) {
     auto rhs = [&] {
@@ -600,7 +604,10 @@ class Solver : public SolverBase {
             if (!xv && wpe == 0) continue;  // register budget only acts in the wave x pass
             for (int zv : {0, 1}) {
               if (zv && !zw) continue;
-              c.push_back(Variant{tw, pf, xv, wpe, zv});
+              for (int eu : {0, 2, 4}) {
+                if (eu && (zv || nz < 256)) continue;  // grouping acts in the block z step at nz >= 256
+                c.push_back(Variant{tw, pf, xv, wpe, zv, eu});
+              }
             }
           }
         }
@@ -617,7 +624,8 @@ class Solver : public SolverBase {
     return std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz) + (FP64 ? ":f64" : ":f32") +
            ":sp" + std::to_string(sponge_code()) + ":m" + std::to_string(batch_n(bat));
   }
-  // v: the chosen variant (5 ints: twave, tpf, xwave, xwpe, zwave); ms: its
+  static constexpr int kVariantInts = 6;
+  // v: the chosen variant (6 ints: twave, tpf, xwave, xwpe, zwave, zeu); ms: its
   // time per right-hand side; force: tune again even when cached
   void autotune(int reps, int* v, double* ms, bool force) {
     need_fft();
@@ -642,7 +650,11 @@ class Solver : public SolverBase {
       tune_cache()[key] = best;
     }
     apply(best.first);
-    if (v) { v[0] = best.first.twave; v[1] = best.first.tpf; v[2] = best.first.xwave; v[3] = best.first.xwpe; v[4] = best.first.zwave; }
+    if (v) {
+      const Variant& b = best.first;
+      const int o[6] = {b.twave, b.tpf, b.xwave, b.xwpe, b.zwave, b.zeu};
+      for (int i = 0; i < kVariantInts; ++i) v[i] = o[i];
+    }
     if (ms) *ms = best.second;
   }
   void autotune_v(int reps, int* v, double* ms) override { autotune(reps, v, ms, false); }
@@ -1139,13 +1151,10 @@ class Solver : public SolverBase {
     if (a.draw_body && a.body_type == 3 && !((scratch || mask_fresh) && caps_d) &&
         (I.HasDetails == "all" || I.HasDetails == "legs" || I.HasDetails == "antennae_proboscis")) {
       // legs, antennae, proboscis (body_geometry.f90:276-345): capsules in the
-      // global frame, end spheres periodized with drawsphere's index boxes
+      // global frame with their end spheres' wrapped centres
       const std::vector<Capsule> cl = bumblebee_capsules();
       std::vector<double> rec(cl.size() * CAPSULE_W, 0.0);
       const double box[3] = {p.xl, p.yl, p.zl};
-      const double ddx[3] = {p.dx(), p.dy(), p.dz()};
-      const int nn[3] = {nx, ny, nz};
-      const double mind = std::min(ddx[0], std::min(ddx[1], ddx[2]));
       const Mat3 MbT = transpose(s.Mb);
       for (size_t k = 0; k < cl.size(); ++k) {
         double* r = rec.data() + k * CAPSULE_W;
@@ -1153,17 +1162,11 @@ class Solver : public SolverBase {
         for (int i = 0; i < 3; ++i) { A[i] += s.xc[i]; B[i] += s.xc[i]; }
         for (int i = 0; i < 3; ++i) { r[CAP_A + i] = A[i]; r[CAP_B + i] = B[i]; }
         r[CAP_R] = cl[k].r;
-        const int ns = fortran_nint((cl[k].r + I.safety) / mind);
-        for (int e = 0; e < 2; ++e) {
+        for (int e = 0; e < 2; ++e) {  // drawsphere's wrap of the centre (body_geometry.f90:1674-1681)
           Vec3 c = e ? B : A;
           for (int i = 0; i < 3; ++i) if (c[i] < 0.0) c[i] += box[i];
           for (int i = 0; i < 3; ++i) if (c[i] >= box[i]) c[i] -= box[i];
-          for (int i = 0; i < 3; ++i) {
-            const int ci = fortran_nint(c[i] / ddx[i]);
-            r[(e ? CAP_SB : CAP_SA) + i] = c[i];
-            r[(e ? CAP_SB_LO : CAP_SA_LO) + i] = std::max(ci - ns, 0);
-            r[(e ? CAP_SB_HI : CAP_SA_HI) + i] = std::min(ci + ns, nn[i] - 1);
-          }
+          for (int i = 0; i < 3; ++i) r[(e ? CAP_SB : CAP_SA) + i] = c[i];
         }
       }
       if (!caps_d) caps_d = (double*)dalloc(sizeof(double) * rec.size());
@@ -1346,6 +1349,23 @@ class Solver : public SolverBase {
     zfwd_plain(nf, dst);
   }
 
+  // save_fields.f90:98-133: the mask at t, the unprojected right-hand side,
+  // pressure (rhs.f90:342-379), ifft, minus |u|^2/2
+  void get_pressure(double t, double* out) override {
+    need_fft();
+    if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    pre_rhs(uk, t);
+    ZStepArgs z = zargs(SM_PRESSURE);
+    z.rhs_out = P; z.udiff = uk;
+    post_rhs(z);
+    ensure_phys();
+    ifft_fields(P, 1, pf(6));
+    ifft_fields(uk, 3, pf(0));
+    launch_pressure_phys<T>(npts, pf(0), pf(6), stream);
+    kcheck("pressure");
+    copy_real_out(pf(6), out, npts);
+  }
+
   void get_u(double* u, bool vort) override {
     need_fft();
     inv_uw_phys(uk);
@@ -1462,7 +1482,10 @@ class Solver : public SolverBase {
     DiagArgs d{};
     d.u = pf(0); d.w = pf(3); d.div = pf(6); d.mask = mask; d.us = us; d.color = color;
     d.eps_inv = 1.0 / p.eps;
+    // moments about (x0, y0, z0), which body_motion.f90:250-255 sets to the
+    // insect's body centre at every update
     d.x0[0] = p.x0; d.x0[1] = p.y0; d.x0[2] = p.z0;
+    if (ins) for (int i = 0; i < 3; ++i) d.x0[i] = insect.st.xc[i];
     d.insect = ins ? 1 : 0;
     if (ins) for (int i = 0; i < 3; ++i) { d.piv_l[i] = insect.st.x_pivot_l_g[i]; d.piv_r[i] = insect.st.x_pivot_r_g[i]; }
     d.partial = dpart;
@@ -1731,7 +1754,7 @@ class Ensemble : public EnsembleBase {
     for (size_t b = 0; b < groups.size(); ++b) {
       Group& g = groups[b];
       BatchScope bs(*g.lead, g.idx.size(), g.stride);
-      g.lead->autotune(reps, variants ? variants + 5 * b : nullptr, ms ? ms + b : nullptr, false);
+      g.lead->autotune(reps, variants ? variants + Solver<T>::kVariantInts * b : nullptr, ms ? ms + b : nullptr, false);
       for (int i : g.idx) S(i).apply(g.lead->variant());
     }
     if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; (void)hipGraphDestroy(graph); graph = nullptr; }
@@ -1998,6 +2021,9 @@ int flusi_set_uk(flusi_ctx* ctx, const double* uk) { return guarded(ctx, [&] { c
 int flusi_get_uk(flusi_ctx* ctx, double* uk) { return guarded(ctx, [&] { ctx->s->get_uk(uk); }); }
 int flusi_get_u(flusi_ctx* ctx, double* u) { return guarded(ctx, [&] { ctx->s->get_u(u, false); }); }
 int flusi_get_vorticity(flusi_ctx* ctx, double* w) { return guarded(ctx, [&] { ctx->s->get_u(w, true); }); }
+int flusi_get_pressure(flusi_ctx* ctx, double time, double* p) {
+  return guarded(ctx, [&] { ctx->s->get_pressure(time, p); });
+}
 int flusi_cal_nlk(flusi_ctx* ctx, double time, int it, double* nlk, double* u) {
   (void)it;
   return guarded(ctx, [&] { ctx->s->cal_nlk(time, nlk, u); });

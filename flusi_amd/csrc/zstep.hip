@@ -25,7 +25,7 @@ __device__ __forceinline__ double vis_factor(double dtv, double nu, double k2, b
 // of its own stepper (the epilogue is unrolled over a thread's elements).
 enum { MG_RHS = 0, MG_AB2 = 1, MG_RK2 = 2, MG_RK4 = 3 };
 __host__ __device__ constexpr int mode_group(int mode) {
-  return mode == SM_RHS_ONLY ? MG_RHS
+  return (mode == SM_RHS_ONLY || mode == SM_PRESSURE) ? MG_RHS
        : (mode == SM_EULER || mode == SM_AB2) ? MG_AB2
        : (mode == SM_RK2_1 || mode == SM_RK2_2) ? MG_RK2 : MG_RK4;
 }
@@ -90,6 +90,20 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       const cplx<T> us_ = UD[c * a.fs + si];
       nr[c] -= a.nu * k2 * (double)us_.x;
       ni[c] -= a.nu * k2 * (double)us_.y;
+    }
+  }
+  if constexpr (MG == MG_RHS) {
+    if (a.mode == SM_PRESSURE) {
+      // rhs.f90:342-379 pressure: -i (k . nlk) / k^2, zero at k = 0
+      double pr = 0.0, pi = 0.0;
+      if (k2 != 0.0) {
+        const double dr = (kxv * nr[0] + kyv * nr[1]) + kzv * nr[2];
+        const double di = (kxv * ni[0] + kyv * ni[1]) + kzv * ni[2];
+        pr = di / k2;
+        pi = -dr / k2;
+      }
+      RO[si] = mk<T>((T)pr, (T)pi);
+      return;
     }
   }
   if (k2 != 0.0) {
@@ -210,7 +224,10 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
   }
 }
 
-template <typename T, int L, int TL, int NT, bool SPONGE, int MG>
+// EU > 0: the epilogue of at most EU positions is scheduled together (a
+// scheduling barrier after every EU positions keeps the compiler from
+// hoisting all positions' state loads, which at L >= 256 spills registers)
+template <typename T, int L, int TL, int NT, bool SPONGE, int MG, int EU>
 __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
   const MOff mo = member_off<T>(a);
   constexpr int NSTG = plan_count_k(PLAN_TILE, L);
@@ -282,6 +299,8 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) rv[f] = r[f][p][q];
     zepilogue<T, SPONGE, MG>(a, mo, kx, ky, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
+    if constexpr (EU > 0)
+      if ((p * RL + q + 1) % EU == 0) __builtin_amdgcn_sched_barrier(0);
   }
   if (final_stage && a.energy_partial) {
     double v = wave_sum(esum + ecomp);
@@ -396,6 +415,17 @@ int zstep_blocks(int nz, int ny, int nxh, bool fp64) {
   return ((ny + TL - 1) / TL) * nxh;
 }
 
+template <typename T, int L, bool SP, int MG, int EU>
+static void run_zb(const ZStepArgs& a, hipStream_t s) {
+  constexpr int TL = Tile<T>::TL;
+  constexpr int NT = pick_nt(TL * L);
+  const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
+  static bool init = false;
+  if (!init) { set_smem(zstep_k<T, L, TL, NT, SP, MG, EU>, smem); init = true; }
+  dim3 grid((a.ny + TL - 1) / TL, a.nxh, batch_n(a.bat));
+  hipLaunchKernelGGL((zstep_k<T, L, TL, NT, SP, MG, EU>), grid, dim3(NT), smem, s, a);
+}
+
 template <typename T, int L, bool SP, int MG>
 static void run_z(const ZStepArgs& a, hipStream_t s) {
   if constexpr (L == 64 || L == 128 || L == 256 || L == 512) {
@@ -408,13 +438,15 @@ static void run_z(const ZStepArgs& a, hipStream_t s) {
       return;
     }
   }
-  constexpr int TL = Tile<T>::TL;
-  constexpr int NT = pick_nt(TL * L);
-  const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
-  static bool init = false;
-  if (!init) { set_smem(zstep_k<T, L, TL, NT, SP, MG>, smem); init = true; }
-  dim3 grid((a.ny + TL - 1) / TL, a.nxh, batch_n(a.bat));
-  hipLaunchKernelGGL((zstep_k<T, L, TL, NT, SP, MG>), grid, dim3(NT), smem, s, a);
+  if constexpr (L >= 256) {
+    switch (a.eu) {
+      case 1: run_zb<T, L, SP, MG, 1>(a, s); return;
+      case 2: run_zb<T, L, SP, MG, 2>(a, s); return;
+      case 4: run_zb<T, L, SP, MG, 4>(a, s); return;
+      default: break;
+    }
+  }
+  run_zb<T, L, SP, MG, 0>(a, s);
 }
 template <typename T, int L, bool SP>
 static void run_z_mg(const ZStepArgs& a, hipStream_t s) {

@@ -175,11 +175,18 @@ module flusi_hip
       type(c_ptr), value :: names
     end function
 
+    integer(c_int) function flusi_get_pressure(ctx, time, p) bind(C, name="flusi_get_pressure")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: time
+      real(c_double), dimension(*), intent(out) :: p
+    end function
+
     integer(c_int) function flusi_autotune(ctx, reps, variant, ms) bind(C, name="flusi_autotune")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       integer(c_int), value :: reps
-      integer(c_int), dimension(5), intent(out) :: variant
+      integer(c_int), dimension(6), intent(out) :: variant
       real(c_double), intent(out) :: ms
     end function
 

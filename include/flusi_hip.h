@@ -51,6 +51,11 @@ int flusi_get_uk(flusi_ctx* ctx, double* uk);
 int flusi_get_u(flusi_ctx* ctx, double* u);
 /* vorticity = ifft3(curl(uk)) (basic_operators.f90:185-215). */
 int flusi_get_vorticity(flusi_ctx* ctx, double* vort);
+/* The pressure field save_fields writes (save_fields.f90:98-133): the mask at
+ * `time` (iMoving=1), the unprojected right-hand side (rhs.f90:147-330),
+ * pressure (rhs.f90:342-379: -i k.nlk/k^2), ifft, minus |u|^2/2.  p: one
+ * real field. */
+int flusi_get_pressure(flusi_ctx* ctx, double time, double* p);
 
 /* rhs.f90:2-115 cal_nlk: projected right-hand side of the current uk at
  * `time` (mask regenerated when iMoving=1).  nlk: 3 spectral components or
@@ -109,9 +114,10 @@ int flusi_timers(flusi_ctx* ctx, int enable, double* ms, const char** names);
  * Runs `reps` right-hand sides of the current state for every variant the
  * grid allows and keeps the fastest for this context; results are cached per
  * grid, precision and batch size for the life of the process.
- * variant[5] (may be NULL): in-wave y/z passes, block-pass prefetch, one-wave
- * x pass, its waves-per-SIMD register budget, in-wave z step.  ms: time of
- * one right-hand side with the chosen variant. */
+ * variant[6] (may be NULL): in-wave y/z passes, block-pass prefetch, one-wave
+ * x pass, its waves-per-SIMD register budget, in-wave z step, epilogue
+ * positions the block z step schedules together (0 = all).  ms: time of one
+ * right-hand side with the chosen variant. */
 int flusi_autotune(flusi_ctx* ctx, int reps, int* variant, double* ms);
 
 /* Discrete adjoint (no counterpart in the reference).  The objective is the
@@ -158,7 +164,7 @@ int flusi_ensemble_size(const flusi_ensemble* ens, int* nmembers, int* nbatches)
 int flusi_ensemble_batch(flusi_ensemble* ens, int b, int* nmem, int* nx, int* ny, int* nz, int* lead);
 int flusi_ensemble_member(flusi_ensemble* ens, int i, flusi_ctx** member);
 int flusi_ensemble_init_fields(flusi_ensemble* ens);
-/* variants: 5 ints per batch, ms: one per batch (either may be NULL) */
+/* variants: 6 ints per batch, ms: one per batch (either may be NULL) */
 int flusi_ensemble_autotune(flusi_ensemble* ens, int reps, int* variants, double* ms);
 /* advances every member by nsteps time steps (time_step.f90:57-167 each) */
 int flusi_ensemble_advance(flusi_ensemble* ens, int nsteps, int mode);

@@ -980,43 +980,51 @@ class Insect:
                 self._draw_cylinder(g, xa, xb, r, mask, color)
 
     def _draw_cylinder(self, g, x1, x2, R0, mask, color):
-        """body_geometry.f90:1531-1647 draw_cylinder_new (global coordinates,
-        not periodized) and its two end spheres.  The reference bounds the loop
-        by a box built from a random perpendicular; with its margin of
-        nint(safety/dx) points the box holds every point of the spherical
-        bound below, so the box is taken here from that sphere."""
-        ddx = (g.dx, g.dy, g.dz)
+        """body_geometry.f90:1531-1647 draw_cylinder_new (global coordinates)
+        and its two end spheres (:1657-1724 drawsphere, centre wrapped into
+        the box).  With a periodic insect the capsule is drawn whole across
+        the box boundary: distances to the segment midpoint and to the end
+        points are periodized.  The source leaves the cylinder unperiodized
+        and clips the spheres' index boxes at the domain, but the reference's
+        insects_flow_BB_tethered_AB2_periodic fixture (insect at the box
+        corner) was made with whole capsules: its t=0 mask keys match this
+        drawing to 1e-9 and miss the clipped one by 5e-2; for an insect clear
+        of the boundary the two drawings are the same."""
+        p = self.p
+        box = (p.xl, p.yl, p.zl)
         sf = self.safety
+        X = (np.arange(g.nx) * g.dx)[None, None, :] + np.zeros((g.nz, g.ny, 1))
+        Y = (np.arange(g.ny) * g.dy)[None, :, None] + np.zeros((g.nz, 1, g.nx))
+        Z = (np.arange(g.nz) * g.dz)[:, None, None] + np.zeros((1, g.ny, g.nx))
         mid = 0.5 * (x1 + x2)
-        half = 0.5 * math.sqrt(float(np.sum((x1 - x2) ** 2)))
-        lo, hi = [], []
-        for i, n in enumerate((g.nx, g.ny, g.nz)):
-            lo.append(max(int(math.floor((mid[i] - half) / ddx[i])) - 1, 0))
-            hi.append(min(int(math.ceil((mid[i] + half) / ddx[i])) + 1, n - 1))
-        if any(l > h for l, h in zip(lo, hi)):
-            return
-        ix = np.arange(lo[0], hi[0] + 1)
-        iy = np.arange(lo[1], hi[1] + 1)
-        iz = np.arange(lo[2], hi[2] + 1)
-        X = (ix * g.dx)[None, None, :] + np.zeros((len(iz), len(iy), 1))
-        Y = (iy * g.dy)[None, :, None] + np.zeros((len(iz), 1, len(ix)))
-        Z = (iz * g.dz)[:, None, None] + np.zeros((1, len(iy), len(ix)))
-        cb = [mid[0] - X, mid[1] - Y, mid[2] - Z]
+        d = periodize([X - mid[0], Y - mid[1], Z - mid[2]], box, p.periodic)
         rb = x1 - x2
-        inb = (cb[0] ** 2 + cb[1] ** 2 + cb[2] ** 2) < 0.25 * float(np.sum(rb ** 2))
-        ab = [X - x1[0], Y - x1[1], Z - x1[2]]
+        inb = (d[0] ** 2 + d[1] ** 2 + d[2] ** 2) < 0.25 * float(np.sum(rb ** 2))
+        if p.periodic:
+            ab = [d[0] + mid[0] - x1[0], d[1] + mid[1] - x1[1], d[2] + mid[2] - x1[2]]
+        else:
+            ab = [X - x1[0], Y - x1[1], Z - x1[2]]
         u = x2 - x1
         vp = [ab[1] * u[2] - ab[2] * u[1], ab[2] * u[0] - ab[0] * u[2], ab[0] * u[1] - ab[1] * u[0]]
         R = np.sqrt((vp[0] ** 2 + vp[1] ** 2 + vp[2] ** 2) / float(np.sum(u ** 2)))
-        sub = (slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
-        m = mask[sub]
-        c = color[sub]
         t = steps(R, R0, self.smooth)
-        sel = inb & (R <= R0 + sf) & (t >= m)
-        m[sel] = t[sel]
-        c[sel] = self.color_body
-        self._drawsphere(g, x1, R0, mask, color)
-        self._drawsphere(g, x2, R0, mask, color)
+        sel = inb & (R <= R0 + sf) & (t >= mask)
+        mask[sel] = t[sel]
+        color[sel] = self.color_body
+        for xe in (x1, x2):
+            xc = np.array(xe, dtype=np.float64)
+            for i in range(3):
+                if xc[i] < 0.0:
+                    xc[i] += box[i]
+            for i in range(3):
+                if xc[i] >= box[i]:
+                    xc[i] -= box[i]
+            d = periodize([X - xc[0], Y - xc[1], Z - xc[2]], box, p.periodic)
+            R = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+            t = steps(R, R0, self.smooth)
+            sel = (R <= R0 + sf) & (t >= mask)
+            mask[sel] = t[sel]
+            color[sel] = self.color_body
 
     def _draw_wings(self, g, mask, color, us):
         """wings_geometry.f90:5-110 draw_insect_wings (stage I right then left
@@ -1253,9 +1261,16 @@ class Simulation:
     def cal_nlk(self, time: float, uk: np.ndarray):
         """rhs.f90:2-115 cal_nlk and :147-330 cal_nlk_fsi (+ add_grad_pressure).
         Returns (nlk, u) with u the physical velocity."""
-        p, g = self.p, self.g
+        p = self.p
         if p.iMoving == 1 and p.iPenalization == 1:
             self.create_mask(time)
+        nlk, u = self.cal_nlk_fsi(time, uk)
+        return add_grad_pressure(self.g, nlk), u
+
+    def cal_nlk_fsi(self, time: float, uk: np.ndarray):
+        """rhs.f90:147-330 cal_nlk_fsi with the current mask: the right-hand
+        side before the projection.  Returns (nlk, u)."""
+        p, g = self.p, self.g
         if p.eps < 1.0e-11:
             raise FlusiAbort(77363800, "Value of eps is very small, maybe even zero.")
         eps_inv = 1.0 / p.eps
@@ -1286,8 +1301,21 @@ class Simulation:
                     nlk[i, 0, 0, 0] += max(0.0, 1.0 - uk[i, 0, 0, 0].real)
         if p.iTimeMethodFluid == "RK4":
             nlk = add_explicit_diffusion(g, uk, nlk, p.nu)
-        nlk = add_grad_pressure(g, nlk)
         return nlk, u
+
+    def pressure_field(self, time: float) -> np.ndarray:
+        """save_fields.f90:98-133: the mask at `time`, cal_nlk_fsi, then
+        pressure (rhs.f90:342-379: p-hat = -i k.nlk / k^2, zero at k = 0),
+        ifft, and p = Pi - |u|^2/2 (the rotational form's total pressure
+        minus the kinetic energy)."""
+        p, g = self.p, self.g
+        if p.iMoving == 1:
+            self.create_mask(time)
+        nlk, u = self.cal_nlk_fsi(time, self.uk)
+        k2 = g.K2
+        div = g.KX * nlk[0] + g.KY * nlk[1] + g.KZ * nlk[2]
+        pk = np.where(k2 != 0.0, (-1j * div) / np.where(k2 != 0.0, k2, 1.0), 0.0)
+        return ifft(g, pk) - 0.5 * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2)
 
     # -- time step control ----------------------------------------------------
     def adjust_dt(self, time: float, u: np.ndarray) -> float:
@@ -1474,9 +1502,12 @@ class Simulation:
         du = u - self.us
         pen = -self.mask[None] * du * eps_inv
         col = self.color.astype(np.int64)
-        X = (g.x[None, None, :] - p.x0) + np.zeros(self.mask.shape)
-        Y = (g.y[None, :, None] - p.y0) + np.zeros(self.mask.shape)
-        Z = (g.z[:, None, None] - p.z0) + np.zeros(self.mask.shape)
+        # moments about (x0, y0, z0), which body_motion.f90:250-255 sets to the
+        # insect's body centre at every update
+        x0, y0, z0 = (self.insect.st["xc"] if p.iMask == "Insect" else (p.x0, p.y0, p.z0))
+        X = (g.x[None, None, :] - x0) + np.zeros(self.mask.shape)
+        Y = (g.y[None, :, None] - y0) + np.zeros(self.mask.shape)
+        Z = (g.z[:, None, None] - z0) + np.zeros(self.mask.shape)
         X, Y, Z = periodize([X, Y, Z], (p.xl, p.yl, p.zl), p.periodic)
         t0 = [-(Y * pen[2] - Z * pen[1]), -(Z * pen[0] - X * pen[2]), -(X * pen[1] - Y * pen[0])]
         if p.iMask == "Insect":
@@ -1570,6 +1601,7 @@ class Simulation:
                         ("vorabs", np.sqrt(vort[0] ** 2 + vort[1] ** 2 + vort[2] ** 2)),
                         ("mask", self.mask), ("usx", self.us[0]), ("usy", self.us[1]), ("usz", self.us[2])):
             out[f"{nm}_{tag}"] = keyvalues(fld, time, self.p)
+        out[f"p_{tag}"] = keyvalues(self.pressure_field(time), time, self.p)
         self.keys.update(out)
         return out
 

@@ -89,7 +89,7 @@ def test_ensemble_autotune_then_graph(tmp_path):
     res = e.autotune(reps=2)
     assert len(res) == e.nbatches
     for v, ms in res:
-        assert len(v) == 5 and all(x in (0, 1, 3) for x in v) and ms > 0.0
+        assert len(v) == 6 and all(x in (0, 1, 2, 3, 4) for x in v) and ms > 0.0
     e.advance(4, flusi_amd.ENSEMBLE_GRAPH)
     f.advance(4, flusi_amd.ENSEMBLE_MEMBERS)
     for i in range(len(paths)):
@@ -106,7 +106,7 @@ def test_single_context_autotune(tmp_path):
     r = flusi_amd.Flusi(paths[1], flusi_amd.FP32_MIXED)
     r.init_fields()
     v, ms = s.autotune(reps=2)
-    assert len(v) == 5 and ms > 0.0
+    assert len(v) == 6 and ms > 0.0
     v2, ms2 = s.autotune(reps=2)  # cached
     assert v2 == v and ms2 == ms
     for _ in range(2):

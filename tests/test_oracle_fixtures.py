@@ -91,6 +91,25 @@ def test_insects_flow_BB_tethered_RK4(golden):
                 "dt.t"])
 
 
+@pytest.mark.slow
+@pytest.mark.parametrize("case", ["insects_flow_BB_tethered_AB2", "insects_flow_BB_tethered_AB2_periodic"])
+def test_insects_flow_BB_tethered_AB2(golden, case):
+    """The bumblebee fixture with AB2 and CFL 0.1, and its variant with the
+    insect near the corner of the periodic box (parts wrap around):
+    forces, per-part forces, kinematics, energy and dt series plus
+    mask/us/|w| keys (insects_flow_BB_tethered_AB2*.sh:80-100)."""
+    _check_run(os.path.join(golden, case), "insect_test.ini",
+               ["forces.t", "forces_part1.t", "forces_part2.t", "forces_part3.t", "kinematics.t", "energy.t",
+                "dt.t"])
+
+
+def test_vortex_ring4_sponge(golden):
+    """AB2 with the cavity vorticity sponge: keys incl. pressure at t=0, 0.5
+    (vortexring4_sponge.sh:21-23)."""
+    _check_run(os.path.join(golden, "vortex_ring4_sponge"), "vortexring4_sponge.ini", ["dt.t", "energy.t",
+                                                                                     "meanflow.t"])
+
+
 def test_vortex_ring1_AB2(golden):
     _check_run(os.path.join(golden, "vortex_ring1_AB2"), "vortexring1_AB2.ini", ["dt.t", "energy.t", "meanflow.t"])
 
