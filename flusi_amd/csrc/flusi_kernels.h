@@ -155,7 +155,9 @@ int x_pass_blocks(int nx, int nlines, bool fp64);
 template <typename T> void launch_x_rhs_wave(const XPassArgs& a, hipStream_t s);
 bool x_rhs_wave_supported(int nx);
 int x_rhs_wave_blocks(int nx, int nlines);
-int zstep_blocks(int nz, int ny, int nxh, bool fp64);
+// energy partial sums one z-step launch writes per member (the block kernel
+// or, wave = true, the in-wave kernel)
+int zstep_blocks(int nz, int ny, int nxh, bool fp64, bool wave);
 bool fft_length_supported(int n);   // y/z line lengths
 bool x_length_supported(int m);     // nx/2 of the real x axis
 

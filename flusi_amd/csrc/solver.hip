@@ -300,7 +300,8 @@ class Solver : public SolverBase {
       const int nxb = std::max(x_rhs_wave_supported(nx) ? x_rhs_wave_blocks(nx, ny * nz) : 0,
                                x_pass_blocks(nx, ny * nz, FP64));
       xpart = (double*)dalloc(sizeof(double) * 4 * nxb);
-      nzblocks = zstep_blocks(nz, ny, nxh, FP64);
+      // room for either z-step form's partials (the variant can change later)
+      nzblocks = std::max(zstep_blocks(nz, ny, nxh, FP64, false), zstep_blocks(nz, ny, nxh, FP64, true));
       zpart = (double*)dalloc(sizeof(double) * nzblocks);
       ndblocks = diag_blocks<T>(geom);
       dpart = (double*)dalloc(sizeof(double) * DIAG_NV * ndblocks);
@@ -550,7 +551,7 @@ class Solver : public SolverBase {
   // for every member after the step)
   void record_ekin(bool force = false) {
     if (out_dir.empty() || (ens_step && !force)) return;
-    launch_reduce_rows(zpart, nzblocks, 1, kinds + 4, rmid, red + 8, stream);
+    launch_reduce_rows(zpart, zstep_blocks(nz, ny, nxh, FP64, zwave), 1, kinds + 4, rmid, red + 8, stream);
     double e = 0.0;
     HIPCHECK(hipMemcpyAsync(&e, red + 8, sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));

@@ -224,11 +224,34 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
   }
 }
 
+// Lines per workgroup of the block kernel.  FP32 at nz = 512 takes 8 lines
+// (512 threads, 64-byte Q segments) and is held to 128 VGPRs, so that two
+// workgroups share a CU and one's transform stages (LDS work between block
+// barriers) overlap the other's epilogue (HBM reads and writes of the state);
+// a 16-line tile at nz = 512 fills a CU with one workgroup.
+template <typename T> __host__ __device__ constexpr int zb_tl(int L) {
+  return (sizeof(T) == 4 && L >= 512) ? 8 : Tile<T>::TL;
+}
+template <typename T> __host__ __device__ constexpr int zb_wpe(int L) {
+  return (sizeof(T) == 4 && L >= 512) ? 4 : 1;
+}
+
+// LDS of the block kernel: the transform tile, and a second tile where field
+// 0's transformed lines wait for the epilogue when both fit in a CU's 160 KB.
+// Parking them there instead of in registers keeps the kernel within its
+// VGPR budget (without it, the 16-line FP32 kernel at nz = 512 spills 21
+// VGPRs, about 1.3 GB of extra memory traffic per launch at 512^3).
+template <typename T, int L, int TL> struct ZBShape {
+  static constexpr size_t tile = sizeof(cplx<T>) * TL * line_stride<T>(L);
+  static constexpr bool park = 2 * tile <= 160 * 1024;
+  static constexpr size_t smem = park ? 2 * tile : tile;
+};
+
 // EU > 0: the epilogue of at most EU positions is scheduled together (a
 // scheduling barrier after every EU positions keeps the compiler from
-// hoisting all positions' state loads, which at L >= 256 spills registers)
+// hoisting all positions' state loads)
 template <typename T, int L, int TL, int NT, bool SPONGE, int MG, int EU>
-__global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
+__global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
   const MOff mo = member_off<T>(a);
   constexpr int NSTG = plan_count_k(PLAN_TILE, L);
   constexpr int R0 = plan_radix_k(PLAN_TILE, L, 0);
@@ -242,15 +265,24 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   __shared__ double red[NT / 64];
   const int tid = threadIdx.x;
-  const int b0 = blockIdx.x * TL;
+  // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
+  // workgroups x and x + 8 of a row run on one XCD one after the other; they
+  // take neighbouring ky tiles, whose Q segments share 128-byte lines.
+  int bx = blockIdx.x;
+  if (TL * sizeof(cplx<T>) < 128 && gridDim.x % 8 == 0) bx = (bx % 8) * (gridDim.x / 8) + bx / 8;
+  const int b0 = bx * TL;
   const int kx = blockIdx.y;
   const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
   const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
+  constexpr bool PARK = ZBShape<T, L, TL>::park;
+  // park: field 0 by (ky line, kz), written and read back by the same thread
+  cplx<T>* park = buf + TL * LSP;
   // r[f][p][q]: field f at (ky = b0 + bb, kz = j + q L/RL) of last-stage butterfly p
   cplx<T> r[NF][PERL][RL];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    cplx<T> v[PER0][R0];
+  // stage-0 operands of a field; the next field's are in flight while the
+  // current one goes through its LDS stages
+  cplx<T> v[PER0][R0];
+  auto loadq = [&](int f) {
 #pragma unroll
     for (int p = 0; p < PER0; ++p) {
       const int beta = tid + p * NT;
@@ -260,12 +292,24 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
 #pragma unroll
         for (int q = 0; q < R0; ++q)
           v[p][q] = ok ? nq[mo.mc + f * a.fs + ((long)kx * a.nz + j + q * (L / R0)) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
+      }
+    }
+  };
+  loadq(0);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+#pragma unroll
+    for (int p = 0; p < PER0; ++p) {
+      const int beta = tid + p * NT;
+      if (NB0 % NT == 0 || beta < NB0) {
+        const int bb = beta % TL, j = beta / TL;
         Bfly<R0, -1, T>::run(v[p]);
 #pragma unroll
         for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
       }
     }
     __syncthreads();
+    if (f + 1 < NF) loadq(f + 1);
     Stages<T, L, TL, NT, PLAN_TILE, -1, LSP, 1, NSTG - 1, R0>::run(buf, tw);
 #pragma unroll
     for (int p = 0; p < PERL; ++p) {
@@ -275,6 +319,10 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
 #pragma unroll
         for (int q = 0; q < RL; ++q) r[f][p][q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
         twiddle_bfly<T, L, RL, -1, NSL>(r[f][p], j, tw);
+        if (PARK && f == 0) {
+#pragma unroll
+          for (int q = 0; q < RL; ++q) park[bb * LSP + lp<T>(j + q * (L / RL))] = r[f][p][q];
+        }
       }
     }
     __syncthreads();
@@ -297,7 +345,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
     if (ky >= a.ny) continue;
     cplx<T> rv[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) rv[f] = r[f][p][q];
+    for (int f = 0; f < NF; ++f) rv[f] = (PARK && f == 0) ? park[bb * LSP + lp<T>(kzi)] : r[f][p][q];
     zepilogue<T, SPONGE, MG>(a, mo, kx, ky, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
     if constexpr (EU > 0)
       if ((p * RL + q + 1) % EU == 0) __builtin_amdgcn_sched_barrier(0);
@@ -309,7 +357,7 @@ __global__ __launch_bounds__(NT) void zstep_k(ZStepArgs a) {
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w];
-      a.energy_partial[mo.md + (long)blockIdx.y * gridDim.x + blockIdx.x] = s;
+      a.energy_partial[mo.md + (long)blockIdx.y * gridDim.x + bx] = s;
     }
   }
 }
@@ -410,16 +458,19 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
   }
 }
 
-int zstep_blocks(int nz, int ny, int nxh, bool fp64) {
-  const int TL = fp64 ? Tile<double>::TL : Tile<float>::TL;
+static bool zwave_supported(int nz) { return nz == 64 || nz == 128 || nz == 256 || nz == 512; }
+
+int zstep_blocks(int nz, int ny, int nxh, bool fp64, bool wave) {
+  const int TL = (wave && zwave_supported(nz)) ? (fp64 ? Tile<double>::TL : Tile<float>::TL)
+                                               : (fp64 ? zb_tl<double>(nz) : zb_tl<float>(nz));
   return ((ny + TL - 1) / TL) * nxh;
 }
 
 template <typename T, int L, bool SP, int MG, int EU>
 static void run_zb(const ZStepArgs& a, hipStream_t s) {
-  constexpr int TL = Tile<T>::TL;
+  constexpr int TL = zb_tl<T>(L);
   constexpr int NT = pick_nt(TL * L);
-  const size_t smem = sizeof(cplx<T>) * TL * line_stride<T>(L);
+  const size_t smem = ZBShape<T, L, TL>::smem;
   static bool init = false;
   if (!init) { set_smem(zstep_k<T, L, TL, NT, SP, MG, EU>, smem); init = true; }
   dim3 grid((a.ny + TL - 1) / TL, a.nxh, batch_n(a.bat));
