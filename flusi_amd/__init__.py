@@ -55,6 +55,9 @@ EXPORTED: Dict[str, tuple] = {
     "flusi_step": (_I, [_P]),
     "flusi_run": (_I, [_P, _I]),
     "flusi_advance": (_I, [_P, _I]),
+    "flusi_save_fields": (_I, [_P]),
+    "flusi_dump_runtime_backup": (_I, [_P]),
+    "flusi_read_runtime_backup": (_I, [_P, ctypes.c_char_p]),
     "flusi_get_time": (_I, [_P, _D, ctypes.POINTER(_I), _D, _D]),
     "flusi_create_mask": (_I, [_P, ctypes.c_double]),
     "flusi_get_mask": (_I, [_P, _D, _D, ctypes.POINTER(ctypes.c_int16)]),
@@ -102,6 +105,55 @@ def load_library(path: Optional[str] = None):
     if path is None:
         _lib = lib
     return lib
+
+
+H5_LIB_PATH = os.path.join(_HERE, "libflusi_h5.so")
+_h5 = None
+
+
+def _h5lib():
+    global _h5
+    if _h5 is None:
+        if not os.path.exists(H5_LIB_PATH):
+            raise RuntimeError(f"{H5_LIB_PATH} is missing: build it with `python -m flusi_amd.build` (needs HDF5)")
+        lib = ctypes.CDLL(H5_LIB_PATH)
+        lib.fh5_field_dims.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                       ctypes.POINTER(_I), ctypes.c_char_p, _I]
+        lib.fh5_read_field.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _I, _I, _I, _P, ctypes.c_char_p, _I]
+        lib.fh5_read_attr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _P, _I, ctypes.c_char_p, _I]
+        _h5 = lib
+    return _h5
+
+
+def read_h5_field(path: str, dset: Optional[str] = None):
+    """A field file written by save_fields or a runtime backup: returns the
+    (nz, ny, nx) float64 array of dataset `dset` (default: the file name up
+    to its first '_', as flusi_hdf5_interface.f90 get_dsetname)."""
+    lib = _h5lib()
+    if dset is None:
+        dset = os.path.basename(path).split("_")[0].split(".")[0]
+    err = ctypes.create_string_buffer(512)
+    nx, ny, nz = _I(), _I(), _I()
+    rc = lib.fh5_field_dims(os.fsencode(path), dset.encode(), ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz),
+                            err, 512)
+    if rc:
+        raise FlusiError(rc, err.value.decode())
+    out = np.empty((nz.value, ny.value, nx.value))
+    rc = lib.fh5_read_field(os.fsencode(path), dset.encode(), nx.value, ny.value, nz.value, out.ctypes.data, err, 512)
+    if rc:
+        raise FlusiError(rc, err.value.decode())
+    return out
+
+
+def read_h5_attr(path: str, dset: str, name: str, n: int):
+    """n values of the attribute `name` of dataset `dset` (as float64)."""
+    lib = _h5lib()
+    err = ctypes.create_string_buffer(512)
+    out = np.empty(n)
+    rc = lib.fh5_read_attr(os.fsencode(path), dset.encode(), name.encode(), out.ctypes.data, n, err, 512)
+    if rc:
+        raise FlusiError(rc, err.value.decode())
+    return out
 
 
 class FlusiError(RuntimeError):
@@ -228,6 +280,18 @@ class Flusi:
 
     def advance(self, nsteps: int):
         self._check(self._lib.flusi_advance(self._ctx, int(nsteps)))
+
+    def save_fields(self):
+        """save_fields.f90 save_fields_fsi: HDF5 snapshots of the iSave* fields."""
+        self._check(self._lib.flusi_save_fields(self._ctx))
+
+    def dump_runtime_backup(self):
+        """runtime_backuping.f90 dump_runtime_backup (runtime_backup0/1.h5)."""
+        self._check(self._lib.flusi_dump_runtime_backup(self._ctx))
+
+    def read_runtime_backup(self, path: str):
+        """runtime_backuping.f90 read_runtime_backup (state, time, dt0, dt1, n1, it)."""
+        self._check(self._lib.flusi_read_runtime_backup(self._ctx, os.fsencode(path)))
 
     def get_time(self):
         t, i, d0, d1 = ctypes.c_double(), _I(), ctypes.c_double(), ctypes.c_double()

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libflusi_hip.so")
 SOURCES = ["passes.hip", "tiles_wave.hip", "zstep.hip", "xrhs_wave.hip", "fields.hip", "adjoint.hip", "solver.hip",
-           "host_params.cpp"]
+           "host_params.cpp", "h5bind.cc"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FLUSI_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -30,6 +30,12 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if src.endswith(".cc"):  # host-only C++ (dlopen of the HDF5 layer), no device pass
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        return obj
     cmd = [HIPCC, *FLAGS]
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
@@ -48,13 +54,38 @@ def build(force: bool = True, jobs: int = 4) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_h5io()
     build_fortran()
     return LIB
+
+
+HDF5_DIR = os.environ.get("FLUSI_HDF5_DIR", "/opt/conda")
+H5LIB = os.path.join(HERE, "libflusi_h5.so")
+
+
+def build_h5io() -> str:
+    """libflusi_h5.so: the HDF5 field-file layer (plain C against the image's
+    libhdf5), loaded by libflusi_hip.so on the first backup or snapshot.
+    Returns its path, or "" when no HDF5 installation is found (those entry
+    points then report an error)."""
+    inc = os.path.join(HDF5_DIR, "include")
+    lib = os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"warning: no hdf5.h under {inc}: HDF5 snapshots and backups unavailable", file=sys.stderr)
+        return ""
+    tmp = H5LIB + ".tmp"
+    cmd = ["gcc", "-O2", "-std=c99", "-shared", "-fPIC", "-I", inc, os.path.join(CSRC, "h5io.c"), "-o", tmp,
+           "-L", lib, "-lhdf5", f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"HDF5 layer build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, H5LIB)
+    return H5LIB
 
 
 FLANG = os.environ.get("FLUSI_FC", "/opt/rocm/bin/amdflang")

@@ -143,6 +143,22 @@ Params load_params(const IniFile& ini_c) {
   p.tintegral = ini.dbl("Saving", "tintegral", 0.01);
   p.tsave_first = ini.dbl("Saving", "tsave_first", 0.0);
   p.itdrag = ini.integer("Saving", "itdrag", 99999);
+  p.iDoBackup = ini.integer("Saving", "iDoBackup", 1);
+  p.backup_type = ini.str("Saving", "backup_type", "one-file-backup");
+  p.iSaveVelocity = ini.integer("Saving", "iSaveVelocity", 0);
+  p.iSavePress = ini.integer("Saving", "iSavePress", 0);
+  p.iSaveVorticity = ini.integer("Saving", "iSaveVorticity", 0);
+  p.iSaveMagVorticity = ini.integer("Saving", "iSaveMagVorticity", 0);
+  p.iSaveMask = ini.integer("Saving", "iSaveMask", 0);
+  p.iSaveSolidVelocity = ini.integer("Saving", "iSaveSolidVelocity", 0);
+  p.truntime = ini.dbl("Saving", "truntime", 1.0);
+  p.wtimemax = ini.dbl("Saving", "wtimemax", 8760.0);
+  p.tsave_period = ini.dbl("Saving", "tsave_period", 1.0);
+  p.save_only_one_period = ini.str("Saving", "save_only_one_period", "no");
+  p.naming = ini.str("Saving", "naming", "time");
+  p.field_precision = ini.str("Saving", "field_precision", "single");
+  p.striding = ini.integer("Saving", "striding", 1);
+  if (p.striding < 1) throw FlusiError(8881, "Striding value is bad, exit!");
   p.iCavity = ini.str("Penalization", "iCavity", "no");
   p.cavity_size = ini.integer("Penalization", "cavity_size", 0);
   p.compute_forces = ini.integer("Penalization", "compute_forces", 1);

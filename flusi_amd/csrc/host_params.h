@@ -54,6 +54,14 @@ struct Params {
   bool periodic = false;
   double tsave = 9e9, tintegral = 0.01, tsave_first = 0.0;
   int itdrag = 99999;
+  // [Saving] field snapshots and runtime backups (params.f90:166-187, 305)
+  int iDoBackup = 1;
+  std::string backup_type = "one-file-backup";
+  int iSaveVelocity = 0, iSavePress = 0, iSaveVorticity = 0, iSaveMagVorticity = 0, iSaveMask = 0,
+      iSaveSolidVelocity = 0;
+  double truntime = 1.0, wtimemax = 8760.0, tsave_period = 1.0;
+  std::string save_only_one_period = "no", naming = "time", field_precision = "single";
+  int striding = 1;
   std::string iCavity = "no";
   int cavity_size = 0;
   int compute_forces = 1;

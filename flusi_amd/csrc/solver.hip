@@ -5,6 +5,7 @@
 #include "fft_core.h"
 #include "fields.h"
 #include "flusi_kernels.h"
+#include "h5bind.h"
 #include "host_params.h"
 
 #include <hip/hip_runtime.h>
@@ -64,6 +65,12 @@ static double startup_conditioner(double time, double trel, double tau) {
 
 static int fortran_nint(double x) { return x >= 0 ? (int)std::floor(x + 0.5) : -(int)std::floor(-x + 0.5); }
 
+static bool file_exists(const std::string& f) {
+  FILE* h = std::fopen(f.c_str(), "rb");
+  if (h) std::fclose(h);
+  return h != nullptr;
+}
+
 static const char* kTimerNames[FLUSI_TIMER_SLOTS] = {"ZINV", "YINV", "XPASS", "YFWD", "ZSTEP", "MASK", "DIAG", "OTHER"};
 enum { TM_ZINV, TM_YINV, TM_X, TM_YFWD, TM_ZSTEP, TM_MASK, TM_DIAG, TM_OTHER };
 
@@ -105,6 +112,24 @@ class SolverBase {
   virtual void adjoint_get_controls(double* theta) = 0;
   virtual double adjoint_forward(const double* theta, int nsteps, int every) = 0;
   virtual void adjoint_backward(double* grad) = 0;
+  virtual void save_fields() = 0;
+  virtual void dump_runtime_backup() = 0;
+  virtual void read_runtime_backup(const std::string& file) = 0;
+
+  // runtime backups: the file written next (runtime_backup0/1, time_step.f90:15)
+  int nbackup = 0;
+  bool restarted = false;  // the fields came from a runtime backup
+
+  // output files: relative names live in the output directory
+  std::string out_path(const std::string& name) const {
+    if (name.empty() || name[0] == '/' || out_dir.empty()) return name;
+    return out_dir + "/" + name;
+  }
+  bool saves_fields() const {
+    return p.iSaveVelocity == 1 || p.iSavePress == 1 || p.iSaveVorticity == 1 || p.iSaveMagVorticity == 1 ||
+           (p.iSaveMask == 1 && p.iPenalization == 1) ||
+           (p.iSaveSolidVelocity == 1 && p.iPenalization == 1 && p.iMoving == 1);
+  }
 
   // time_step.f90:57-167 loop body
   void step() {
@@ -114,6 +139,33 @@ class SolverBase {
     it += 1;
     if (time_for_output(time, dt1, it, p.tintegral, p.itdrag, p.tmax, 0.0, p.intelligent_dt == "yes"))
       write_integrals(time, nullptr);
+    // time_step.f90:97-105 field snapshots (written only with an output directory)
+    if (!out_dir.empty() && saves_fields() &&
+        time_for_output(time, dt1, it, p.tsave, 99999999, p.tmax, p.tsave_first, p.intelligent_dt == "yes"))
+      save_fields();
+  }
+
+  // time_step.f90:1-180: the loop, with the wall-clock backups every truntime
+  // hours, the wtimemax stop and the final backup
+  void run(int nsteps) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto hours = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 3600.0; };
+    double truntimenext = p.truntime;
+    const bool backups = p.iDoBackup == 1 && !out_dir.empty();
+    // init_fields.f90:108-111 and time_step.f90:47-50: initial snapshot and
+    // integrals, unless resuming a backup
+    if (!restarted) {
+      if (!out_dir.empty() && saves_fields() && time >= p.tsave_first) save_fields();
+      write_integrals(time, nullptr);
+    }
+    int n = 0;
+    while (time < p.tmax && it <= p.nt && (nsteps <= 0 || n < nsteps)) {
+      step();
+      ++n;
+      if (backups && truntimenext < hours()) { dump_runtime_backup(); truntimenext += p.truntime; }
+      if (p.wtimemax < hours()) break;
+    }
+    if (backups) dump_runtime_backup();
   }
 
   void append(const std::string& name, const std::vector<double>& row, const char* header) {
@@ -1367,6 +1419,148 @@ class Solver : public SolverBase {
     copy_real_out(pf(6), out, npts);
   }
 
+  // ------------------------------------------------------------ HDF5 outputs
+  // flusi_hdf5_interface.f90:21-81 save_field_hdf5: one field per file
+  // "<name>.h5", dataset = the name up to its first '_', with the
+  // attributes time, viscosity, epsi, domain_size, origin and nxyz
+  void save_field_h5(const std::string& name, const double* f) {
+    const std::string file = out_path(name + ".h5");
+    const std::string dset = name.substr(0, name.find('_'));
+    h5_write(file, dset, nx, ny, nz, f, p.field_precision != "double", true,
+             {{"time", {time}, 0}, {"viscosity", {p.nu}, 0}, {"epsi", {p.eps}, 0},
+              {"domain_size", {p.xl, p.yl, p.zl}, 0}, {"origin", {0.0, 0.0, 0.0}, 0},
+              {"nxyz", {(double)nx, (double)ny, (double)nz}, 1}});
+  }
+
+  // save_fields.f90:46-237 save_fields_fsi at the current time
+  void save_fields() override {
+    need_fft();
+    if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    if (p.striding != 1) throw FlusiError(8881, "striding > 1 is not supported by this build's field writer");
+    char name[32];
+    if (p.naming == "timestep") std::snprintf(name, sizeof(name), "%06d", it);
+    else if (p.save_only_one_period == "yes")  // save_fields.f90:86 (t minus the period count)
+      std::snprintf(name, sizeof(name), "%06d", fortran_nint((time - std::floor(time / p.tsave_period)) * 1000.0));
+    else std::snprintf(name, sizeof(name), "%06d", fortran_nint(time * 1000.0));
+    const std::string tag(name);
+    if (p.iMoving == 1) create_mask(time);
+    std::vector<double> a(3 * (size_t)npts), b;
+    if (p.iSaveVelocity == 1) {
+      get_u(a.data(), false);
+      save_field_h5("ux_" + tag, a.data());
+      save_field_h5("uy_" + tag, a.data() + npts);
+      save_field_h5("uz_" + tag, a.data() + 2 * npts);
+    }
+    if (p.iSavePress == 1) {
+      // rhs.f90:342-379 and save_fields.f90:127-132: the pressure of the
+      // unprojected right-hand side at t, minus |u|^2/2
+      b.resize(npts);
+      get_pressure(time, b.data());
+      save_field_h5("p_" + tag, b.data());
+    }
+    if (p.iSaveVorticity == 1 || p.iSaveMagVorticity == 1) {
+      get_u(a.data(), true);
+      if (p.iSaveVorticity == 1) {
+        save_field_h5("vorx_" + tag, a.data());
+        save_field_h5("vory_" + tag, a.data() + npts);
+        save_field_h5("vorz_" + tag, a.data() + 2 * npts);
+      }
+      if (p.iSaveMagVorticity == 1) {
+        b.resize(npts);
+        for (long i = 0; i < npts; ++i)
+          b[i] = std::sqrt(a[i] * a[i] + a[npts + i] * a[npts + i] + a[2 * npts + i] * a[2 * npts + i]);
+        save_field_h5("vorabs_" + tag, b.data());
+      }
+    }
+    const bool smask = p.iSaveMask == 1 && p.iPenalization == 1;
+    const bool sus = p.iSaveSolidVelocity == 1 && p.iPenalization == 1 && p.iMoving == 1;
+    if (smask || sus) {
+      b.resize(npts);
+      std::vector<int16_t> col(npts);
+      get_mask(b.data(), a.data(), col.data());
+      if (smask) save_field_h5("mask_" + tag, b.data());
+      if (sus) {
+        save_field_h5("usx_" + tag, a.data());
+        save_field_h5("usy_" + tag, a.data() + npts);
+        save_field_h5("usz_" + tag, a.data() + 2 * npts);
+      }
+    }
+  }
+
+  // runtime_backuping.f90:3-193 (fluid part): u and the registers nlk(0),
+  // nlk(1) in physical space, each dataset with the attribute
+  // bckp = (time, dt1, dt0, n1, it, nx, ny, nz); then nbackup = 1 - nbackup
+  void dump_runtime_backup() override {
+    need_fft();
+    if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    ensure_phys();
+    const bool one = p.backup_type == "one-file-backup";
+    const std::string base = out_path("runtime_backup" + std::to_string(nbackup));
+    const std::vector<double> bckp = {time, dt1, dt0, (double)n1, (double)it, (double)nx, (double)ny, (double)nz};
+    std::vector<double> f(npts);
+    bool first = true;
+    const void* src[3] = {uk, reg[0], reg[1]};
+    const char* names[3][3] = {{"ux", "uy", "uz"}, {"nlkx0", "nlky0", "nlkz0"}, {"nlkx1", "nlky1", "nlkz1"}};
+    for (int g = 0; g < 3; ++g)
+      for (int c = 0; c < 3; ++c) {
+        ifft_fields(cf(const_cast<void*>(src[g]), c), 1, pf(0));
+        copy_real_out(pf(0), f.data(), npts);
+        const std::string file = one ? base + ".h5" : base + "_" + names[g][c] + ".h5";
+        h5_write(file, names[g][c], nx, ny, nz, f.data(), p.field_precision != "double", one ? first : true,
+                 {{"bckp", bckp, 0}});
+        first = false;
+      }
+    nbackup = 1 - nbackup;
+  }
+
+  // runtime_backuping.f90:198-361 (fluid part): the state, time, dt0, dt1,
+  // n1 and it from a backup; the integrating factor is that of dt1
+  void read_runtime_backup(const std::string& given) override {
+    need_fft();
+    ensure_phys();
+    const bool one = p.backup_type == "one-file-backup";
+    std::string base = out_path(given);
+    if (base.size() > 3 && base.compare(base.size() - 3, 3, ".h5") == 0) base.resize(base.size() - 3);
+    auto file_of = [&](const char* dset) { return one ? base + ".h5" : base + "_" + dset + ".h5"; };
+    if (!file_exists(file_of("ux"))) throw FlusiError(81, "backup file not found: " + file_of("ux"));
+    double b[8];
+    h5_attr(file_of("ux"), "ux", "bckp", b, 8);
+    if ((int)b[5] != nx || (int)b[6] != ny || (int)b[7] != nz)
+      throw FlusiError(77776, "ERROR! read_runtime_backup(): Resolution mismatch");
+    std::vector<double> f(npts);
+    void* dst[3] = {uk, reg[0], reg[1]};
+    const char* names[3][3] = {{"ux", "uy", "uz"}, {"nlkx0", "nlky0", "nlkz0"}, {"nlkx1", "nlky1", "nlkz1"}};
+    for (int g = 0; g < 3; ++g)
+      for (int c = 0; c < 3; ++c) {
+        h5_read(file_of(names[g][c]), names[g][c], nx, ny, nz, f.data());
+        for (long i = 0; i < npts; ++i)
+          if (std::isnan(f[i])) throw FlusiError(100011, "NaN in recently read backup file!! (" + std::string(names[g][c]) + ")");
+        copy_real_in(f.data(), pf(0), npts);
+        fft_fields(pf(0), 1, cf(dst[g], c));
+      }
+    time = b[0]; dt1 = b[1]; dt0 = b[2]; n1 = (int)b[3]; it = (int)b[4];
+    n0 = 1 - n1;
+    vis_dt = dt1;  // runtime_backuping.f90:348 cal_vis(dt1, explin)
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
+  // params.f90:199-233 inicond = "backup": the newer of runtime_backup0/1.h5
+  std::string newest_backup() {
+    double best = -9.0e9;
+    std::string pick;
+    const bool one = p.backup_type == "one-file-backup";
+    for (int k = 0; k < 2; ++k) {
+      const std::string nm = "runtime_backup" + std::to_string(k) + ".h5";
+      const std::string probe = one ? out_path(nm) : out_path("runtime_backup" + std::to_string(k) + "_ux.h5");
+      if (!file_exists(probe)) continue;
+      double b[8];
+      h5_attr(probe, "ux", "bckp", b, 8);
+      if (b[0] > best) { best = b[0]; pick = nm; }
+    }
+    if (best < -1.0) throw FlusiError(81, "no backup files found...");
+    return pick;
+  }
+
   void get_u(double* u, bool vort) override {
     need_fft();
     inv_uw_phys(uk);
@@ -1394,6 +1588,7 @@ class Solver : public SolverBase {
     // init_fields_fsi.f90:33-48
     if (!fft_ok && p.inicond != "quiescent") need_fft();
     time = 0.0; dt1 = p.tsave; dt0 = p.tsave; it = 0; vis_dt = -1.0;
+    restarted = false;
     const size_t cs = sizeof(C) * nc;
     if (fft_ok) {
       launch_zero(uk, 3 * cs, stream);
@@ -1416,10 +1611,15 @@ class Solver : public SolverBase {
       need_fft();
       for (int c = 0; c < 3; ++c) launch_set_mode0<T>(geom, uk, c, p.umean[c], 0.0, stream);
     } else if (p.inicond == "quiescent") {
+    } else if (p.inicond == "backup" || p.inicond.rfind("backup::", 0) == 0) {
+      // init_fields_fsi.f90:740-747 (and params.f90:199-233 for "backup")
+      need_fft();
+      read_runtime_backup(p.inicond == "backup" ? newest_backup() : p.inicond.substr(8));
+      restarted = true;
     } else {
-      throw FlusiError(1, "inicond not supported: " + p.inicond);
+      throw FlusiError(55523, "??? ERROR: Invalid initial condition" + p.inicond);
     }
-    n1 = 1; n0 = 0;
+    if (!restarted) { n1 = 1; n0 = 0; }
     if (p.iMask == "Insect" && p.iPenalization == 1) {
       is_insect = true;
       insect = Insect();
@@ -2044,9 +2244,19 @@ int flusi_run(flusi_ctx* ctx, int nsteps) {
   return guarded(ctx, [&] {
     flusi::SolverBase* s = ctx->s.get();
     if (!s->fields_initialized) s->init_fields();
-    s->write_integrals(s->time, nullptr);
-    int n = 0;
-    while (s->time < s->p.tmax && s->it <= s->p.nt && (nsteps <= 0 || n < nsteps)) { s->step(); ++n; }
+    s->run(nsteps);
+  });
+}
+int flusi_save_fields(flusi_ctx* ctx) { return guarded(ctx, [&] { ctx->s->save_fields(); }); }
+int flusi_dump_runtime_backup(flusi_ctx* ctx) { return guarded(ctx, [&] { ctx->s->dump_runtime_backup(); }); }
+int flusi_read_runtime_backup(flusi_ctx* ctx, const char* file) {
+  return guarded(ctx, [&] {
+    if (!file) throw FlusiError(900000, "no backup file given");
+    flusi::SolverBase* s = ctx->s.get();
+    if (!s->fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    s->read_runtime_backup(file);
+    s->restarted = true;
+    if (s->p.iPenalization == 1) s->create_mask(s->time);  // init_fields.f90:98-101
   });
 }
 int flusi_advance(flusi_ctx* ctx, int nsteps) {

@@ -109,6 +109,25 @@ module flusi_hip
       integer(c_int), value :: nsteps
     end function
 
+    ! save_fields.f90:2 save_fields
+    integer(c_int) function flusi_save_fields(ctx) bind(C, name="flusi_save_fields")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    ! runtime_backuping.f90:3 dump_runtime_backup
+    integer(c_int) function flusi_dump_runtime_backup(ctx) bind(C, name="flusi_dump_runtime_backup")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    ! runtime_backuping.f90:198 read_runtime_backup (file: NUL-terminated)
+    integer(c_int) function flusi_read_runtime_backup(ctx, file) bind(C, name="flusi_read_runtime_backup")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*) :: file
+    end function
+
     integer(c_int) function flusi_get_time(ctx, time, it, dt0, dt1) bind(C, name="flusi_get_time")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -68,13 +68,36 @@ int flusi_cal_nlk(flusi_ctx* ctx, double time, int it, double* nlk, double* u);
 int flusi_fluid_timestep(flusi_ctx* ctx, double* time, int* it, double* dt0, double* dt1);
 
 /* One iteration of time_step.f90:57-167: dt0=dt1, FluidTimestep, swap
- * n0/n1, time+=dt1, it+=1, write_integrals on the tintegral/itdrag cadence. */
+ * n0/n1, time+=dt1, it+=1, write_integrals on the tintegral/itdrag cadence,
+ * save_fields on the tsave cadence (time_step.f90:97-105; HDF5 snapshots are
+ * written only when an output directory is set). */
 int flusi_step(flusi_ctx* ctx);
-/* time_step.f90:1-180: initial write_integrals, then steps while
- * time<Tmax and it<=nt; nsteps>0 stops after nsteps iterations. */
+/* time_step.f90:1-180: unless the fields came from a backup, the initial
+ * snapshot (init_fields.f90:108-111) and write_integrals; then steps while
+ * time<Tmax and it<=nt (nsteps>0 stops after nsteps iterations), a runtime
+ * backup every truntime wall-clock hours, the wtimemax stop, and a final
+ * backup (iDoBackup=1 and an output directory set). */
 int flusi_run(flusi_ctx* ctx, int nsteps);
 /* nsteps iterations of flusi_step without the initial write_integrals. */
 int flusi_advance(flusi_ctx* ctx, int nsteps);
+
+/* save_fields.f90:46-237 save_fields_fsi at the current time: the iSave*
+ * fields of the [Saving] section as HDF5 files "<field>_<tag>.h5" in the
+ * output directory (tag: nint(1000 t), or it with naming=timestep), dataset
+ * <field>, field_precision single (default) or double, with the attributes
+ * of flusi_hdf5_interface.f90:58-63.  Needs libflusi_h5.so (HDF5). */
+int flusi_save_fields(flusi_ctx* ctx);
+/* runtime_backuping.f90:3-193 dump_runtime_backup: u and the registers
+ * nlk(0), nlk(1) in physical space to runtime_backup<n>.h5 (n alternates
+ * 0, 1; backup_type one-file-backup) or runtime_backup<n>_<dset>.h5, each
+ * dataset with bckp = (time, dt1, dt0, n1, it, nx, ny, nz). */
+int flusi_dump_runtime_backup(flusi_ctx* ctx);
+/* runtime_backuping.f90:198-361 read_runtime_backup: state, time, dt0, dt1,
+ * n1, it from a backup file (relative names in the output directory), the
+ * integrating factor of dt1, and the mask at the restored time; 77776 on a
+ * resolution mismatch.  inicond=backup::<file> or backup (the newer of
+ * runtime_backup0/1.h5) in the .ini does the same inside flusi_init_fields. */
+int flusi_read_runtime_backup(flusi_ctx* ctx, const char* file);
 int flusi_get_time(const flusi_ctx* ctx, double* time, int* it, double* dt0, double* dt1);
 
 /* mask.f90:2-31 create_mask at `time` (sphere, tethered insect, cavity). */

@@ -1,0 +1,129 @@
+"""HDF5 field snapshots (save_fields.f90) and runtime backup / restart
+(runtime_backuping.f90) through the C ABI, on the GPU.
+
+* The reference's restart test (sphere_restart.sh): run the sphere fixture
+  to T=1 with a final backup, restart it from runtime_backup0.h5 to T=2, and
+  compare the key values of the snapshots it writes (ux uy uz p vorx vory
+  vorz at t=0 and t=2) with the fixture's .ref keys under compare_key.f90.
+* A backup in double precision continues a run: k steps, backup, restore in
+  a fresh context, n-k more steps = n uninterrupted steps.
+* A snapshot holds what the getters return (float32-rounded with the default
+  field_precision=single), with the reference's file name and attributes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import flusi_amd
+from oracle import flusi_oracle as fo
+from tests._util import GOLDEN, rewrite_ini, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _key_of_file(path, p):
+    pre = os.path.basename(path).split("_")[0]
+    f = flusi_amd.read_h5_field(path)
+    t = flusi_amd.read_h5_attr(path, pre, "time", 1)[0]
+    return fo.keyvalues(f, t, p, single=False)
+
+
+def test_sphere_restart_matches_fixture(tmp_path):
+    d = os.path.join(GOLDEN, "sphere")
+    run = str(tmp_path / "run")
+    src = os.path.join(d, "testing_sphere.ini")
+    # sphere_restart.sh: testing_sphere_start.ini = Tmax 1.0, backups on
+    ini1 = rewrite_ini(src, run, "start.ini", Time__Tmax="1.0", Saving__iDoBackup="1")
+    with flusi_amd.Flusi(ini1, flusi_amd.FP64, output_dir=run) as s:
+        s.init_fields()
+        s.run()
+        t1, it1, _, dt1 = s.get_time()
+    assert abs(t1 - 1.0) < 1e-12
+    assert os.path.exists(os.path.join(run, "runtime_backup0.h5"))
+    bckp = flusi_amd.read_h5_attr(os.path.join(run, "runtime_backup0.h5"), "ux", "bckp", 8)
+    assert bckp[0] == t1 and bckp[4] == it1 and bckp[1] == dt1 and list(bckp[5:]) == [32, 32, 32]
+    # testing_sphere_restart.ini = Tmax 2.0, inicond=backup::runtime_backup0.h5
+    ini2 = rewrite_ini(src, run, "restart.ini", Time__Tmax="2.0", Saving__iDoBackup="1",
+                       InitialCondition__inicond="backup::runtime_backup0.h5")
+    with flusi_amd.Flusi(ini2, flusi_amd.FP64, output_dir=run) as s:
+        s.init_fields()
+        t, it, _, _ = s.get_time()
+        assert t == t1 and it == it1
+        s.run()
+        assert abs(s.get_time()[0] - 2.0) < 1e-12
+    p, _ = fo.get_params(src)
+    n = 0
+    for pre in ("ux", "uy", "uz", "p", "vorx", "vory", "vorz"):
+        for tag in ("000000", "002000"):
+            f = os.path.join(run, f"{pre}_{tag}.h5")
+            assert os.path.exists(f), f
+            ok, errs = fo.compare_key(_key_of_file(f, p), fo.read_key(os.path.join(d, f"{pre}_{tag}.ref")))
+            assert ok, (pre, tag, errs)
+            n += 1
+    assert n == 14
+    # nbackup starts at 0 in every run (time_step.f90:15): the restarted run's
+    # final backup replaced runtime_backup0.h5 with its own end state
+    b2 = flusi_amd.read_h5_attr(os.path.join(run, "runtime_backup0.h5"), "ux", "bckp", 8)
+    assert abs(b2[0] - 2.0) < 1e-12 and b2[4] > it1
+    assert not os.path.exists(os.path.join(run, "runtime_backup1.h5"))
+
+
+@pytest.mark.parametrize("case,ini,over", [
+    ("AB2", ("vortex_ring1_AB2", "vortexring1_AB2.ini"), {}),
+    ("RK4", ("insects_flow_BB_tethered_RK4", "insect_test.ini"), {"Resolution__nx": "32", "Resolution__ny": "32",
+                                                                   "Resolution__nz": "32"}),
+])
+def test_double_backup_continues_the_run(tmp_path, case, ini, over):
+    src = os.path.join(GOLDEN, *ini)
+    out = str(tmp_path / "o")
+    path = rewrite_ini(src, out, "case.ini", Saving__field_precision="double", Saving__iDoBackup="0", **over)
+    n, k = 12, 5
+    with flusi_amd.Flusi(path, flusi_amd.FP64) as s:
+        s.init_fields()
+        s.advance(n)
+        ref_uk, ref_t = s.get_uk(), s.get_time()
+    with flusi_amd.Flusi(path, flusi_amd.FP64, output_dir=out) as s:
+        s.init_fields()
+        s.advance(k)
+        s.dump_runtime_backup()
+    assert os.path.exists(os.path.join(out, "runtime_backup0.h5"))
+    with flusi_amd.Flusi(path, flusi_amd.FP64, output_dir=out) as s:
+        s.init_fields()
+        s.read_runtime_backup("runtime_backup0.h5")
+        assert s.get_time()[1] == k
+        s.advance(n - k)
+        uk, t = s.get_uk(), s.get_time()
+    assert t[1] == ref_t[1] and abs(t[0] - ref_t[0]) < 1e-14
+    err = rel_l2(uk, ref_uk)
+    print(f"{case}: restart after {k} of {n} steps, rel L2 vs uninterrupted = {err:.2e}", flush=True)
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("prec", [flusi_amd.FP64, flusi_amd.FP32_MIXED])
+def test_snapshot_files_hold_the_fields(tmp_path, prec):
+    src = os.path.join(GOLDEN, "insects_flow_BB_tethered_RK4", "insect_test.ini")
+    out = str(tmp_path / "snap")
+    path = rewrite_ini(src, out, "case.ini", Resolution__nx="32", Resolution__ny="32", Resolution__nz="32",
+                       Saving__iSaveVelocity="1", Saving__iSavePress="1", Saving__iSaveVorticity="1",
+                       Saving__iSaveMagVorticity="1", Saving__iSaveMask="1", Saving__iSaveSolidVelocity="1",
+                       Saving__iDoBackup="0")
+    with flusi_amd.Flusi(path, prec, output_dir=out) as s:
+        s.init_fields()
+        s.advance(3)
+        t = s.get_time()[0]
+        s.save_fields()
+        u, w, pr = s.get_u(), s.get_vorticity(), s.get_pressure(t)
+        mask, us, _ = s.get_mask()
+    tag = "%06d" % int(np.floor(t * 1000.0 + 0.5))
+    f32 = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)  # noqa: E731
+    expect = {"ux": u[0], "uy": u[1], "uz": u[2], "vorx": w[0], "vory": w[1], "vorz": w[2],
+              "vorabs": np.sqrt(w[0] ** 2 + w[1] ** 2 + w[2] ** 2), "p": pr, "mask": mask,
+              "usx": us[0], "usy": us[1], "usz": us[2]}
+    for name, ref in expect.items():
+        fpath = os.path.join(out, f"{name}_{tag}.h5")
+        got = flusi_amd.read_h5_field(fpath)
+        assert got.shape == (32, 32, 32)
+        np.testing.assert_allclose(got, f32(ref), rtol=0, atol=1e-6 * max(1.0, np.abs(ref).max()), err_msg=name)
+        assert flusi_amd.read_h5_attr(fpath, name, "time", 1)[0] == t
+        np.testing.assert_array_equal(flusi_amd.read_h5_attr(fpath, name, "nxyz", 3), [32, 32, 32])
