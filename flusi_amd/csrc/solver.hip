@@ -411,7 +411,9 @@ class Solver : public SolverBase {
   // kernel variants (chosen by autotune(), or by the FLUSI_* environment overrides)
   bool twave = true;        // y/z passes with the line transform inside one wavefront
   bool tpf = true;          // block y/z passes: prefetch the next job's operand
-  int xwpe = 3;             // one-wave-per-line x pass: waves-per-SIMD register budget (3 or 0)
+  // one-wave-per-line x pass: waves-per-SIMD register budget (3 or 0); FP64
+  // keeps the unconstrained allocation (at 3 it would spill ~480 B per lane)
+  int xwpe = FP64 ? 0 : 3;
   void tile(const TilePassArgs& a0, int L, int dir, bool contig_in) {
     TilePassArgs a = a0;
     a.pf = tpf ? 1 : 0;

@@ -242,9 +242,21 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
         }
         umax2 = fmax(umax2, ux * ux + uy * uy + uz * uz);
       }
-      nv[0][h] = uy * wz - uz * wy - pv[0];
-      nv[1][h] = uz * wx - ux * wz - pv[1];
-      nv[2][h] = ux * wy - uy * wx - pv[2];
+      if constexpr (sizeof(T) == 4) {
+        // FP32-mixed: u x w in FP32, the precision of the transforms that
+        // produced u and w; the penalisation term stays FP64 (chi = mask/eta
+        // is large at small eta).  Keeps the kernel at 3 waves per SIMD
+        // without spilling.
+        const float fx = h ? X[0][q].y : X[0][q].x, fy = h ? X[1][q].y : X[1][q].x, fz = h ? X[2][q].y : X[2][q].x;
+        const float gx = h ? X[3][q].y : X[3][q].x, gy = h ? X[4][q].y : X[4][q].x, gz = h ? X[5][q].y : X[5][q].x;
+        nv[0][h] = (double)(fy * gz - fz * gy) - pv[0];
+        nv[1][h] = (double)(fz * gx - fx * gz) - pv[1];
+        nv[2][h] = (double)(fx * gy - fy * gx) - pv[2];
+      } else {
+        nv[0][h] = uy * wz - uz * wy - pv[0];
+        nv[1][h] = uz * wx - ux * wz - pv[1];
+        nv[2][h] = ux * wy - uy * wx - pv[2];
+      }
       if constexpr (SPONGE) {
         const int ix = 2 * n + h, th = a.sponge_thickness;
         bool in_sp = false;
@@ -324,7 +336,8 @@ void run_wave(const XPassArgs& a, hipStream_t s) {
     if (a.want_stats) hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, true, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
     else hipLaunchKernelGGL((x_rhs_wave_k<T, M, true, false, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
   } else if (a.want_stats) {
-    hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, true, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
+    if (a.wpe == 3) hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, true, 3>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
+    else hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, true, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
   } else {
     if (a.wpe == 3) hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false, 3>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
     else hipLaunchKernelGGL((x_rhs_wave_k<T, M, false, false, 0>), dim3(blocks, batch_n(a.bat)), dim3(XW_NT), 0, s, a);
