@@ -46,10 +46,18 @@ template <typename T> __device__ __forceinline__ MOff member_off(const ZStepArgs
 // (rhs.f90:575-615), dealiasing and the stepper update
 // (fluid_time_step.f90:519-832), the zero-mode overwrite and the Parseval
 // partial sum (integrals.f90:522-552).
+//
+// Arithmetic type R = T: the FP64 path computes in FP64 throughout; the
+// FP32-mixed path computes the pointwise terms in FP32, the precision of its
+// stored fields and transforms.  In both, the dealiasing predicate is the
+// reference's FP64 comparison (so that the same modes are removed), the
+// integrating factors exp(-nu k^2 dt) are evaluated in FP64, and the
+// Parseval sum is accumulated in compensated FP64.
 template <typename T, bool SPONGE, int MG>
 __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, int kx, int ky, int kzi, const cplx<T>* rv,
-                                          double kxv, double kxt2, double wE,
+                                          double kxv_d, double kxt2, double wE,
                                           bool final_stage, double& esum, double& ecomp) {
+  using R = T;
   cplx<T>* U = reinterpret_cast<cplx<T>*>(a.u);
   cplx<T>* UST = reinterpret_cast<cplx<T>*>(a.ustage);
   const cplx<T>* UD = reinterpret_cast<const cplx<T>*>(a.udiff);
@@ -58,21 +66,23 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
   const cplx<T>* RG1 = reinterpret_cast<const cplx<T>*>(a.r1);
   cplx<T>* RO = reinterpret_cast<cplx<T>*>(a.rhs_out);
   const long si = ((long)kx * a.ny + ky) * a.nz + kzi + mo.mc;
-  const double kyv = a.ky[ky + mo.md], kzv = a.kz[kzi + mo.md];
-  const double k2 = (kxv * kxv + kyv * kyv) + kzv * kzv;
+  const double kyv_d = a.ky[ky + mo.md], kzv_d = a.kz[kzi + mo.md];
+  const double k2d = (kxv_d * kxv_d + kyv_d * kyv_d) + kzv_d * kzv_d;  // FP64: integrating factors
+  const R kxv = (R)kxv_d, kyv = (R)kyv_d, kzv = (R)kzv_d, k2 = (R)k2d;
   const bool alias = ((kxt2 + a.kyt2[ky + mo.md]) + a.kzt2[kzi + mo.md]) >= 1.0;
-  double nr[3], ni[3];
+  const R scale = (R)a.scale, nu = (R)a.nu, dt = (R)a.dt;
+  R nr[3], ni[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) { nr[c] = (double)rv[c].x * a.scale; ni[c] = (double)rv[c].y * a.scale; }
+  for (int c = 0; c < 3; ++c) { nr[c] = (R)rv[c].x * scale; ni[c] = (R)rv[c].y * scale; }
   if constexpr (SPONGE) {
     // sponge.f90:22-83: i k x (w/k^2), zero at k = 0
-    if (k2 != 0.0) {
-      double sr[3], sm[3];
+    if (k2d != 0.0) {
+      R sr[3], sm[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) { sr[c] = (double)rv[3 + c].x * a.scale / k2; sm[c] = (double)rv[3 + c].y * a.scale / k2; }
-      const double cr0 = kyv * sr[2] - kzv * sr[1], ci0 = kyv * sm[2] - kzv * sm[1];
-      const double cr1 = kzv * sr[0] - kxv * sr[2], ci1 = kzv * sm[0] - kxv * sm[2];
-      const double cr2 = kxv * sr[1] - kyv * sr[0], ci2 = kxv * sm[1] - kyv * sm[0];
+      for (int c = 0; c < 3; ++c) { sr[c] = (R)rv[3 + c].x * scale / k2; sm[c] = (R)rv[3 + c].y * scale / k2; }
+      const R cr0 = kyv * sr[2] - kzv * sr[1], ci0 = kyv * sm[2] - kzv * sm[1];
+      const R cr1 = kzv * sr[0] - kxv * sr[2], ci1 = kzv * sm[0] - kxv * sm[2];
+      const R cr2 = kxv * sr[1] - kyv * sr[0], ci2 = kxv * sm[1] - kyv * sm[0];
       nr[0] += -ci0; ni[0] += cr0;
       nr[1] += -ci1; ni[1] += cr1;
       nr[2] += -ci2; ni[2] += cr2;
@@ -81,24 +91,24 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
   if (kx == 0 && ky == 0 && kzi == 0) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      if (a.meanflow_dynamic[c]) { nr[c] = -a.pen_sum[c + mo.md] * a.dynamic_scale; ni[c] = 0.0; }
+      if (a.meanflow_dynamic[c]) { nr[c] = (R)(-a.pen_sum[c + mo.md] * a.dynamic_scale); ni[c] = R(0); }
   }
   if (a.explicit_diffusion) {
     // rhs.f90:441-468
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const cplx<T> us_ = UD[c * a.fs + si];
-      nr[c] -= a.nu * k2 * (double)us_.x;
-      ni[c] -= a.nu * k2 * (double)us_.y;
+      nr[c] -= nu * k2 * (R)us_.x;
+      ni[c] -= nu * k2 * (R)us_.y;
     }
   }
   if constexpr (MG == MG_RHS) {
     if (a.mode == SM_PRESSURE) {
       // rhs.f90:342-379 pressure: -i (k . nlk) / k^2, zero at k = 0
-      double pr = 0.0, pi = 0.0;
-      if (k2 != 0.0) {
-        const double dr = (kxv * nr[0] + kyv * nr[1]) + kzv * nr[2];
-        const double di = (kxv * ni[0] + kyv * ni[1]) + kzv * ni[2];
+      R pr = R(0), pi = R(0);
+      if (k2d != 0.0) {
+        const R dr = (kxv * nr[0] + kyv * nr[1]) + kzv * nr[2];
+        const R di = (kxv * ni[0] + kyv * ni[1]) + kzv * ni[2];
         pr = di / k2;
         pi = -dr / k2;
       }
@@ -106,15 +116,15 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       return;
     }
   }
-  if (k2 != 0.0) {
+  if (k2d != 0.0) {
     // rhs.f90:575-615
-    const double qr = (kxv * nr[0] + kyv * nr[1] + kzv * nr[2]) / k2;
-    const double qi = (kxv * ni[0] + kyv * ni[1] + kzv * ni[2]) / k2;
+    const R qr = (kxv * nr[0] + kyv * nr[1] + kzv * nr[2]) / k2;
+    const R qi = (kxv * ni[0] + kyv * ni[1] + kzv * ni[2]) / k2;
     nr[0] -= kxv * qr; ni[0] -= kxv * qi;
     nr[1] -= kyv * qr; ni[1] -= kyv * qi;
     nr[2] -= kzv * qr; ni[2] -= kzv * qi;
   }
-  double unr[3] = {0, 0, 0}, uni[3] = {0, 0, 0};
+  R unr[3] = {0, 0, 0}, uni[3] = {0, 0, 0};
   const int mode = MG == MG_RHS ? SM_RHS_ONLY : (MG == MG_RK4 && a.mode < SM_RK4_1 ? SM_RK4_1 : a.mode);
   switch (mode) {
     case SM_RHS_ONLY:
@@ -125,12 +135,12 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       break;
     case SM_EULER: if constexpr (MG == MG_AB2) {
       // fluid_time_step.f90:718-727
-      const double E = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
+      const R E = (R)vis_factor(a.vis_dt_new, a.nu, k2d, a.dealias_on && alias);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const cplx<T> u0 = U[c * a.fs + si];
-        unr[c] = ((double)u0.x + a.dt * nr[c]) * E;
-        uni[c] = ((double)u0.y + a.dt * ni[c]) * E;
+        unr[c] = ((R)u0.x + dt * nr[c]) * E;
+        uni[c] = ((R)u0.y + dt * ni[c]) * E;
         U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
         RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
       }
@@ -138,13 +148,14 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
     }
     case SM_AB2: if constexpr (MG == MG_AB2) {
       // fluid_time_step.f90:802-811
-      const double E = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
+      const R E = (R)vis_factor(a.vis_dt_new, a.nu, k2d, a.dealias_on && alias);
+      const R b10 = (R)a.b10, b11 = (R)a.b11;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const cplx<T> u0 = U[c * a.fs + si];
         const cplx<T> n1 = RG1[c * a.fs + si];
-        unr[c] = ((double)u0.x + a.b10 * nr[c] + a.b11 * (double)n1.x) * E;
-        uni[c] = ((double)u0.y + a.b10 * ni[c] + a.b11 * (double)n1.y) * E;
+        unr[c] = ((R)u0.x + b10 * nr[c] + b11 * (R)n1.x) * E;
+        uni[c] = ((R)u0.y + b10 * ni[c] + b11 * (R)n1.y) * E;
         U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
         RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
       }
@@ -152,14 +163,14 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
     }
     case SM_RK2_1: if constexpr (MG == MG_RK2) {
       // fluid_time_step.f90:554-575 (N0 weighted with the IF of the previous dt)
-      const double Eo = vis_factor(a.vis_dt_old, a.nu, k2, a.dealias_on && alias);
-      const double En = vis_factor(a.vis_dt_new, a.nu, k2, a.dealias_on && alias);
+      const R Eo = (R)vis_factor(a.vis_dt_old, a.nu, k2d, a.dealias_on && alias);
+      const R En = (R)vis_factor(a.vis_dt_new, a.nu, k2d, a.dealias_on && alias);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double n0r = nr[c] * Eo, n0i = ni[c] * Eo;
+        const R n0r = nr[c] * Eo, n0i = ni[c] * Eo;
         const cplx<T> u0 = U[c * a.fs + si];
         RG0[c * a.fs + si] = mk<T>((T)n0r, (T)n0i);
-        U[c * a.fs + si] = mk<T>((T)((double)u0.x * En + a.dt * n0r), (T)((double)u0.y * En + a.dt * n0i));
+        U[c * a.fs + si] = mk<T>((T)((R)u0.x * En + dt * n0r), (T)((R)u0.y * En + dt * n0i));
       }
       break;
     }
@@ -169,31 +180,32 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       for (int c = 0; c < 3; ++c) {
         const cplx<T> u0 = U[c * a.fs + si];
         const cplx<T> n0 = RG0[c * a.fs + si];
-        unr[c] = (double)u0.x + 0.5 * a.dt * (-(double)n0.x + nr[c]);
-        uni[c] = (double)u0.y + 0.5 * a.dt * (-(double)n0.y + ni[c]);
+        unr[c] = (R)u0.x + R(0.5) * dt * (-(R)n0.x + nr[c]);
+        uni[c] = (R)u0.y + R(0.5) * dt * (-(R)n0.y + ni[c]);
       }
       break;
     }
     default: if constexpr (MG == MG_RK4) {
       // RK4 (fluid_time_step.f90:636-669) with an accumulator register
-      const double kill = alias ? 0.0 : 1.0;
+      const R kill = alias ? R(0) : R(1);
+      const R dt6 = (R)(a.dt / 6.0), dt3 = (R)(a.dt / 3.0);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double kr = nr[c] * kill, ki = ni[c] * kill;
+        const R kr = nr[c] * kill, ki = ni[c] * kill;
         if (a.mode == SM_RK4_1) {
           const cplx<T> u0 = U[c * a.fs + si];
-          ACC[c * a.fs + si] = mk<T>((T)((double)u0.x + (a.dt / 6.0) * kr), (T)((double)u0.y + (a.dt / 6.0) * ki));
-          UST[c * a.fs + si] = mk<T>((T)((double)u0.x + 0.5 * a.dt * kr), (T)((double)u0.y + 0.5 * a.dt * ki));
+          ACC[c * a.fs + si] = mk<T>((T)((R)u0.x + dt6 * kr), (T)((R)u0.y + dt6 * ki));
+          UST[c * a.fs + si] = mk<T>((T)((R)u0.x + R(0.5) * dt * kr), (T)((R)u0.y + R(0.5) * dt * ki));
         } else if (a.mode == SM_RK4_2 || a.mode == SM_RK4_3) {
           const cplx<T> u0 = U[c * a.fs + si];
           const cplx<T> ac = ACC[c * a.fs + si];
-          const double cst = (a.mode == SM_RK4_2) ? 0.5 * a.dt : a.dt;
-          ACC[c * a.fs + si] = mk<T>((T)((double)ac.x + (a.dt / 3.0) * kr), (T)((double)ac.y + (a.dt / 3.0) * ki));
-          UST[c * a.fs + si] = mk<T>((T)((double)u0.x + cst * kr), (T)((double)u0.y + cst * ki));
+          const R cst = (a.mode == SM_RK4_2) ? R(0.5) * dt : dt;
+          ACC[c * a.fs + si] = mk<T>((T)((R)ac.x + dt3 * kr), (T)((R)ac.y + dt3 * ki));
+          UST[c * a.fs + si] = mk<T>((T)((R)u0.x + cst * kr), (T)((R)u0.y + cst * ki));
         } else {
           const cplx<T> ac = ACC[c * a.fs + si];
-          unr[c] = ((double)ac.x + (a.dt / 6.0) * kr) * kill;
-          uni[c] = ((double)ac.y + (a.dt / 6.0) * ki) * kill;
+          unr[c] = ((R)ac.x + dt6 * kr) * kill;
+          uni[c] = ((R)ac.y + dt6 * ki) * kill;
         }
       }
       break;
@@ -204,7 +216,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       // fluid_time_step.f90:1066-1090 set_mean_flow
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        if (a.set_mean[c]) { unr[c] = a.mean_value[c]; uni[c] = 0.0; }
+        if (a.set_mean[c]) { unr[c] = (R)a.mean_value[c]; uni[c] = R(0); }
     }
     if (a.mode == SM_RK2_2 || a.mode == SM_RK4_4 || (kx == 0 && ky == 0 && kzi == 0)) {
 #pragma unroll
