@@ -55,6 +55,11 @@ struct InsectArgs {
   // and no wing point of the previous call lies there (the whole grid on a
   // full reset or for periodic insects).
   int reg_lo[3], reg_hi[3];
+  // points that can change: [clip_lo, clip_hi] per axis (the region above
+  // when it is the whole grid only because the launch is batched or captured
+  // with one fixed extent); clip = 0: every visited point
+  int clip;
+  int clip_lo[3], clip_hi[3];
 };
 
 // capsule record: segment end points (global, not periodized), radius, and

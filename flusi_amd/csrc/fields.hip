@@ -124,6 +124,9 @@ __global__ void insect_k(GridGeom g, const InsectArgs* __restrict__ ap, T* mask,
   const int ix = a.reg_lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = a.reg_lo[1] + blockIdx.y, iz = a.reg_lo[2] + (blockIdx.z - mem * ez);
   if (ix > a.reg_hi[0]) return;
+  if (a.clip && (iy < a.clip_lo[1] || iy > a.clip_hi[1] || iz < a.clip_lo[2] || iz > a.clip_hi[2] ||
+                 ix < a.clip_lo[0] || ix > a.clip_hi[0]))
+    return;  // neither drawn now nor holding a wing point of the previous update
   const long p = ((long)iz * g.ny + iy) * g.nx + ix;
   const uint8_t c_old = color[p];
   const bool old_wing = (c_old == 2 || c_old == 3);

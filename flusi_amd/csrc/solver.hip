@@ -1292,8 +1292,11 @@ class Solver : public SolverBase {
     // region visited: the parts drawn now and the wings of the previous call
     // (the only places where wing colours can be left); everything on the
     // first call and for periodic insects
+    // (a batched or captured launch visits the whole grid and skips the
+    // points outside that region, a.clip)
     const int nn[3] = {nx, ny, nz};
-    if (!a.use_box || a.full_reset || full || scratch) {
+    a.clip = 0;
+    if (!a.use_box || a.full_reset || scratch) {
       for (int i = 0; i < 3; ++i) { a.reg_lo[i] = 0; a.reg_hi[i] = nn[i] - 1; }
     } else {
       for (int i = 0; i < 3; ++i) { a.reg_lo[i] = nn[i]; a.reg_hi[i] = -1; }
@@ -1308,6 +1311,13 @@ class Solver : public SolverBase {
       for (int w = 0; w < 2; ++w) {
         if (a.draw_wing[w]) include(a.box_lo[1 + w], a.box_hi[1 + w]);
         if (prev_wing_valid[w]) include(prev_wing_lo[w], prev_wing_hi[w]);
+      }
+      if (full) {
+        a.clip = 1;
+        for (int i = 0; i < 3; ++i) {
+          a.clip_lo[i] = a.reg_lo[i]; a.clip_hi[i] = a.reg_hi[i];  // empty when lo > hi: every point returns
+          a.reg_lo[i] = 0; a.reg_hi[i] = nn[i] - 1;
+        }
       }
     }
     if (!scratch)
