@@ -24,13 +24,16 @@ template <typename T> void launch_axpby(long n, double a, const void* x, double 
 // in place phi -> w x phi - chi/eta phi; aw = phi x u (3 physical fields each)
 template <typename T> void launch_adj_point(long npts, double eps_inv, void* phi, const void* u, const void* w,
                                             const void* mask, void* aw, hipStream_t s);
+// index box [lo, hi] of an nx x ny x nz grid (x fastest)
+struct GridBox { int lo[3], hi[3]; int nx, ny; };
 // per-block partial sums of f . ((mP - mM)(u - us) - mask (usP - usM)), f = phi (3
-// fields) or e_z when phi is null, into partial[block * nv] for
+// fields) or e_z when phi is null, over the points of box (outside it the
+// perturbed drawings agree), into partial[block * nv] for
 // adjoint_reduce_blocks(npts) blocks.  The perturbed masks mP, mM and us
 // fields usP, usM are FP64 in both precisions.
-template <typename T> void launch_theta(long npts, const void* phi, const void* u, const void* us, const void* mask,
-                                        const double* mP, const double* usP, const double* mM, const double* usM,
-                                        double* partial, int nv, hipStream_t s);
+template <typename T> void launch_theta(long npts, const GridBox& box, const void* phi, const void* u, const void* us,
+                                        const void* mask, const double* mP, const double* usP, const double* mM,
+                                        const double* usM, double* partial, int nv, hipStream_t s);
 // per-block partial sums of mask (u_z - us_z)
 template <typename T> void launch_lift(long npts, const void* u, const void* us, const void* mask, double* partial,
                                        hipStream_t s);

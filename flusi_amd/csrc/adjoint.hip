@@ -136,13 +136,16 @@ __device__ __forceinline__ void block_partial(double v, double* partial, int nv 
 
 // sum over points of f . (dm (u - us) - m dus), f = phi or e_z
 template <typename T>
-__global__ __launch_bounds__(RED_NT) void theta_k(long npts, const T* __restrict__ phi, const T* __restrict__ u,
-                                                  const T* __restrict__ us, const T* __restrict__ mask,
-                                                  const double* __restrict__ mP, const double* __restrict__ usP,
-                                                  const double* __restrict__ mM, const double* __restrict__ usM,
-                                                  double* __restrict__ partial, int nv) {
+__global__ __launch_bounds__(RED_NT) void theta_k(long npts, GridBox box, const T* __restrict__ phi,
+                                                  const T* __restrict__ u, const T* __restrict__ us,
+                                                  const T* __restrict__ mask, const double* __restrict__ mP,
+                                                  const double* __restrict__ usP, const double* __restrict__ mM,
+                                                  const double* __restrict__ usM, double* __restrict__ partial, int nv) {
   double acc = 0.0;
-  for (long p = (long)blockIdx.x * RED_NT + threadIdx.x; p < npts; p += (long)gridDim.x * RED_NT) {
+  const long ex = box.hi[0] - box.lo[0] + 1, ey = box.hi[1] - box.lo[1] + 1, ez = box.hi[2] - box.lo[2] + 1;
+  const long nbox = (ex > 0 && ey > 0 && ez > 0) ? ex * ey * ez : 0;
+  for (long q = (long)blockIdx.x * RED_NT + threadIdx.x; q < nbox; q += (long)gridDim.x * RED_NT) {
+    const long p = ((box.lo[2] + q / (ex * ey)) * box.ny + box.lo[1] + (q / ex) % ey) * box.nx + box.lo[0] + q % ex;
     const double dm = mP[p] - mM[p];
     const double m = mask[p];
     if (dm == 0.0 && m == 0.0) continue;
@@ -209,9 +212,15 @@ void launch_adj_point(long npts, double eps_inv, void* phi, const void* u, const
                      (const T*)w, (const T*)mask, (T*)aw);
 }
 template <typename T>
-void launch_theta(long npts, const void* phi, const void* u, const void* us, const void* mask, const double* mP,
-                  const double* usP, const double* mM, const double* usM, double* partial, int nv, hipStream_t s) {
-  hipLaunchKernelGGL(theta_k<T>, dim3(adjoint_reduce_blocks(npts)), dim3(RED_NT), 0, s, npts, (const T*)phi,
+void launch_theta(long npts, const GridBox& box, const void* phi, const void* u, const void* us, const void* mask,
+                  const double* mP, const double* usP, const double* mM, const double* usM, double* partial, int nv,
+                  hipStream_t s) {
+  const int nz = (int)(npts / ((long)box.nx * box.ny));
+  for (int i = 0; i < 3; ++i) {
+    const int n = i == 0 ? box.nx : (i == 1 ? box.ny : nz);
+    if (box.lo[i] < 0 || box.hi[i] >= n) throw std::runtime_error("theta terms: box outside the grid");
+  }
+  hipLaunchKernelGGL(theta_k<T>, dim3(adjoint_reduce_blocks(npts)), dim3(RED_NT), 0, s, npts, box, (const T*)phi,
                      (const T*)u, (const T*)us, (const T*)mask, mP, usP, mM, usM, partial, nv);
 }
 template <typename T>
@@ -230,8 +239,9 @@ void launch_ez_mask(long npts, double scale, const void* mask, void* out, hipStr
   template void launch_axpby<T>(long, double, const void*, double, const void*, void*, hipStream_t);              \
   template void launch_adj_point<T>(long, double, void*, const void*, const void*, const void*, void*,            \
                                     hipStream_t);                                                                 \
-  template void launch_theta<T>(long, const void*, const void*, const void*, const void*, const double*,          \
-                                const double*, const double*, const double*, double*, int, hipStream_t);          \
+  template void launch_theta<T>(long, const GridBox&, const void*, const void*, const void*, const void*,       \
+                                const double*, const double*, const double*, const double*, double*, int,         \
+                                hipStream_t);                                                                     \
   template void launch_lift<T>(long, const void*, const void*, const void*, double*, hipStream_t);                \
   template void launch_ez_mask<T>(long, double, const void*, void*, hipStream_t);
 FLUSI_ADJ_INST(float)

@@ -805,6 +805,7 @@ class Solver : public SolverBase {
     void* ph = nullptr;             // physical: phi (3), a_w (3)
     double *mP = nullptr, *usP = nullptr, *mM = nullptr, *usM = nullptr;
     uint8_t* colS = nullptr;
+    GridBox box{};                  // where the last pair of perturbed drawings can differ
     double* part = nullptr;         // reduction partials (blocks x controls)
     double* gstage = nullptr;       // one call's per-control sums
     double* gacc = nullptr;         // the gradient, accumulated on the device
@@ -928,13 +929,45 @@ class Solver : public SolverBase {
     adj.have_forward = true;
     return J;
   }
-  // central-difference drawings of the insect at t with control i moved by +-h
+  // central-difference drawings of the insect at t with control i moved by
+  // +-h.  The controls are wing kinematics coefficients, so the two drawings
+  // differ only inside the wing boxes of the two perturbed states: both are
+  // drawn over the union of those boxes (all parts are evaluated there), and
+  // the theta terms sum over it (adj.box).  Periodic insects, whose parts may
+  // wrap, use the whole grid.
   void perturbed_masks(double t, int i) {
     const double v = insect.get_control(i);
-    for (int sgn : {+1, -1}) {
-      insect.set_control(i, v + sgn * mask_h);
+    InsectArgs ap[2];
+    for (int k = 0; k < 2; ++k) {
+      insect.set_control(i, v + (k == 0 ? 1 : -1) * mask_h);
       insect.update(t);
-      InsectArgs a = insect_args(true, true);
+      ap[k] = insect_args(true, true);
+    }
+    insect.set_control(i, v);
+    const int nn[3] = {nx, ny, nz};
+    GridBox& b = adj.box;
+    b.nx = nx; b.ny = ny;
+    if (ap[0].use_box) {
+      for (int d = 0; d < 3; ++d) { b.lo[d] = nn[d]; b.hi[d] = -1; }
+      for (int k = 0; k < 2; ++k)
+        for (int w = 0; w < 2; ++w) {
+          if (!ap[k].draw_wing[w]) continue;
+          const int* lo = ap[k].box_lo[1 + w];
+          const int* hi = ap[k].box_hi[1 + w];
+          bool inside = true;
+          for (int d = 0; d < 3; ++d) inside = inside && lo[d] <= hi[d];
+          if (!inside) continue;
+          for (int d = 0; d < 3; ++d) { b.lo[d] = std::min(b.lo[d], lo[d]); b.hi[d] = std::max(b.hi[d], hi[d]); }
+        }
+    } else {
+      for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = nn[d] - 1; }
+    }
+    const bool empty = b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2];
+    for (int k = 0; k < 2; ++k) {
+      if (empty) break;  // no wing in the grid: both drawings agree everywhere
+      InsectArgs& a = ap[k];
+      for (int d = 0; d < 3; ++d) { a.reg_lo[d] = b.lo[d]; a.reg_hi[d] = b.hi[d]; }
+      a.clip = 0;
       if (!ins_h) {  // the eager argument ring
         HIPCHECK(hipStreamSynchronize(stream));
         ins_per_slot = std::max(ins_per_slot, 1);
@@ -950,10 +983,11 @@ class Solver : public SolverBase {
       InsectArgs* d = ins_d + (size_t)slot * ins_per_slot;
       HIPCHECK(hipMemcpyAsync(d, h, sizeof(InsectArgs), hipMemcpyHostToDevice, stream));
       HIPCHECK(hipEventRecord(ins_ev[slot], stream));
-      launch_insect<double>(geom, a, d, sgn > 0 ? adj.mP : adj.mM, sgn > 0 ? adj.usP : adj.usM, adj.colS, stream,
+      launch_insect<double>(geom, a, d, k == 0 ? adj.mP : adj.mM, k == 0 ? adj.usP : adj.usM, adj.colS, stream,
                             Batch{1, 0});
     }
-    insect.set_control(i, v);
+    if (empty)
+      for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = -1; }
     kcheck("perturbed masks");
   }
   // gacc_i += scale * sum f . (dchi_i (u - us) - chi dus_i) / (2h) at time t,
@@ -964,7 +998,8 @@ class Solver : public SolverBase {
     const int nb = adjoint_reduce_blocks(npts);
     for (int i = 0; i < ncon; ++i) {
       perturbed_masks(t, i);
-      launch_theta<T>(npts, phi, pf(0), us, mask, adj.mP, adj.usP, adj.mM, adj.usM, adj.part + i, ncon, stream);
+      launch_theta<T>(npts, adj.box, phi, pf(0), us, mask, adj.mP, adj.usP, adj.mM, adj.usM, adj.part + i, ncon,
+                      stream);
     }
     launch_reduce_rows(adj.part, nb, ncon, adj.kzero, rmid, adj.gstage, stream);
     launch_axpby<double>(ncon, 1.0, adj.gacc, scale / (2.0 * mask_h), adj.gstage, adj.gacc, stream);
