@@ -1,13 +1,15 @@
 // x pass of the right-hand side with one wavefront per x line (nx/2 = 256),
-// two per wave (128) or four (64).
+// two per wave (128), four (64), five (48) or eight (32).
 //
 // Each line is a length-M = nx/2 complex transform (the real transform of
 // length nx through the even/odd split).  A line lives in the registers of
 // G = M/4 lanes, lane l holding positions l + qG (q = 0..3) before and after
 // every transform.  The Stockham stages of radix 4 (one radix-2 stage for
-// M = 128) exchange data through a small per-wave LDS slot with wave-scope
-// ordering only: waves never wait for each other, so a compute unit keeps
-// as many lines in flight as its register file holds.
+// M = 128 and 32; radix 3, 4, 4 for M = 48) exchange data through a small
+// per-wave LDS slot with wave-scope ordering only: waves never wait for each
+// other, so a compute unit keeps as many lines in flight as its register
+// file holds.  For M = 48 the five lines use 60 lanes; the other four work
+// on a spare LDS row and store nothing.
 //
 // Per line (rhs.f90:147-330 cal_nlk_fsi, nonlinear and penalisation part):
 //   load the six spectra V_x, V_y, V_z, w_x, A = dz u_x, B = dy u_x
@@ -51,12 +53,12 @@ __device__ __forceinline__ double wmax(double v) {
 }
 
 // Radix plan of the in-wave transform: radix 4 throughout, the last stage of
-// M = 128 is radix 2 (two butterflies per lane).
+// M = 128 and M = 32 is radix 2 (two butterflies per lane).
 template <int M> struct WPlan {
-  static_assert(M == 64 || M == 128 || M == 256, "in-wave transform: M = 64, 128 or 256");
+  static_assert(M == 32 || M == 64 || M == 128 || M == 256, "in-wave transform: M = 32, 64, 128 or 256");
   static constexpr int G = M / 4;
-  static constexpr int NST = (M == 64) ? 3 : 4;
-  static constexpr __host__ __device__ int radix(int s) { return (M == 128 && s == 3) ? 2 : 4; }
+  static constexpr int NST = (M <= 64) ? 3 : 4;
+  static constexpr __host__ __device__ int radix(int s) { return ((M == 128 && s == 3) || (M == 32 && s == 2)) ? 2 : 4; }
   // sub-transform length entering stage s (only the last stage can be radix 2)
   static constexpr __host__ __device__ int ns(int s) { return 1 << (2 * s); }
 };
@@ -124,6 +126,63 @@ __device__ __forceinline__ void wfft(cplx<T> (&v)[4], const cplx<T>* tw, cplx<T>
   if constexpr (WPlan<M>::NST > 3) wstage<T, M, S, 3>(v, tw, ln, l);
 }
 
+// M = 48 = 3 x 4 x 4 (nx = 96).  Stage 0 (radix 3) has 16 butterflies for
+// the 12 lanes of a line, so every stage reads its butterflies' inputs from
+// the line's LDS row: butterfly j of stage 0 reads j + 16q and writes 3j + q
+// (lanes 0..3 also take butterflies 12..15); stage 1 (radix 4, Ns = 3) reads
+// j + 12q and writes (j/3) 12 + j%3 + 3q; stage 2 (radix 4, Ns = 12) reads
+// j + 12q and leaves its outputs at j + 12q, the lane's own positions.
+template <typename T, int S>
+__device__ __forceinline__ void wfft48(cplx<T> (&v)[4], const cplx<T>* tw, cplx<T>* ln, int l_) {
+  int l = l_;
+  asm volatile("" : "+v"(l));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) ln[lp<T>(l + 12 * q)] = v[q];
+  wave_sync();
+  const bool two = l < 4;
+  cplx<T> a[3], b[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    a[q] = ln[lp<T>(l + 16 * q)];
+    b[q] = ln[lp<T>((two ? 12 + l : l) + 16 * q)];
+  }
+  wave_sync();
+  Bfly<3, S, T>::run(a);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) ln[lp<T>(3 * l + q)] = a[q];
+  if (two) {
+    Bfly<3, S, T>::run(b);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) ln[lp<T>(3 * (12 + l) + q)] = b[q];
+  }
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = ln[lp<T>(l + 12 * q)];
+  wave_sync();
+  {
+    const int step = (l % 3) * 4;
+#pragma unroll
+    for (int q = 1; q < 4; ++q) v[q] = twm<S>(v[q], tw[q * step]);
+    Bfly<4, S, T>::run(v);
+    const int base = (l / 3) * 12 + (l % 3);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ln[lp<T>(base + 3 * q)] = v[q];
+  }
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = ln[lp<T>(l + 12 * q)];
+  wave_sync();
+#pragma unroll
+  for (int q = 1; q < 4; ++q) v[q] = twm<S>(v[q], tw[q * l]);
+  Bfly<4, S, T>::run(v);
+}
+
+template <typename T, int M, int S>
+__device__ __forceinline__ void wtransform(cplx<T> (&v)[4], const cplx<T>* tw, cplx<T>* ln, int l) {
+  if constexpr (M == 48) wfft48<T, S>(v, tw, ln, l);
+  else wfft<T, M, S>(v, tw, ln, l);
+}
+
 // WPE: minimum waves per SIMD the register allocation must allow (0: none)
 template <typename T, int M, bool SPONGE, bool STATS, int WPE>
 __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1))) void x_rhs_wave_k(XPassArgs a) {
@@ -132,13 +191,14 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
   constexpr int LS = line_stride<T>(M + 1);
   constexpr int WPB = XW_NT / 64;
   constexpr int NFO = SPONGE ? 6 : 3;
-  __shared__ cplx<T> sh[WPB][LPW * LS];
+  constexpr int ROWS = LPW + (LPW * G < 64 ? 1 : 0);  // a spare row for lanes beyond LPW lines
+  __shared__ cplx<T> sh[WPB][ROWS * LS];
   __shared__ cplx<T> twl[M], twn[NXH];
   __shared__ double red[WPB][4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l = lane % G;
   const int gl = (blockIdx.x * WPB + wv) * LPW + lane / G;
-  const bool live = gl < a.nlines;
+  const bool live = lane < LPW * G && gl < a.nlines;
   cplx<T>* ln = sh[wv] + (lane / G) * LS;
 
   const cplx<T>* twM = reinterpret_cast<const cplx<T>*>(a.twM);
@@ -204,7 +264,7 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
       const cplx<T> s1 = add(xa, conj_w(xm)), d1 = sub(xa, conj_w(xm));
       X[f][q] = add(s1, ik_w<T>(T(1), cmul(conj_w(twn[l + q * G]), d1)));
     }
-    wfft<T, M, +1>(X[f], twl, ln, l);
+    wtransform<T, M, +1>(X[f], twl, ln, l);
   }
 
   // ---- u x w - chi (u - us) [+ sponge]
@@ -285,13 +345,16 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
 #pragma unroll
   for (int c = 0; c < NFO; ++c) {
     __builtin_amdgcn_sched_barrier(0);
-    wfft<T, M, -1>(O[c], twl, ln, l);
+    wtransform<T, M, -1>(O[c], twl, ln, l);
 #pragma unroll
     for (int q = 0; q < 4; ++q) ln[lp<T>(l + q * G)] = O[c][q];
     wave_sync();
     cplx<T> zm[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) zm[q] = ln[lp<T>((M - l - q * G) & (M - 1))];
+    for (int q = 0; q < 4; ++q) {
+      const int n = l + q * G;
+      zm[q] = ln[lp<T>(n == 0 ? 0 : M - n)];
+    }
     wave_sync();
     if (live) {
       cplx<T>* o = out + c * a.fstride_out + (long)gl * a.prow;
@@ -346,7 +409,7 @@ void run_wave(const XPassArgs& a, hipStream_t s) {
 
 }  // namespace
 
-bool x_rhs_wave_supported(int nx) { return nx == 128 || nx == 256 || nx == 512; }
+bool x_rhs_wave_supported(int nx) { return nx == 64 || nx == 96 || nx == 128 || nx == 256 || nx == 512; }
 
 int x_rhs_wave_blocks(int nx, int nlines) {
   const int lpb = (XW_NT / 64) * (256 / (nx / 2));
@@ -356,10 +419,12 @@ int x_rhs_wave_blocks(int nx, int nlines) {
 template <typename T>
 void launch_x_rhs_wave(const XPassArgs& a, hipStream_t s) {
   switch (a.nx / 2) {
+    case 32: run_wave<T, 32>(a, s); break;
+    case 48: run_wave<T, 48>(a, s); break;
     case 64: run_wave<T, 64>(a, s); break;
     case 128: run_wave<T, 128>(a, s); break;
     case 256: run_wave<T, 256>(a, s); break;
-    default: throw std::runtime_error("in-wave x pass: nx must be 128, 256 or 512");
+    default: throw std::runtime_error("in-wave x pass: nx must be 64, 96, 128, 256 or 512");
   }
 }
 template void launch_x_rhs_wave<float>(const XPassArgs&, hipStream_t);

@@ -97,12 +97,14 @@ def test_sphere_cavity_sponge_cal_nlk(tmp_path, prec, tol):
         assert rel_l2(s.get_pressure(0.0), pr_ref) < tol * 50
 
 
-@pytest.mark.parametrize("nx", [128, 256, 512])
+@pytest.mark.parametrize("nx", [64, 96, 128, 256, 512])
 @pytest.mark.parametrize("prec,tol", PRECS)
 def test_x_pass_per_wave_grids(tmp_path, nx, prec, tol):
     """The right-hand-side x pass that holds one x line per wavefront (two for
-    nx = 256, four for nx = 128): cal_nlk with penalisation and the cavity
-    sponge, and one CFL-adapted RK2 step (max |u| comes from that pass)."""
+    nx = 256, four for nx = 128, five for nx = 96 with its radix-3 stage,
+    eight for nx = 64): cal_nlk with penalisation and the cavity sponge, and
+    one CFL-adapted RK2 step (max |u| comes from that pass).  256 lines, so
+    the last workgroup at nx = 96 is partly empty."""
     ini = rewrite_ini(os.path.join(GOLDEN, "sphere_sponge_RK2", "sphere_sponge_RK2.ini"), str(tmp_path),
                       Resolution__nx=nx, Resolution__ny=16, Resolution__nz=16)
     sim = fo.Simulation(ini)
