@@ -34,6 +34,12 @@ struct GridBox { int lo[3], hi[3]; int nx, ny; };
 template <typename T> void launch_theta(long npts, const GridBox& box, const void* phi, const void* u, const void* us,
                                         const void* mask, const double* mP, const double* usP, const double* mM,
                                         const double* usM, double* partial, int nv, hipStream_t s);
+// the same for ncon controls in one launch: control i's drawings 2i (+h) and
+// 2i+1 (-h) lie at slab + (2i) stride and slab + (2i+1) stride, each an FP64
+// mask (npts) followed by its us (3 npts); partial[block * ncon + i]
+template <typename T> void launch_theta_batch(long npts, const GridBox& box, const void* phi, const void* u,
+                                              const void* us, const void* mask, const void* slab, long stride,
+                                              int ncon, double* partial, hipStream_t s);
 // per-block partial sums of mask (u_z - us_z)
 template <typename T> void launch_lift(long npts, const void* u, const void* us, const void* mask, double* partial,
                                        hipStream_t s);

@@ -134,13 +134,13 @@ __device__ __forceinline__ void block_partial(double v, double* partial, int nv 
   }
 }
 
-// sum over points of f . (dm (u - us) - m dus), f = phi or e_z
+// sum over the box's points of f . (dm (u - us) - m dus), f = phi or e_z
 template <typename T>
-__global__ __launch_bounds__(RED_NT) void theta_k(long npts, GridBox box, const T* __restrict__ phi,
-                                                  const T* __restrict__ u, const T* __restrict__ us,
-                                                  const T* __restrict__ mask, const double* __restrict__ mP,
-                                                  const double* __restrict__ usP, const double* __restrict__ mM,
-                                                  const double* __restrict__ usM, double* __restrict__ partial, int nv) {
+__device__ __forceinline__ double theta_sum(long npts, const GridBox& box, const T* __restrict__ phi,
+                                            const T* __restrict__ u, const T* __restrict__ us,
+                                            const T* __restrict__ mask, const double* __restrict__ mP,
+                                            const double* __restrict__ usP, const double* __restrict__ mM,
+                                            const double* __restrict__ usM) {
   double acc = 0.0;
   const long ex = box.hi[0] - box.lo[0] + 1, ey = box.hi[1] - box.lo[1] + 1, ez = box.hi[2] - box.lo[2] + 1;
   const long nbox = (ex > 0 && ey > 0 && ez > 0) ? ex * ey * ez : 0;
@@ -157,7 +157,29 @@ __global__ __launch_bounds__(RED_NT) void theta_k(long npts, GridBox box, const 
       acc += f * (dm * ((double)u[c * npts + p] - (double)us[c * npts + p]) - m * dus);
     }
   }
-  block_partial(acc, partial, nv);
+  return acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_NT) void theta_k(long npts, GridBox box, const T* __restrict__ phi,
+                                                  const T* __restrict__ u, const T* __restrict__ us,
+                                                  const T* __restrict__ mask, const double* __restrict__ mP,
+                                                  const double* __restrict__ usP, const double* __restrict__ mM,
+                                                  const double* __restrict__ usM, double* __restrict__ partial, int nv) {
+  block_partial(theta_sum<T>(npts, box, phi, u, us, mask, mP, usP, mM, usM), partial, nv);
+}
+
+// the same for every control at once (blockIdx.y = control i): its drawings
+// 2i (+h) and 2i+1 (-h) are DrawSlab records `stride` bytes apart from `slab`
+template <typename T>
+__global__ __launch_bounds__(RED_NT) void theta_batch_k(long npts, GridBox box, const T* __restrict__ phi,
+                                                        const T* __restrict__ u, const T* __restrict__ us,
+                                                        const T* __restrict__ mask, const char* __restrict__ slab,
+                                                        long stride, double* __restrict__ partial, int nv) {
+  const int i = blockIdx.y;
+  const double* mP = reinterpret_cast<const double*>(slab + (2L * i) * stride);
+  const double* mM = reinterpret_cast<const double*>(slab + (2L * i + 1) * stride);
+  block_partial(theta_sum<T>(npts, box, phi, u, us, mask, mP, mP + npts, mM, mM + npts), partial + i, nv);
 }
 
 // sum over points of m (u_z - us_z)
@@ -224,6 +246,19 @@ void launch_theta(long npts, const GridBox& box, const void* phi, const void* u,
                      (const T*)u, (const T*)us, (const T*)mask, mP, usP, mM, usM, partial, nv);
 }
 template <typename T>
+void launch_theta_batch(long npts, const GridBox& box, const void* phi, const void* u, const void* us,
+                        const void* mask, const void* slab, long stride, int ncon, double* partial, hipStream_t s) {
+  const int nz = (int)(npts / ((long)box.nx * box.ny));
+  for (int i = 0; i < 3; ++i) {
+    const int n = i == 0 ? box.nx : (i == 1 ? box.ny : nz);
+    if (box.lo[i] < 0 || box.hi[i] >= n) throw std::runtime_error("theta terms: box outside the grid");
+  }
+  if (ncon < 1 || ncon > 65535) throw std::runtime_error("theta terms: bad control count");
+  hipLaunchKernelGGL(theta_batch_k<T>, dim3(adjoint_reduce_blocks(npts), ncon), dim3(RED_NT), 0, s, npts, box,
+                     (const T*)phi, (const T*)u, (const T*)us, (const T*)mask, (const char*)slab, stride, partial,
+                     ncon);
+}
+template <typename T>
 void launch_lift(long npts, const void* u, const void* us, const void* mask, double* partial, hipStream_t s) {
   hipLaunchKernelGGL(lift_k<T>, dim3(adjoint_reduce_blocks(npts)), dim3(RED_NT), 0, s, npts, (const T*)u,
                      (const T*)us, (const T*)mask, partial);
@@ -242,6 +277,8 @@ void launch_ez_mask(long npts, double scale, const void* mask, void* out, hipStr
   template void launch_theta<T>(long, const GridBox&, const void*, const void*, const void*, const void*,       \
                                 const double*, const double*, const double*, const double*, double*, int,         \
                                 hipStream_t);                                                                     \
+  template void launch_theta_batch<T>(long, const GridBox&, const void*, const void*, const void*, const void*,   \
+                                      const void*, long, int, double*, hipStream_t);                              \
   template void launch_lift<T>(long, const void*, const void*, const void*, double*, hipStream_t);                \
   template void launch_ez_mask<T>(long, double, const void*, void*, hipStream_t);
 FLUSI_ADJ_INST(float)

@@ -806,6 +806,9 @@ class Solver : public SolverBase {
     double *mP = nullptr, *usP = nullptr, *mM = nullptr, *usM = nullptr;
     uint8_t* colS = nullptr;
     GridBox box{};                  // where the last pair of perturbed drawings can differ
+    char* slab = nullptr;           // batched drawings: per drawing an FP64 mask, its us, a colour field
+    long slab_stride = 0;           // bytes between two drawings' records
+    int slab_n = 0;                 // drawings the slab holds
     double* part = nullptr;         // reduction partials (blocks x controls)
     double* gstage = nullptr;       // one call's per-control sums
     double* gacc = nullptr;         // the gradient, accumulated on the device
@@ -990,16 +993,105 @@ class Solver : public SolverBase {
       for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = -1; }
     kcheck("perturbed masks");
   }
+  // n insect argument records into the next slot of the eager ring (grown
+  // to n records per slot when needed); returns their device copy
+  const InsectArgs* upload_insect_args(const InsectArgs* recs, int n) {
+    if (!ins_h || ins_per_slot < n) {
+      HIPCHECK(hipStreamSynchronize(stream));
+      dfree(ins_d);
+      if (ins_h) HIPCHECK(hipHostFree(ins_h));
+      ins_per_slot = std::max(ins_per_slot, n);
+      ins_d = (InsectArgs*)dalloc(sizeof(InsectArgs) * INS_SLOTS * ins_per_slot);
+      HIPCHECK(hipHostMalloc((void**)&ins_h, sizeof(InsectArgs) * INS_SLOTS * ins_per_slot, hipHostMallocDefault));
+    }
+    const int slot = ins_next;
+    ins_next = (ins_next + 1) % INS_SLOTS;
+    if (ins_ev[slot]) HIPCHECK(hipEventSynchronize(ins_ev[slot]));
+    else HIPCHECK(hipEventCreateWithFlags(&ins_ev[slot], hipEventDisableTiming));
+    InsectArgs* h = ins_h + (size_t)slot * ins_per_slot;
+    std::copy(recs, recs + n, h);
+    InsectArgs* d = ins_d + (size_t)slot * ins_per_slot;
+    HIPCHECK(hipMemcpyAsync(d, h, sizeof(InsectArgs) * n, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipEventRecord(ins_ev[slot], stream));
+    return d;
+  }
+
+  // All 2 ncon perturbed drawings of one theta_terms call in one batched
+  // launch (member 2i: control i at +h, 2i+1: at -h), over the union of their
+  // wing boxes; each drawing has its own record in adj.slab.  Used while the
+  // slab fits kSlabMax bytes.
+  static constexpr size_t kSlabMax = size_t(4) << 30;
+  bool batched_drawings(int ncon) {
+    const long stride = ((33L * npts + 255) / 256) * 256;  // mask 8 B, us 24 B, colour 1 B per point
+    if ((size_t)stride * 2 * ncon > kSlabMax || (long)nz * 2 * ncon > 65535) return false;
+    if (adj.slab_n < 2 * ncon || adj.slab_stride != stride) {
+      HIPCHECK(hipStreamSynchronize(stream));
+      dfree(adj.slab);
+      adj.slab = (char*)dalloc((size_t)stride * 2 * ncon);
+      adj.slab_stride = stride;
+      adj.slab_n = 2 * ncon;
+    }
+    return true;
+  }
+  void perturbed_masks_all(double t, int ncon) {
+    std::vector<InsectArgs> recs(2 * ncon);
+    for (int i = 0; i < ncon; ++i) {
+      const double v = insect.get_control(i);
+      for (int k = 0; k < 2; ++k) {
+        insect.set_control(i, v + (k == 0 ? 1 : -1) * mask_h);
+        insect.update(t);
+        recs[2 * i + k] = insect_args(true, true);
+      }
+      insect.set_control(i, v);
+    }
+    const int nn[3] = {nx, ny, nz};
+    GridBox& b = adj.box;
+    b.nx = nx; b.ny = ny;
+    if (recs[0].use_box) {
+      for (int d = 0; d < 3; ++d) { b.lo[d] = nn[d]; b.hi[d] = -1; }
+      for (const InsectArgs& a : recs)
+        for (int w = 0; w < 2; ++w) {
+          if (!a.draw_wing[w]) continue;
+          bool inside = true;
+          for (int d = 0; d < 3; ++d) inside = inside && a.box_lo[1 + w][d] <= a.box_hi[1 + w][d];
+          if (!inside) continue;
+          for (int d = 0; d < 3; ++d) {
+            b.lo[d] = std::min(b.lo[d], a.box_lo[1 + w][d]);
+            b.hi[d] = std::max(b.hi[d], a.box_hi[1 + w][d]);
+          }
+        }
+    } else {
+      for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = nn[d] - 1; }
+    }
+    if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) {
+      for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = -1; }
+      return;  // no wing in the grid: every pair of drawings agrees everywhere
+    }
+    for (InsectArgs& a : recs) {
+      for (int d = 0; d < 3; ++d) { a.reg_lo[d] = b.lo[d]; a.reg_hi[d] = b.hi[d]; }
+      a.clip = 0;
+    }
+    const InsectArgs* dev = upload_insect_args(recs.data(), 2 * ncon);
+    launch_insect<double>(geom, recs[0], dev, adj.slab, adj.slab + 8 * npts, (uint8_t*)(adj.slab + 32 * npts), stream,
+                          Batch{2 * ncon, adj.slab_stride});
+    kcheck("perturbed masks (batched)");
+  }
+
   // gacc_i += scale * sum f . (dchi_i (u - us) - chi dus_i) / (2h) at time t,
   // with f = phi or e_z (phi null), u in pf(0..2), mask and us at t; all on
   // the device
   void theta_terms(double t, const void* phi, double scale) {
     const int ncon = (int)adj.theta.size();
     const int nb = adjoint_reduce_blocks(npts);
-    for (int i = 0; i < ncon; ++i) {
-      perturbed_masks(t, i);
-      launch_theta<T>(npts, adj.box, phi, pf(0), us, mask, adj.mP, adj.usP, adj.mM, adj.usM, adj.part + i, ncon,
-                      stream);
+    if (batched_drawings(ncon)) {
+      perturbed_masks_all(t, ncon);
+      launch_theta_batch<T>(npts, adj.box, phi, pf(0), us, mask, adj.slab, adj.slab_stride, ncon, adj.part, stream);
+    } else {
+      for (int i = 0; i < ncon; ++i) {
+        perturbed_masks(t, i);
+        launch_theta<T>(npts, adj.box, phi, pf(0), us, mask, adj.mP, adj.usP, adj.mM, adj.usM, adj.part + i, ncon,
+                        stream);
+      }
     }
     launch_reduce_rows(adj.part, nb, ncon, adj.kzero, rmid, adj.gstage, stream);
     launch_axpby<double>(ncon, 1.0, adj.gacc, scale / (2.0 * mask_h), adj.gstage, adj.gacc, stream);
