@@ -1575,7 +1575,7 @@ class Solver : public SolverBase {
   void save_fields() override {
     need_fft();
     if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
-    if (p.striding != 1) throw FlusiError(8881, "striding > 1 is not supported by this build's field writer");
+    if (p.striding != 1) throw FlusiError(8881, "striding > 1 is not supported by the HDF5 field writer");
     char name[32];
     if (p.naming == "timestep") std::snprintf(name, sizeof(name), "%06d", it);
     else if (p.save_only_one_period == "yes")  // save_fields.f90:86 (t minus the period count)

@@ -19,9 +19,10 @@ forward multiplied by 1/N, inverse unnormalised (cof_p3dfft.f90:33-83).  It is
 pinned by the closed-form checks of the reference's own start-up self test
 (fft_unit_test.f90:81-253) and brute-force DFTs in tests/test_oracle_core.py.
 
-Pinning status of each function (what fails if it is wrong) is listed in
-DESIGN.md; functions without an independent check say "parity unpinned" in
-their docstring.
+Each function is pinned by the tests in tests/test_oracle_core.py (closed
+forms, brute force, invariants) and tests/test_oracle_fixtures.py (the
+reference's own fixtures); functions without an independent check say
+"parity unpinned" in their docstring.
 """
 from __future__ import annotations
 
