@@ -22,7 +22,9 @@ pinned by the closed-form checks of the reference's own start-up self test
 Each function is pinned by the tests in tests/test_oracle_core.py (closed
 forms, brute force, invariants) and tests/test_oracle_fixtures.py (the
 reference's own fixtures); functions without an independent check say
-"parity unpinned" in their docstring.
+"parity unpinned" in their docstring: the outlet_x sponge region
+(penalize_vort), the ellipsoid_plain body (_draw_body, a build extension)
+and the sinusoidal mean flow (set_mean_flow).
 """
 from __future__ import annotations
 
@@ -853,7 +855,8 @@ class Insect:
     def _draw_body(self, g, mask, color, us):
         """body_geometry.f90:5-122 draw_insect_body; :511-595 draw_body_jerry.
         'ellipsoid_plain' (build extension) is Jerry's ellipsoid with b_body and
-        L_body from the ini file and no head or eyes."""
+        L_body from the ini file and no head or eyes; it has no reference
+        counterpart and is parity unpinned beyond sharing Jerry's formula."""
         bt = self.BodyType
         if bt == "nobody":
             return
@@ -1219,7 +1222,10 @@ class Simulation:
 
     # -- sponge ----------------------------------------------------------------
     def penalize_vort(self, vort_c: np.ndarray) -> np.ndarray:
-        """sponge.f90:94-329 penalize_vort (cavity, outlet_x)."""
+        """sponge.f90:94-329 penalize_vort (cavity, outlet_x).  The cavity
+        region is pinned by the sphere_sponge_RK2 fixture; the outlet_x region
+        predicate (sponge.f90 outlet_x branch) is parity unpinned: no fixture
+        of the reference uses it."""
         p, g = self.p, self.g
         th = p.sponge_thickness
         eps_inv = 1.0 / p.eps_sponge
@@ -1397,7 +1403,9 @@ class Simulation:
             self.create_mask(self.time)
 
     def set_mean_flow(self, time):
-        """fluid_time_step.f90:1066-1090."""
+        """fluid_time_step.f90:1066-1090.  The fixed mean flow is pinned by the
+        sphere fixtures; the sinusoidal branch is parity unpinned (no fixture
+        uses it)."""
         p = self.p
         for i in range(3):
             if p.iMeanFlow[i] == "fixed":
