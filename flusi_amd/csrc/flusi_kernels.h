@@ -40,6 +40,9 @@ struct TilePassArgs {
   TileJob jobs[6];
   int njobs;
   int nb, nc;          // extents of the tile axis and the outer axis
+  int nb_store;        // in-wave passes: lines the segment-side store covers (>= nb; lines
+                       // beyond nb are row padding no kernel reads, stored as zeros so that
+                       // the last tile writes whole 128-byte segments); 0 = nb
   Strides sin, sout;
   double kscale;       // k_a = kscale * wrap(a): wavenumbers along the FFT axis
   double scale;        // applied to every stored value

@@ -135,7 +135,8 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
 #pragma unroll
           for (int q = 0; q < RL; ++q) w[q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
           twiddle_bfly<T, L, RL, S, NSL>(w, j, tw);
-          if (b0 + bb < a.nb) {
+          // segment-side stores may cover the row padding (zeros; see nb_store)
+          if (b0 + bb < (CONTIG_IN && a.nb_store > a.nb ? a.nb_store : a.nb)) {
             const long base = !CONTIG_IN ? c * a.sout.c + (long)(b0 + bb) * a.sout.b + j
                                          : c * a.sout.c + (long)(b0 + bb) + (long)j * a.sout.a;
             const long step = !CONTIG_IN ? (long)(L / RL) : (long)(L / RL) * a.sout.a;

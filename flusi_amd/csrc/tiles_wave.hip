@@ -58,6 +58,7 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
   cplx<T>* ln = tile + line * LS;
   const int b0 = blockIdx.x * TL;
   const long c = blockIdx.y;
+  const int nbs = a.nb_store > a.nb ? a.nb_store : a.nb;  // lines >= nb hold zeros
   const long moff = (long)blockIdx.z * a.bat.mstride;  // batch member's slab
   const T sc = (T)a.scale;
   const cplx<T>* tw;
@@ -118,7 +119,7 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
         for (int k = 0; k < K; ++k) {
           const int e = tid + k * NT, bb = e % TL, aa = e / TL;
           const cplx<T> x = tile[bb * LS + lp<T>(aa)];
-          if (b0 + bb < a.nb) out[c * a.sout.c + (long)(b0 + bb) + (long)aa * a.sout.a] = mk<T>(x.x * sc, x.y * sc);
+          if (b0 + bb < nbs) out[c * a.sout.c + (long)(b0 + bb) + (long)aa * a.sout.a] = mk<T>(x.x * sc, x.y * sc);
         }
         __syncthreads();
       } else {
