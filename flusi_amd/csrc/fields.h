@@ -113,5 +113,15 @@ template <typename T> void launch_diag(const GridGeom& g, const DiagArgs& a, hip
 template <typename T> void launch_from_f64(const double* src, void* dst, long n, hipStream_t s);
 template <typename T> void launch_to_f64(const void* src, double* dst, long n, hipStream_t s);
 void launch_zero(void* p, size_t bytes, hipStream_t s);
+// integrals.f90:522-552 Parseval sums of 3 spectral fields: energy_k_blocks()
+// partials (to be summed by launch_reduce_rows)
+int energy_k_blocks();
+template <typename T> void launch_energy_k(const GridGeom& g, const void* uk, double* partial, hipStream_t s);
+// vis.f90:9-66 cal_vis into a device table of ncplx doubles
+void launch_cal_vis(const GridGeom& g, const double* kx, const double* ky, const double* kz, const double* kxt2,
+                    const double* kyt2, const double* kzt2, double dt, double nu, int dealias, double* vis,
+                    hipStream_t s);
+// out = |w| of 3 physical fields (component stride npts)
+template <typename T> void launch_magnitude(long npts, const void* w, void* out, hipStream_t s);
 
 }  // namespace flusi

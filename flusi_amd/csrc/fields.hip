@@ -682,6 +682,75 @@ void launch_pressure_phys(long npts, const void* u, void* p, hipStream_t s) {
 template void launch_pressure_phys<float>(long, const void*, void*, hipStream_t);
 template void launch_pressure_phys<double>(long, const void*, void*, hipStream_t);
 
+// integrals.f90:522-552 compute_energies_k: per-block compensated FP64 sums
+// of w |uk|^2 over the 3 spectral fields (S layout, w = 1/2 on the kx = 0 and
+// kx = nx/2 planes), one partial per block for the deterministic row reducer
+template <typename T>
+__global__ __launch_bounds__(256) void energy_k_k(GridGeom g, const typename V2<T>::t* __restrict__ uk,
+                                                  double* __restrict__ partial) {
+  __shared__ double red[256 / 64];
+  const long n = 3 * g.ncplx, plane = (long)g.ny * g.nz;
+  double s = 0.0, c = 0.0;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const long ix = (e % g.ncplx) / plane;
+    const double w = (ix == 0 || ix == g.nx / 2) ? 0.5 : 1.0;
+    const double re = (double)uk[e].x, im = (double)uk[e].y;
+    const double v = w * (re * re + im * im);
+    const double t = s + v;
+    c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  double v = s + c;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+int energy_k_blocks() { return 512; }
+template <typename T>
+void launch_energy_k(const GridGeom& g, const void* uk, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(energy_k_k<T>, dim3(energy_k_blocks()), dim3(256), 0, s, g,
+                     (const typename V2<T>::t*)uk, partial);
+}
+template void launch_energy_k<float>(const GridGeom&, const void*, double*, hipStream_t);
+template void launch_energy_k<double>(const GridGeom&, const void*, double*, hipStream_t);
+
+// vis.f90:9-66 cal_vis: exp(-dt nu k^2), 0 on the dealiased modes, in the
+// spectral layout (ix, iy, iz), iz fastest
+__global__ void cal_vis_k(GridGeom g, const double* kx, const double* ky, const double* kz, const double* kxt2,
+                          const double* kyt2, const double* kzt2, double dt, double nu, int dealias, double* vis) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.ncplx) return;
+  const int iz = (int)(e % g.nz), iy = (int)((e / g.nz) % g.ny), ix = (int)(e / ((long)g.nz * g.ny));
+  const double k2 = (kx[ix] * kx[ix] + ky[iy] * ky[iy]) + kz[iz] * kz[iz];
+  const bool alias = ((kxt2[ix] + kyt2[iy]) + kzt2[iz]) >= 1.0;
+  vis[e] = (alias && dealias) ? 0.0 : exp(-dt * nu * k2);
+}
+void launch_cal_vis(const GridGeom& g, const double* kx, const double* ky, const double* kz, const double* kxt2,
+                    const double* kyt2, const double* kzt2, double dt, double nu, int dealias, double* vis,
+                    hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(cal_vis_k, dim3((g.ncplx + nt - 1) / nt), dim3(nt), 0, s, g, kx, ky, kz, kxt2, kyt2, kzt2, dt,
+                     nu, dealias, vis);
+}
+
+// |w| of 3 physical fields (save_fields.f90:156-160 vorabs)
+template <typename T>
+__global__ void magnitude_k(long npts, const T* __restrict__ w, T* __restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const double a = w[p], b = w[npts + p], c = w[2 * npts + p];
+  out[p] = (T)sqrt(a * a + b * b + c * c);
+}
+template <typename T>
+void launch_magnitude(long npts, const void* w, void* out, hipStream_t s) {
+  const int nt = 256;
+  hipLaunchKernelGGL(magnitude_k<T>, dim3((npts + nt - 1) / nt), dim3(nt), 0, s, npts, (const T*)w, (T*)out);
+}
+template void launch_magnitude<float>(long, const void*, void*, hipStream_t);
+template void launch_magnitude<double>(long, const void*, void*, hipStream_t);
+
 __global__ void zero_k(uint32_t* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;

@@ -1611,8 +1611,9 @@ class Solver : public SolverBase {
       }
       if (p.iSaveMagVorticity == 1) {
         b.resize(npts);
-        for (long i = 0; i < npts; ++i)
-          b[i] = std::sqrt(a[i] * a[i] + a[npts + i] * a[npts + i] + a[2 * npts + i] * a[2 * npts + i]);
+        launch_magnitude<T>(npts, pf(3), pf(6), stream);  // get_u left the vorticity in pf(3..5)
+        kcheck("vorabs");
+        copy_real_out(pf(6), b.data(), npts);
         save_field_h5("vorabs_" + tag, b.data());
       }
     }
@@ -1782,32 +1783,30 @@ class Solver : public SolverBase {
   }
 
   // ------------------------------------------------------------ diagnostics
+  // integrals.f90:522-552 compute_energies_k of the current uk, on the device
+  double* epart = nullptr;
   double energy_k() override {
     need_fft();
-    std::vector<double> h(6 * nc);
-    get_uk(h.data());
+    if (!epart) epart = (double*)dalloc(sizeof(double) * energy_k_blocks());
+    launch_energy_k<T>(geom, uk, epart, stream);
+    launch_reduce_rows(epart, energy_k_blocks(), 1, kinds + 4, rmid, red + 9, stream);
+    kcheck("energy_k");
     double e = 0.0;
-    for (int c = 0; c < 3; ++c)
-      for (int ix = 0; ix < nxh; ++ix) {
-        const double w = (ix == 0 || ix == nx / 2) ? 0.5 : 1.0;
-        for (int iy = 0; iy < ny; ++iy)
-          for (int iz = 0; iz < nz; ++iz) {
-            const long k = (long)c * nc + ((long)ix * ny + iy) * nz + iz;
-            e += w * (h[2 * k] * h[2 * k] + h[2 * k + 1] * h[2 * k + 1]);
-          }
-      }
+    HIPCHECK(hipMemcpyAsync(&e, red + 9, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
     return e * p.xl * p.yl * p.zl;
   }
 
+  // vis.f90:9-66 cal_vis into a host table (the boundary copy; the steppers
+  // evaluate the same factors inside the z step)
+  double* visd = nullptr;
   void cal_vis(double dt, double* vis) override {
-    // vis.f90:9-66, host side (only needed at the boundary)
-    for (int ix = 0; ix < nxh; ++ix)
-      for (int iy = 0; iy < ny; ++iy)
-        for (int iz = 0; iz < nz; ++iz) {
-          const double kx2 = kx_h[ix] * kx_h[ix], ky2 = ky_h[iy] * ky_h[iy], kz2 = kz_h[iz] * kz_h[iz];
-          const bool alias = ((kxt2_h[ix] + kyt2_h[iy]) + kzt2_h[iz]) >= 1.0;
-          vis[((long)ix * ny + iy) * nz + iz] = (alias && p.iDealias == 1) ? 0.0 : std::exp(-dt * p.nu * ((kx2 + ky2) + kz2));
-        }
+    if (!visd) visd = (double*)dalloc(sizeof(double) * nc);
+    double* d = visd;
+    launch_cal_vis(geom, kx_d, ky_d, kz_d, kxt2_d, kyt2_d, kzt2_d, dt, p.nu, p.iDealias == 1 ? 1 : 0, d, stream);
+    kcheck("cal_vis");
+    HIPCHECK(hipMemcpyAsync(vis, d, sizeof(double) * nc, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
   }
 
   // integrals.f90:48-238 write_integrals_fsi
