@@ -21,6 +21,26 @@ __device__ __forceinline__ double vis_factor(double dtv, double nu, double k2, b
   return exp(-dtv * nu * k2);
 }
 
+// The epilogue's state reads and writes stream (each element is touched once
+// per launch): non-temporal accesses keep them from crowding the caches that
+// the transform's operands go through.
+template <typename T> __device__ __forceinline__ cplx<T> lds(const cplx<T>* p) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(cplx<T>, __builtin_nontemporal_load(reinterpret_cast<const double*>(p)));
+  } else {
+    using V = long long __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(cplx<T>, __builtin_nontemporal_load(reinterpret_cast<const V*>(p)));
+  }
+}
+template <typename T> __device__ __forceinline__ void sts(cplx<T>* p, cplx<T> v) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_nontemporal_store(__builtin_bit_cast(double, v), reinterpret_cast<double*>(p));
+  } else {
+    using V = long long __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(__builtin_bit_cast(V, v), reinterpret_cast<V*>(p));
+  }
+}
+
 // Stepper modes are grouped so that each kernel carries only the update code
 // of its own stepper (the epilogue is unrolled over a thread's elements).
 enum { MG_RHS = 0, MG_AB2 = 1, MG_RK2 = 2, MG_RK4 = 3 };
@@ -93,13 +113,14 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
     for (int c = 0; c < 3; ++c)
       if (a.meanflow_dynamic[c]) { nr[c] = (R)(-a.pen_sum[c + mo.md] * a.dynamic_scale); ni[c] = R(0); }
   }
+  cplx<T> udv[3];  // the diffusion operand, reused when it is u itself (RK4 stage 1)
   if (a.explicit_diffusion) {
     // rhs.f90:441-468
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const cplx<T> us_ = UD[c * a.fs + si];
-      nr[c] -= nu * k2 * (R)us_.x;
-      ni[c] -= nu * k2 * (R)us_.y;
+      udv[c] = lds<T>(UD + c * a.fs + si);
+      nr[c] -= nu * k2 * (R)udv[c].x;
+      ni[c] -= nu * k2 * (R)udv[c].y;
     }
   }
   if constexpr (MG == MG_RHS) {
@@ -112,7 +133,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
         pr = di / k2;
         pi = -dr / k2;
       }
-      RO[si] = mk<T>((T)pr, (T)pi);
+      sts<T>(RO + si, mk<T>((T)pr, (T)pi));
       return;
     }
   }
@@ -130,7 +151,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
     case SM_RHS_ONLY:
       if constexpr (MG == MG_RHS) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) RO[c * a.fs + si] = mk<T>((T)nr[c], (T)ni[c]);
+        for (int c = 0; c < 3; ++c) sts<T>(RO + c * a.fs + si, mk<T>((T)nr[c], (T)ni[c]));
       }
       break;
     case SM_EULER: if constexpr (MG == MG_AB2) {
@@ -138,11 +159,11 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       const R E = (R)vis_factor(a.vis_dt_new, a.nu, k2d, a.dealias_on && alias);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const cplx<T> u0 = U[c * a.fs + si];
+        const cplx<T> u0 = lds<T>(U + c * a.fs + si);
         unr[c] = ((R)u0.x + dt * nr[c]) * E;
         uni[c] = ((R)u0.y + dt * ni[c]) * E;
-        U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-        RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
+        sts<T>(U + c * a.fs + si, mk<T>((T)unr[c], (T)uni[c]));
+        sts<T>(RG0 + c * a.fs + si, mk<T>((T)(nr[c] * E), (T)(ni[c] * E)));
       }
       break;
     }
@@ -152,12 +173,12 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       const R b10 = (R)a.b10, b11 = (R)a.b11;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const cplx<T> u0 = U[c * a.fs + si];
-        const cplx<T> n1 = RG1[c * a.fs + si];
+        const cplx<T> u0 = lds<T>(U + c * a.fs + si);
+        const cplx<T> n1 = lds<T>(RG1 + c * a.fs + si);
         unr[c] = ((R)u0.x + b10 * nr[c] + b11 * (R)n1.x) * E;
         uni[c] = ((R)u0.y + b10 * ni[c] + b11 * (R)n1.y) * E;
-        U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
-        RG0[c * a.fs + si] = mk<T>((T)(nr[c] * E), (T)(ni[c] * E));
+        sts<T>(U + c * a.fs + si, mk<T>((T)unr[c], (T)uni[c]));
+        sts<T>(RG0 + c * a.fs + si, mk<T>((T)(nr[c] * E), (T)(ni[c] * E)));
       }
       break;
     }
@@ -168,9 +189,9 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const R n0r = nr[c] * Eo, n0i = ni[c] * Eo;
-        const cplx<T> u0 = U[c * a.fs + si];
-        RG0[c * a.fs + si] = mk<T>((T)n0r, (T)n0i);
-        U[c * a.fs + si] = mk<T>((T)((R)u0.x * En + dt * n0r), (T)((R)u0.y * En + dt * n0i));
+        const cplx<T> u0 = lds<T>(U + c * a.fs + si);
+        sts<T>(RG0 + c * a.fs + si, mk<T>((T)n0r, (T)n0i));
+        sts<T>(U + c * a.fs + si, mk<T>((T)((R)u0.x * En + dt * n0r), (T)((R)u0.y * En + dt * n0i)));
       }
       break;
     }
@@ -178,8 +199,8 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       // fluid_time_step.f90:594-597
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const cplx<T> u0 = U[c * a.fs + si];
-        const cplx<T> n0 = RG0[c * a.fs + si];
+        const cplx<T> u0 = lds<T>(U + c * a.fs + si);
+        const cplx<T> n0 = lds<T>(RG0 + c * a.fs + si);
         unr[c] = (R)u0.x + R(0.5) * dt * (-(R)n0.x + nr[c]);
         uni[c] = (R)u0.y + R(0.5) * dt * (-(R)n0.y + ni[c]);
       }
@@ -193,17 +214,17 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       for (int c = 0; c < 3; ++c) {
         const R kr = nr[c] * kill, ki = ni[c] * kill;
         if (a.mode == SM_RK4_1) {
-          const cplx<T> u0 = U[c * a.fs + si];
-          ACC[c * a.fs + si] = mk<T>((T)((R)u0.x + dt6 * kr), (T)((R)u0.y + dt6 * ki));
-          UST[c * a.fs + si] = mk<T>((T)((R)u0.x + R(0.5) * dt * kr), (T)((R)u0.y + R(0.5) * dt * ki));
+          const cplx<T> u0 = (a.explicit_diffusion && a.udiff == a.u) ? udv[c] : lds<T>(U + c * a.fs + si);
+          sts<T>(ACC + c * a.fs + si, mk<T>((T)((R)u0.x + dt6 * kr), (T)((R)u0.y + dt6 * ki)));
+          sts<T>(UST + c * a.fs + si, mk<T>((T)((R)u0.x + R(0.5) * dt * kr), (T)((R)u0.y + R(0.5) * dt * ki)));
         } else if (a.mode == SM_RK4_2 || a.mode == SM_RK4_3) {
-          const cplx<T> u0 = U[c * a.fs + si];
-          const cplx<T> ac = ACC[c * a.fs + si];
+          const cplx<T> u0 = lds<T>(U + c * a.fs + si);
+          const cplx<T> ac = lds<T>(ACC + c * a.fs + si);
           const R cst = (a.mode == SM_RK4_2) ? R(0.5) * dt : dt;
-          ACC[c * a.fs + si] = mk<T>((T)((R)ac.x + dt3 * kr), (T)((R)ac.y + dt3 * ki));
-          UST[c * a.fs + si] = mk<T>((T)((R)u0.x + cst * kr), (T)((R)u0.y + cst * ki));
+          sts<T>(ACC + c * a.fs + si, mk<T>((T)((R)ac.x + dt3 * kr), (T)((R)ac.y + dt3 * ki)));
+          sts<T>(UST + c * a.fs + si, mk<T>((T)((R)u0.x + cst * kr), (T)((R)u0.y + cst * ki)));
         } else {
-          const cplx<T> ac = ACC[c * a.fs + si];
+          const cplx<T> ac = lds<T>(ACC + c * a.fs + si);
           unr[c] = ((R)ac.x + dt6 * kr) * kill;
           uni[c] = ((R)ac.y + dt6 * ki) * kill;
         }
@@ -220,7 +241,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
     }
     if (a.mode == SM_RK2_2 || a.mode == SM_RK4_4 || (kx == 0 && ky == 0 && kzi == 0)) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) U[c * a.fs + si] = mk<T>((T)unr[c], (T)uni[c]);
+      for (int c = 0; c < 3; ++c) sts<T>(U + c * a.fs + si, mk<T>((T)unr[c], (T)uni[c]));
     }
     // integrals.f90:522-552 Parseval sum of the stored (rounded) field
     double ev = 0.0;
