@@ -14,6 +14,25 @@ __host__ __device__ constexpr int pick_nt(int n) {
   return (n / 8) >= 1024 ? 1024 : ((n / 8) <= 64 ? 64 : ((n / 8 + 63) / 64) * 64);
 }
 
+// Streamed complex elements (each read or written once per launch):
+// non-temporal accesses keep them from crowding the caches.
+template <typename T> __device__ __forceinline__ cplx<T> lds(const cplx<T>* p) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(cplx<T>, __builtin_nontemporal_load(reinterpret_cast<const double*>(p)));
+  } else {
+    using V = long long __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(cplx<T>, __builtin_nontemporal_load(reinterpret_cast<const V*>(p)));
+  }
+}
+template <typename T> __device__ __forceinline__ void sts(cplx<T>* p, cplx<T> v) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_nontemporal_store(__builtin_bit_cast(double, v), reinterpret_cast<double*>(p));
+  } else {
+    using V = long long __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(__builtin_bit_cast(V, v), reinterpret_cast<V*>(p));
+  }
+}
+
 template <typename C> __device__ __forceinline__ C conj_(C a) { a.y = -a.y; return a; }
 template <typename T> __device__ __forceinline__ cplx<T> ik_(T k, cplx<T> v) { return mk<T>(-k * v.y, k * v.x); }
 

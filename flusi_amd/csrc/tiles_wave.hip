@@ -82,7 +82,7 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
       const long base = c * a.sin.c + (long)(b0 + line) * a.sin.b + l;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const cplx<T> x = ok ? in[base + q * G] : mk<T>(T(0), T(0));
+        const cplx<T> x = ok ? lds<T>(in + base + q * G) : mk<T>(T(0), T(0));
         if (which) r1[q] = x; else r0[q] = x;
       }
     } else {
@@ -90,7 +90,7 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
       for (int k = 0; k < K; ++k) {
         const int e = tid + k * NT, bb = e % TL, aa = e / TL;
         const bool ok = b0 + bb < a.nb;
-        const cplx<T> x = ok ? in[c * a.sin.c + (long)(b0 + bb) + (long)aa * a.sin.a] : mk<T>(T(0), T(0));
+        const cplx<T> x = ok ? lds<T>(in + c * a.sin.c + (long)(b0 + bb) + (long)aa * a.sin.a) : mk<T>(T(0), T(0));
         if (which) r1[k] = x; else r0[k] = x;
       }
     }
@@ -119,7 +119,7 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
         for (int k = 0; k < K; ++k) {
           const int e = tid + k * NT, bb = e % TL, aa = e / TL;
           const cplx<T> x = tile[bb * LS + lp<T>(aa)];
-          if (b0 + bb < nbs) out[c * a.sout.c + (long)(b0 + bb) + (long)aa * a.sout.a] = mk<T>(x.x * sc, x.y * sc);
+          if (b0 + bb < nbs) sts<T>(out + c * a.sout.c + (long)(b0 + bb) + (long)aa * a.sout.a, mk<T>(x.x * sc, x.y * sc));
         }
         __syncthreads();
       } else {
@@ -137,7 +137,7 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
         if (b0 + line < a.nb) {
           const long base = c * a.sout.c + (long)(b0 + line) * a.sout.b + l;
 #pragma unroll
-          for (int q = 0; q < 8; ++q) out[base + q * G] = mk<T>(v[q].x * sc, v[q].y * sc);
+          for (int q = 0; q < 8; ++q) sts<T>(out + base + q * G, mk<T>(v[q].x * sc, v[q].y * sc));
         }
         __syncthreads();
       }

@@ -21,26 +21,6 @@ __device__ __forceinline__ double vis_factor(double dtv, double nu, double k2, b
   return exp(-dtv * nu * k2);
 }
 
-// The epilogue's state reads and writes stream (each element is touched once
-// per launch): non-temporal accesses keep them from crowding the caches that
-// the transform's operands go through.
-template <typename T> __device__ __forceinline__ cplx<T> lds(const cplx<T>* p) {
-  if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(cplx<T>, __builtin_nontemporal_load(reinterpret_cast<const double*>(p)));
-  } else {
-    using V = long long __attribute__((ext_vector_type(2)));
-    return __builtin_bit_cast(cplx<T>, __builtin_nontemporal_load(reinterpret_cast<const V*>(p)));
-  }
-}
-template <typename T> __device__ __forceinline__ void sts(cplx<T>* p, cplx<T> v) {
-  if constexpr (sizeof(T) == 4) {
-    __builtin_nontemporal_store(__builtin_bit_cast(double, v), reinterpret_cast<double*>(p));
-  } else {
-    using V = long long __attribute__((ext_vector_type(2)));
-    __builtin_nontemporal_store(__builtin_bit_cast(V, v), reinterpret_cast<V*>(p));
-  }
-}
-
 // Stepper modes are grouped so that each kernel carries only the update code
 // of its own stepper (the epilogue is unrolled over a thread's elements).
 enum { MG_RHS = 0, MG_AB2 = 1, MG_RK2 = 2, MG_RK4 = 3 };
