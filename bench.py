@@ -64,8 +64,9 @@ def pass_bytes(n, s, stepper="RK4"):
         "YINV": 11 * C,           # 5 in, 6 out
         "XPASS": 9 * C + R,       # 6 in, 3 out, mask (us only inside the mask)
         "YFWD": 6 * C,            # 3 in, 3 out
-        # RK4: average of 12/18/18/12 C; RK2: 12 C in both stages
-        "ZSTEP": (15 if stepper == "RK4" else 12) * C,
+        # RK4: average of 12/15/18/12 C (stage 2 rebuilds u_n from its two
+        # stage-1 registers); RK2: 12 C in both stages
+        "ZSTEP": (14.25 if stepper == "RK4" else 12) * C,
     }
 
 

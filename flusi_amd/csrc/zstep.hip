@@ -198,11 +198,21 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
           sts<T>(ACC + c * a.fs + si, mk<T>((T)((R)u0.x + dt6 * kr), (T)((R)u0.y + dt6 * ki)));
           sts<T>(UST + c * a.fs + si, mk<T>((T)((R)u0.x + R(0.5) * dt * kr), (T)((R)u0.y + R(0.5) * dt * ki)));
         } else if (a.mode == SM_RK4_2 || a.mode == SM_RK4_3) {
-          const cplx<T> u0 = lds<T>(U + c * a.fs + si);
           const cplx<T> ac = lds<T>(ACC + c * a.fs + si);
+          R u0r, u0i;
+          if (a.mode == SM_RK4_2 && a.explicit_diffusion && a.udiff == a.ustage) {
+            // stage 1 left acc = u_n + dt/6 k1 and ustage = u_n + dt/2 k1 (the
+            // diffusion operand held in udv), so u_n = acc + (acc - ustage)/2
+            // without a read of u_n
+            u0r = (R)ac.x + R(0.5) * ((R)ac.x - (R)udv[c].x);
+            u0i = (R)ac.y + R(0.5) * ((R)ac.y - (R)udv[c].y);
+          } else {
+            const cplx<T> u0 = lds<T>(U + c * a.fs + si);
+            u0r = (R)u0.x; u0i = (R)u0.y;
+          }
           const R cst = (a.mode == SM_RK4_2) ? R(0.5) * dt : dt;
           sts<T>(ACC + c * a.fs + si, mk<T>((T)((R)ac.x + dt3 * kr), (T)((R)ac.y + dt3 * ki)));
-          sts<T>(UST + c * a.fs + si, mk<T>((T)((R)u0.x + cst * kr), (T)((R)u0.y + cst * ki)));
+          sts<T>(UST + c * a.fs + si, mk<T>((T)(u0r + cst * kr), (T)(u0i + cst * ki)));
         } else {
           const cplx<T> ac = lds<T>(ACC + c * a.fs + si);
           unr[c] = ((R)ac.x + dt6 * kr) * kill;
