@@ -305,7 +305,7 @@ def main():
         print(json.dumps({"copy_bytes_per_launch": a.numel() * 4, "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
+        print(f"bench.py runs on one GPU only: the solver keeps the whole domain on one GPU and has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
