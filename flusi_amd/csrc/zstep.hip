@@ -5,7 +5,8 @@
 // One workgroup transforms TL (kx, ky) lines of length nz for each of the 3
 // (6 with the sponge) Q-layout fields; the last Stockham stage leaves every
 // thread with the same (ky, kz) positions of all fields, where the epilogue
-// runs in FP64 and writes the S-layout state.
+// runs (pointwise terms in the field precision; dealias predicate, integrating
+// factors and Parseval sum in FP64) and writes the S-layout state.
 #include "fft_core.h"
 #include "flusi_kernels.h"
 #include "pass_common.h"
