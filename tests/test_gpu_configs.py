@@ -4,7 +4,8 @@ against the FP64 oracle:
 * the penalised sphere in the 4 x 2 x 2 box with the outlet_x vorticity
   sponge, fixed mean flow and CFL-adapted RK2 (flusi_amd/configs/
   cfg2_sphere_256x128x128.ini): steps at reduced resolution in both
-  precisions, and one right-hand side at the full 256 x 128 x 128;
+  precisions, one right-hand side at the full 256 x 128 x 128, and one on a
+  64 x 64 x 512 grid (the six-field z step at nz = 512);
 * the 64^3 Gaussian-core vortex ring with AB2 (cfg1_vortex_ring_64.ini).
 """
 import os
@@ -73,6 +74,26 @@ def test_sphere_full_size_rhs_fp64(tmp_path):
         err = rel_l2(s.cal_nlk(0.0), nlk_ref)
     print(f"sphere 256x128x128 cal_nlk FP64 rel L2 {err:.2e}", flush=True)
     assert err < 1e-11
+
+
+@pytest.mark.parametrize("prec,tol", [(flusi_amd.FP64, 1e-11), (flusi_amd.FP32_MIXED, 1e-5)])
+def test_sphere_sponge_rhs_nz512(tmp_path, prec, tol):
+    """One right-hand side of the sphere workload with the outlet_x sponge on a
+    64 x 64 x 512 grid: the z step's six-field (sponge) variant at nz = 512,
+    which runs with spilled registers in both precisions."""
+    ini = rewrite_ini(os.path.join(CONFIGS, "cfg2_sphere_256x128x128.ini"), str(tmp_path),
+                      Resolution__nx=64, Resolution__ny=64, Resolution__nz=512, Sponge__sponge_thickness=7,
+                      Saving__tintegral=100.0, Saving__itdrag=100000)
+    sim = fo.Simulation(ini, workers=8)
+    sim.init_fields()
+    uk = _perturbed(sim)
+    nlk_ref, _ = sim.cal_nlk(0.0, uk)
+    with flusi_amd.Flusi(ini, prec) as s:
+        s.init_fields()
+        s.set_uk(uk)
+        err = rel_l2(s.cal_nlk(0.0), nlk_ref)
+    print(f"sphere sponge 64x64x512 cal_nlk precision {prec}: rel L2 {err:.2e}", flush=True)
+    assert err < tol
 
 
 def test_vortex_ring_64_ab2_steps_fp64(tmp_path):
