@@ -369,7 +369,7 @@ void Insect::init(const IniFile& ini_c, const Params& p, const std::string& bdir
   b_body = ini.dbl("Insects", "b_body", 0.1);
   L_body = ini.dbl("Insects", "L_body", 1.0);
   R_head = ini.dbl("Insects", "R_head", 0.1);
-  R_eye = ini.dbl("Insects", "R_eye", 1.0);
+  R_eye = ini.dbl("Insects", "R_eye", 0.0);  // insect_init_clean.f90:142 default 0.d1 = 0
   WingThickness = ini.dbl("Insects", "WingThickness", 4.0 * dxr);
   auto v3 = [](const std::vector<double>& v) { return Vec3{v[0], v[1], v[2]}; };
   x0 = v3(ini.vct("Insects", "x0", {0.5 * p.xl, 0.5 * p.yl, 0.5 * p.zl}));

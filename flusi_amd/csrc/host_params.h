@@ -130,7 +130,7 @@ struct FourierWing {
 struct Insect {
   std::string WingShape, BodyType, BodyMotion, LeftWing, RightWing, HasDetails;
   FourierWing fwing;
-  double b_top = 0, b_bot = 0, L_span = 0, b_body = 0.1, L_body = 1.0, R_head = 0.1, R_eye = 1.0;
+  double b_top = 0, b_bot = 0, L_span = 0, b_body = 0.1, L_body = 1.0, R_head = 0.1, R_eye = 0.0;
   double WingThickness = 0, eta0 = 0, Jxx = 0, Jyy = 0, Jzz = 0, Jxy = 0, smooth = 0, safety = 0;
   Vec3 x0, yawpitchroll_0, x_head, x_eye_r, x_eye_l, x_pivot_l_b, x_pivot_r_b;
   WingKinematics kine_l, kine_r;

@@ -20,11 +20,13 @@ pinned by the closed-form checks of the reference's own start-up self test
 (fft_unit_test.f90:81-253) and brute-force DFTs in tests/test_oracle_core.py.
 
 Each function is pinned by the tests in tests/test_oracle_core.py (closed
-forms, brute force, invariants) and tests/test_oracle_fixtures.py (the
-reference's own fixtures); functions without an independent check say
-"parity unpinned" in their docstring: the outlet_x sponge region
-(penalize_vort), the ellipsoid_plain body (_draw_body, a build extension)
-and the sinusoidal mean flow (set_mean_flow).
+forms, brute force, invariants), tests/test_oracle_fixtures.py (the
+reference's own fixtures) and tests/test_oracle_pinning.py (the branches no
+fixture runs: the outlet_x sponge region against the fixture-pinned cavity
+predicate, its first column and its single-precision mask threshold; the
+ellipsoid_plain body against the insects_mask_jerry keys and the ellipsoid's
+volume; the sinusoidal mean flow's zero mode against the step's start time).
+No function is left without an independent check.
 """
 from __future__ import annotations
 
@@ -706,7 +708,7 @@ class Insect:
         self.b_body = ini.dbl("Insects", "b_body", 0.1)
         self.L_body = ini.dbl("Insects", "L_body", 1.0)
         self.R_head = ini.dbl("Insects", "R_head", 0.1)
-        self.R_eye = ini.dbl("Insects", "R_eye", 1.0)
+        self.R_eye = ini.dbl("Insects", "R_eye", 0.0)  # insect_init_clean.f90:142: 0.d1 = 0
         self.WingThickness = ini.dbl("Insects", "WingThickness", 4.0 * dxr)
         self.x0 = ini.vct("Insects", "x0", [0.5 * p.xl, 0.5 * p.yl, 0.5 * p.zl])
         self.yawpitchroll_0 = ini.vct("Insects", "yawpitchroll_0", [0.0, 0.0, 0.0]) * (PI / 180.0)
@@ -855,8 +857,10 @@ class Insect:
     def _draw_body(self, g, mask, color, us):
         """body_geometry.f90:5-122 draw_insect_body; :511-595 draw_body_jerry.
         'ellipsoid_plain' (build extension) is Jerry's ellipsoid with b_body and
-        L_body from the ini file and no head or eyes; it has no reference
-        counterpart and is parity unpinned beyond sharing Jerry's formula."""
+        L_body from the ini file and no head or eyes.  It is pinned by
+        tests/test_oracle_pinning.py: with Jerry's values in the file and his
+        head and eyes added it reproduces the insects_mask_jerry keys, and its
+        volume converges to (4/3) pi a b^2 at second order."""
         bt = self.BodyType
         if bt == "nobody":
             return
@@ -1223,9 +1227,10 @@ class Simulation:
     # -- sponge ----------------------------------------------------------------
     def penalize_vort(self, vort_c: np.ndarray) -> np.ndarray:
         """sponge.f90:94-329 penalize_vort (cavity, outlet_x).  The cavity
-        region is pinned by the sphere_sponge_RK2 fixture; the outlet_x region
-        predicate (sponge.f90 outlet_x branch) is parity unpinned: no fixture
-        of the reference uses it."""
+        region is pinned by the sphere_sponge_RK2 fixture; outlet_x
+        (sponge.f90:262-277), which no fixture runs, by
+        tests/test_oracle_pinning.py: its first column, its agreement with the
+        cavity predicate's upper-x clause and its default-real 1e-12."""
         p, g = self.p, self.g
         th = p.sponge_thickness
         eps_inv = 1.0 / p.eps_sponge
@@ -1404,8 +1409,10 @@ class Simulation:
 
     def set_mean_flow(self, time):
         """fluid_time_step.f90:1066-1090.  The fixed mean flow is pinned by the
-        sphere fixtures; the sinusoidal branch is parity unpinned (no fixture
-        uses it)."""
+        sphere fixtures; the sinusoidal branch (no fixture uses it) by
+        tests/test_oracle_pinning.py: the zero mode after each step is
+        U sin(2 pi f t) at the step's start time, the rest of a uniform flow
+        stays zero."""
         p = self.p
         for i in range(3):
             if p.iMeanFlow[i] == "fixed":
@@ -1714,8 +1721,15 @@ def read_timeseries(path: str) -> List[List[float]]:
 
 
 def compare_timeseries(present: List[List[float]], reference: List[List[float]]) -> Tuple[bool, float]:
-    """compare_timeseries.f90:60-126: per entry, relative error where
-    |ref| > 1e-4 and |diff| > 1e-7 (else 0); fail if any exceeds 1e-4."""
+    """compare_timeseries.f90:30-126: an empty file aborts 107712 (:38-39,
+    :51-52); files whose first rows have different column counts fail
+    (:63-76); then, row by row until either file ends, per entry the relative
+    error where |ref| > 1e-4 and |diff| > 1e-7 (else 0); fail if any exceeds
+    1e-4."""
+    if not present or not reference:
+        raise FlusiAbort(107712, "I/O error in ascii file (Maybe empty?)")
+    if len(present[0]) != len(reference[0]):
+        return False, float("inf")
     worst = 0.0
     for r1, r2 in zip(present, reference):
         for a, b in zip(r1, r2):

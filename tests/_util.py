@@ -59,3 +59,38 @@ def rel_l2(a, b) -> float:
     b = np.asarray(b)
     nb = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))
+
+
+def series_ok(got, ref_path: str):
+    """A written time series (rows, or the path of a .t file) against the
+    reference's file under compare_timeseries.f90's rule: both non-empty, the
+    same column count, no more than the one row fewer that a run ending at
+    Tmax by another dt may leave, then the row-by-row 1e-4 rule over the
+    common rows.  Returns (ok, worst relative error, message)."""
+    from oracle import flusi_oracle as fo
+    ref = fo.read_timeseries(ref_path)
+    if isinstance(got, str):
+        got = fo.read_timeseries(got)
+    if not got or not ref:
+        return False, float("inf"), f"empty series ({len(got)} rows written, {len(ref)} in the reference)"
+    if len(got) < len(ref) - 1:
+        return False, float("inf"), f"{len(got)} rows written, reference has {len(ref)}"
+    if len(got[0]) != len(ref[0]):
+        return False, float("inf"), f"{len(got[0])} columns written, reference has {len(ref[0])}"
+    n = min(len(got), len(ref))
+    ok, worst = fo.compare_timeseries(got[:n], ref[:n])
+    where = ""
+    for i in range(n):
+        for j, (a, b) in enumerate(zip(got[i], ref[i])):
+            d = a - b
+            if abs(b) > 1.0e-4 and abs(d) > 1.0e-7 and abs(d / b) == worst and worst > 0.0:
+                where = f" at row {i} (time {ref[i][0]:.8g}) column {j}: {a!r} vs {b!r}"
+    return ok, worst, f"worst relative error {worst:.3e} over {n} rows{where}"
+
+
+def check_err(err: float, bound: float, what: str = "") -> None:
+    """Asserts err < bound and prints the measured value with the test's name,
+    so that logs of GPU runs show how far each bound is from the error."""
+    name = os.environ.get("PYTEST_CURRENT_TEST", "?").split(" ")[0]
+    print(f"ERR {name} {what}: {err:.3e} (bound {bound:.1e})", flush=True)
+    assert err < bound, (what, err, bound)

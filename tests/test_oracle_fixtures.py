@@ -8,6 +8,7 @@ import os
 import pytest
 
 from oracle import flusi_oracle as fo
+from tests._util import series_ok
 
 
 def _ref_keys(d):
@@ -32,13 +33,8 @@ def _check_run(d, ini, series):
         ok, errs = fo.compare_key(mine[key], ref)
         assert ok, (key, errs)
     for ts in series:
-        ref = fo.read_timeseries(os.path.join(d, ts))
-        got = sim.series[ts]
-        n = min(len(ref), len(got))
-        assert n >= len(ref) - 1
-        ncol = min(len(ref[0]), len(got[0]))
-        ok, worst = fo.compare_timeseries([r[:ncol] for r in got[:n]], [r[:ncol] for r in ref[:n]])
-        assert ok, (ts, worst)
+        ok, worst, msg = series_ok(sim.series.get(ts, []), os.path.join(d, ts))
+        assert ok, (ts, msg)
     return sim
 
 
