@@ -62,6 +62,7 @@ EXPORTED: Dict[str, tuple] = {
     "flusi_create_mask": (_I, [_P, ctypes.c_double]),
     "flusi_get_mask": (_I, [_P, _D, _D, ctypes.POINTER(ctypes.c_int16)]),
     "flusi_write_integrals": (_I, [_P, ctypes.c_double, _D]),
+    "flusi_cal_drag": (_I, [_P, ctypes.c_double, _D, _D]),
     "flusi_energy_k": (_I, [_P, _D]),
     "flusi_cal_vis": (_I, [_P, ctypes.c_double, _D]),
     "flusi_fft": (_I, [_P, _D, _D]),
@@ -315,6 +316,15 @@ class Flusi:
     def write_integrals(self, time: float) -> np.ndarray:
         out = np.zeros(INTEGRALS_NV)
         self._check(self._lib.flusi_write_integrals(self._ctx, float(time), _dp(out)))
+        return out
+
+    def cal_drag(self, time: float, u: Optional[np.ndarray] = None) -> np.ndarray:
+        """drag.f90:17-454 cal_drag of the physical velocity u ((3, nz, ny, nx)
+        float64; None: the velocity of the current uk) with the current mask:
+        the INTEGRALS_NV layout of write_integrals, force slots only."""
+        out = np.zeros(INTEGRALS_NV)
+        a = None if u is None else np.ascontiguousarray(u, dtype=np.float64).reshape(self.physical_shape)
+        self._check(self._lib.flusi_cal_drag(self._ctx, float(time), _dp(a) if a is not None else None, _dp(out)))
         return out
 
     def energy_k(self) -> float:

@@ -19,6 +19,10 @@ template <> struct V2<double> { using t = double2; };
 template <typename T> using cplx = typename V2<T>::t;
 
 template <typename T> __device__ __forceinline__ cplx<T> mk(T x, T y) { cplx<T> r; r.x = x; r.y = y; return r; }
+// Maximum that keeps a NaN: fmax drops a NaN operand, while the reference's
+// maxval over a field with NaN entries is NaN, which adjust_dt turns into
+// abort 100011 (fluid_time_step.f90:947-950).  A NaN operand wins.
+__device__ __forceinline__ double nanmax(double a, double b) { return (b > a || b != b) ? b : a; }
 __device__ __forceinline__ float2 add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }

@@ -74,6 +74,7 @@ struct DiagArgs {
   double x0[3];                    // torque reference point
   double piv_l[3], piv_r[3];       // insect pivots (global frame)
   int insect;                      // colours 2/3 use the pivots
+  int want_w;                      // 0: w and div are not read (cal_drag's sums only)
   double* partial;                 // per block DIAG_NV doubles
 };
 enum {

@@ -454,7 +454,7 @@ __device__ __forceinline__ double wred(double x, int kind) {  // 0 sum, 1 max, 2
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const double y = __shfl_xor(x, o, 64);
-    x = kind == 2 ? fmin(x, y) : (kind == 1 ? fmax(x, y) : x + y);
+    x = kind == 2 ? fmin(x, y) : (kind == 1 ? nanmax(x, y) : x + y);
   }
   return x;
 }
@@ -480,7 +480,8 @@ __global__ __launch_bounds__(DIAG_NT) void diag_k(GridGeom g, DiagArgs a) {
     const long pp = live ? p : 0;
     const double m = live ? (double)mask[pp] : 0.0;
     const double uu[3] = {(double)u[pp], (double)u[n + pp], (double)u[2 * n + pp]};
-    const double ww[3] = {(double)w[pp], (double)w[n + pp], (double)w[2 * n + pp]};
+    const double ww[3] = {a.want_w ? (double)w[pp] : 0.0, a.want_w ? (double)w[n + pp] : 0.0,
+                          a.want_w ? (double)w[2 * n + pp] : 0.0};
     const double ss[3] = {(double)us[pp], (double)us[n + pp], (double)us[2 * n + pp]};
     const double du[3] = {uu[0] - ss[0], uu[1] - ss[1], uu[2] - ss[2]};
     const double pen[3] = {-m * du[0] * a.eps_inv, -m * du[1] * a.eps_inv, -m * du[2] * a.eps_inv};
@@ -495,9 +496,9 @@ __global__ __launch_bounds__(DIAG_NT) void diag_k(GridGeom g, DiagArgs a) {
         dsf[i] += (ww[i] * fm) * (ww[i] * fm);
         mus[i] += m * ss[i];
       }
-      const double d = (double)dv[pp];
-      dmax = fmax(dmax, d);
-      dmaxf = fmax(dmaxf, d * fm);
+      const double d = a.want_w ? (double)dv[pp] : 0.0;
+      dmax = nanmax(dmax, d);
+      dmaxf = nanmax(dmaxf, d * fm);
       mvol += m;
     }
     if (__any(m > 0.0)) {
@@ -562,7 +563,7 @@ __global__ __launch_bounds__(DIAG_NT) void diag_k(GridGeom g, DiagArgs a) {
     double r = colv ? colacc[0][v < 36 ? v : 36 + (v - DG_URES)] : red[0][v];
     for (int i = 1; i < NW; ++i) {
       const double x = colv ? colacc[i][v < 36 ? v : 36 + (v - DG_URES)] : red[i][v];
-      r = is_min ? fmin(r, x) : (is_max ? fmax(r, x) : r + x);
+      r = is_min ? fmin(r, x) : (is_max ? nanmax(r, x) : r + x);
     }
     a.partial[(long)blockIdx.x * DIAG_NV + v] = r;
   }
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(RED_NT) void reduce_rows_p1(const double* partial, 
   double s = (kind == 1) ? -1.0e308 : (kind == 2 ? 1.0e308 : 0.0), comp = 0.0;
   for (int r = r0 + tid; r < r1; r += RED_NT) {
     const double x = partial[(long)r * nv + v];
-    if (kind == 1) s = fmax(s, x);
+    if (kind == 1) s = nanmax(s, x);
     else if (kind == 2) s = fmin(s, x);
     else neu_add(s, comp, x);
   }
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(RED_NT) void reduce_rows_p1(const double* partial, 
   __syncthreads();
   for (int w = RED_NT / 2; w > 0; w >>= 1) {
     if (tid < w) {
-      if (kind == 1) sv[tid] = fmax(sv[tid], sv[tid + w]);
+      if (kind == 1) sv[tid] = nanmax(sv[tid], sv[tid + w]);
       else if (kind == 2) sv[tid] = fmin(sv[tid], sv[tid + w]);
       else {
         double a = sv[tid], c = sc[tid] + sc[tid + w];
@@ -632,7 +633,7 @@ __global__ void reduce_rows_p2(const double* mid, int nv, const int* kinds, doub
   double s = mid[(long)v * RED_CHUNKS * 2], c = mid[(long)v * RED_CHUNKS * 2 + 1];
   for (int b = 1; b < RED_CHUNKS; ++b) {
     const double x = mid[(long)v * RED_CHUNKS * 2 + 2 * b];
-    if (kind == 1) s = fmax(s, x);
+    if (kind == 1) s = nanmax(s, x);
     else if (kind == 2) s = fmin(s, x);
     else { c += mid[(long)v * RED_CHUNKS * 2 + 2 * b + 1]; neu_add(s, c, x); }
   }

@@ -129,6 +129,7 @@ Params load_params(const IniFile& ini_c) {
   p.cfl_eta = ini.dbl("Time", "CFL_eta", 0.99);
   p.dt_max = ini.dbl("Time", "dt_max", 0.0);
   p.dt_fixed = ini.dbl("Time", "dt_fixed", 0.0);
+  p.checknan_every = ini.integer("Time", "checknan_every", 0);
   p.nu = ini.dbl("ReynoldsNumber", "nu", 1e-2);
   p.inicond = ini.str("InitialCondition", "inicond", "none");
   p.omega1 = ini.dbl("InitialCondition", "omega1", 0.0);

@@ -44,6 +44,9 @@ struct Params {
   std::string method = "AB2";
   std::string intelligent_dt = "no";
   double tmax = 1e9, cfl = 0.1, cfl_eta = 0.99, dt_max = 0.0, dt_fixed = 0.0;
+  // build extension: check uk for NaN after every N-th step (0: never), the
+  // reference's checknan_cmplx (basic_operators.f90:925-950, abort 202)
+  int checknan_every = 0;
   double nu = 1e-2;
   std::string inicond = "none";
   double omega1 = 0.0, core_a = 0.0;

@@ -52,7 +52,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
   return v;
 }
 

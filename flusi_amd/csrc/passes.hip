@@ -411,7 +411,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
                 fc[c] += (fabs(fs[c]) >= fabs(pv[c])) ? (fs[c] - t) + pv[c] : (pv[c] - t) + fs[c];
                 fs[c] = t;
               }
-              umax2 = fmax(umax2, ux * ux + uy * uy + uz * uz);
+              umax2 = nanmax(umax2, ux * ux + uy * uy + uz * uz);
             }
             nv[0][h] = uy * wz - uz * wy - pv[0];
             nv[1][h] = uz * wx - ux * wz - pv[1];
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
       if (tid == 0) {
         double r[4] = {red[0][0], red[0][1], red[0][2], red[0][3]};
         for (int w = 1; w < NT / 64; ++w) {
-          r[0] = fmax(r[0], red[w][0]);
+          r[0] = nanmax(r[0], red[w][0]);
           for (int c = 1; c < 4; ++c) r[c] += red[w][c];
         }
         for (int c = 0; c < 4; ++c) a.partial[(long)blockIdx.x * 4 + c] = r[c];

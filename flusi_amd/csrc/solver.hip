@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -102,6 +103,7 @@ class SolverBase {
   virtual void create_mask(double time) = 0;
   virtual void get_mask(double* mask, double* us, int16_t* color) = 0;
   virtual void write_integrals(double time, double* out) = 0;
+  virtual void cal_drag(double time, const double* u, double* out) = 0;
   virtual double energy_k() = 0;
   virtual void cal_vis(double dt, double* vis) = 0;
   virtual void fft1(const double* f, double* fk) = 0;
@@ -131,12 +133,27 @@ class SolverBase {
            (p.iSaveSolidVelocity == 1 && p.iPenalization == 1 && p.iMoving == 1);
   }
 
+  // wall clock of the time-stepping loop's start (time_step.f90:53, t4)
+  std::chrono::steady_clock::time_point wtime_start{};
+  bool wtime_set = false;
+  virtual void check_nan() = 0;
+
   // time_step.f90:57-167 loop body
   void step() {
+    if (!wtime_set) { wtime_start = std::chrono::steady_clock::now(); wtime_set = true; }
     dt0 = dt1;
     fluid_timestep();
     time = time + dt1;
     it += 1;
+    if (p.checknan_every > 0 && it % p.checknan_every == 0) check_nan();
+    // time_step.f90:163, 189-201 save_time_stepping_info: time, wall time
+    // since the loop started, it, dt, number of MPI ranks (one)
+    if (!out_dir.empty()) {
+      const double wt = std::chrono::duration<double>(std::chrono::steady_clock::now() - wtime_start).count();
+      char line[160];
+      std::snprintf(line, sizeof(line), "%15.8E %15.8E %9d %15.8E %6d\n", time, wt, it, dt1, 1);
+      append_line("timesteps_info.t", line);
+    }
     if (time_for_output(time, dt1, it, p.tintegral, p.itdrag, p.tmax, 0.0, p.intelligent_dt == "yes"))
       write_integrals(time, nullptr);
     // time_step.f90:97-105 field snapshots (written only with an output directory)
@@ -159,6 +176,8 @@ class SolverBase {
       write_integrals(time, nullptr);
     }
     int n = 0;
+    wtime_start = std::chrono::steady_clock::now();
+    wtime_set = true;
     while (time < p.tmax && it <= p.nt && (nsteps <= 0 || n < nsteps)) {
       step();
       ++n;
@@ -168,6 +187,14 @@ class SolverBase {
     if (backups) dump_runtime_backup();
   }
 
+  void append_line(const std::string& name, const char* line) {
+    if (out_dir.empty()) return;
+    const std::string path = out_dir + "/" + name;
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) throw FlusiError(900010, "cannot open " + path);
+    std::fputs(line, f);
+    std::fclose(f);
+  }
   void append(const std::string& name, const std::vector<double>& row, const char* header) {
     if (out_dir.empty()) return;
     const std::string path = out_dir + "/" + name;
@@ -615,6 +642,19 @@ class Solver : public SolverBase {
     HIPCHECK(hipMemcpyAsync(&e, red + 8, sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     append("ekin.t", {time + dt1, e * p.xl * p.yl * p.zl}, "%          time            ekin");
+  }
+
+  // basic_operators.f90:925-950 checknan_cmplx on uk, from the Parseval sum
+  // the last z step of a step accumulates over every entry of the new uk (a
+  // NaN or Inf anywhere makes it non-finite)
+  void check_nan() override {
+    need_fft();
+    launch_reduce_rows(zpart, zstep_blocks(nz, ny, nxh, FP64, zwave), 1, kinds + 4, rmid, red + 10, stream);
+    kcheck("checknan");
+    double e = 0.0;
+    HIPCHECK(hipMemcpyAsync(&e, red + 10, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (!std::isfinite(e)) throw FlusiError(202, "NaN in uk after time step " + std::to_string(it));
   }
 
   // ------------------------------------------------------------ autotuning
@@ -1809,6 +1849,105 @@ class Solver : public SolverBase {
     HIPCHECK(hipStreamSynchronize(stream));
   }
 
+  // drag.f90:17-454 and integrals.f90:48-238 reductions over the physical
+  // velocity u (and, with w, vorticity and divergence) with the current mask,
+  // us and colour: the DIAG_NV sums of the diagnostics kernel
+  std::vector<double> diag_sums(const void* u, const void* w, const void* div) {
+    DiagArgs d{};
+    d.u = u; d.w = w; d.div = div; d.want_w = w ? 1 : 0;
+    d.mask = mask; d.us = us; d.color = color;
+    d.eps_inv = 1.0 / p.eps;
+    // moments about (x0, y0, z0), which body_motion.f90:250-255 sets to the
+    // insect's body centre at every update
+    d.x0[0] = p.x0; d.x0[1] = p.y0; d.x0[2] = p.z0;
+    if (is_insect) for (int i = 0; i < 3; ++i) d.x0[i] = insect.st.xc[i];
+    d.insect = is_insect ? 1 : 0;
+    if (is_insect) for (int i = 0; i < 3; ++i) { d.piv_l[i] = insect.st.x_pivot_l_g[i]; d.piv_r[i] = insect.st.x_pivot_r_g[i]; }
+    if (!dpart) {
+      ndblocks = diag_blocks<T>(geom);
+      dpart = (double*)dalloc(sizeof(double) * DIAG_NV * ndblocks);
+    }
+    d.partial = dpart;
+    timed(TM_DIAG, [&] { launch_diag<T>(geom, d, stream); });
+    kcheck("diag");
+    launch_reduce_rows(dpart, ndblocks, DIAG_NV, kinds + 8, rmid, red + 16, stream);
+    kcheck("reduce diag");
+    std::vector<double> r(DIAG_NV);
+    HIPCHECK(hipMemcpyAsync(r.data(), red + 16, sizeof(double) * DIAG_NV, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    return r;
+  }
+
+  // drag.f90:336-453: forces, moments, penalisation power, residual velocity
+  // and (insects) aerodynamic and inertial power from the sums r into the
+  // packed integrals o (slots 0..9 and 32..60); with `forces` (cal_drag
+  // runs) the penalisation power is updated and the forces.t,
+  // forces_partN.t and u_residual.t rows are written, otherwise slots 6..9
+  // keep the last cal_drag's power
+  void drag_assemble(double t, const std::vector<double>& r, double* o, bool forces) {
+    const double dv = p.dx() * p.dy() * p.dz();
+    double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
+    for (int c = 1; c < 4; ++c)
+      for (int i = 0; i < 3; ++i) { F[i] += r[DG_FORCE + 3 * c + i] * dv; Tq[i] += r[DG_TORQUE0 + 3 * c + i] * dv; }
+    if (forces) {
+      double pw[3];
+      for (int i = 0; i < 3; ++i) pw[i] = r[DG_POWER + i] * dv;
+      last_power_total = pw[0] + pw[1] + pw[2];
+      last_power4_xyz = {pw[0], pw[1], pw[2]};
+    }
+    for (int i = 0; i < 3; ++i) { o[i] = F[i]; o[3 + i] = Tq[i]; }
+    o[6] = last_power_total; o[7] = last_power4_xyz[0]; o[8] = last_power4_xyz[1]; o[9] = last_power4_xyz[2];
+    for (int c = 1; c < 4; ++c)
+      for (int i = 0; i < 3; ++i) { o[32 + 3 * (c - 1) + i] = r[DG_FORCE + 3 * c + i] * dv; o[44 + 3 * (c - 1) + i] = r[DG_TORQUE + 3 * c + i] * dv; }
+    for (int c = 0; c < 6; ++c) o[53 + c] = std::sqrt(c < 4 ? r[DG_URES + c] : 9.0e9);
+    if (is_insect) {
+      // module_insects.f90:469-586 aero_power / inert_power
+      const InsectMotion& s = insect.st;
+      double ap_l = 0, ap_r = 0;
+      for (int i = 0; i < 3; ++i) { ap_l -= o[44 + 3 + i] * s.rot_rel_l_g[i]; ap_r -= o[44 + 6 + i] * s.rot_rel_r_g[i]; }
+      auto ipow = [&](const Vec3& rw, const Vec3& rdt) {
+        const Insect& I = insect;
+        const double a0 = I.Jxx * rdt[0] + I.Jxy * rdt[1], a1 = I.Jxy * rdt[0] + I.Jyy * rdt[1], a2 = I.Jzz * rdt[2];
+        const double b0 = I.Jxx * rw[0] + I.Jxy * rw[1], b1 = I.Jxy * rw[0] + I.Jyy * rw[1], b2 = I.Jzz * rw[2];
+        return rw[0] * (a0 + rw[1] * b2 - rw[2] * b1) + rw[1] * (a1 + rw[2] * b0 - rw[0] * b2) + rw[2] * (a2 + rw[0] * b1 - rw[1] * b0);
+      };
+      const double ip_l = ipow(s.rot_rel_l_w, s.rot_dt_l_w), ip_r = ipow(s.rot_rel_r_w, s.rot_dt_r_w);
+      o[59] = 0.0 + ap_l + ap_r;
+      o[60] = ip_r + ip_l;
+      if (!forces) return;
+      append("forces.t", {t, F[0], F[1], F[2], 0, 0, 0, Tq[0], Tq[1], Tq[2], 0, 0, 0, o[59], o[60]}, forces_header(true));
+      const double apc[3] = {0.0, ap_l, ap_r}, ipc[3] = {0.0, ip_l, ip_r};
+      for (int c = 1; c <= 3; ++c) {
+        std::vector<double> row = {t};
+        for (int i = 0; i < 3; ++i) row.push_back(o[32 + 3 * (c - 1) + i]);
+        row.insert(row.end(), {0, 0, 0});
+        for (int i = 0; i < 3; ++i) row.push_back(o[44 + 3 * (c - 1) + i]);
+        row.insert(row.end(), {0, 0, 0, apc[c - 1], ipc[c - 1]});
+        append("forces_part" + std::to_string(c) + ".t", row, forces_header(true));
+      }
+    } else {
+      if (!forces) return;
+      append("forces.t", {t, F[0], F[1], F[2], 0, 0, 0, Tq[0], Tq[1], Tq[2], 0, 0, 0}, forces_header(false));
+    }
+    append("u_residual.t", {t, o[53], o[54], o[55], o[56], o[57], o[58]}, "%          time     u_residual(0:5)");
+  }
+
+  // drag.f90:17-454 cal_drag(time, u, Insect) as its own entry point: the
+  // given physical velocity (host, reference layout, float64; null: the
+  // velocity of the current uk) with the mask, us and colour as they are (the
+  // caller redraws them with create_mask, as integrals.f90:87-100 does)
+  void cal_drag(double t, const double* u_host, double* out) override {
+    if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
+    double o[FLUSI_INTEGRALS_NV] = {0};
+    if (p.iPenalization == 1) {  // drag.f90:53: nothing to do without penalisation
+      ensure_phys();
+      if (u_host) copy_real_in(u_host, pf(0), 3 * npts);
+      else { need_fft(); ifft_fields(uk, 3, pf(0)); }
+      drag_assemble(t, diag_sums(pf(0), nullptr, nullptr), o, true);
+    }
+    if (out) std::memcpy(out, o, sizeof(o));
+  }
+
   // integrals.f90:48-238 write_integrals_fsi
   void write_integrals(double t, double* out) override {
     need_fft();
@@ -1823,39 +1962,14 @@ class Solver : public SolverBase {
     launch_divergence<T>(geom, kx_d, ky_d, kz_d, uk, dk, stream);
     kcheck("divergence");
     ifft_fields(dk, 1, pf(6));
-    DiagArgs d{};
-    d.u = pf(0); d.w = pf(3); d.div = pf(6); d.mask = mask; d.us = us; d.color = color;
-    d.eps_inv = 1.0 / p.eps;
-    // moments about (x0, y0, z0), which body_motion.f90:250-255 sets to the
-    // insect's body centre at every update
-    d.x0[0] = p.x0; d.x0[1] = p.y0; d.x0[2] = p.z0;
-    if (ins) for (int i = 0; i < 3; ++i) d.x0[i] = insect.st.xc[i];
-    d.insect = ins ? 1 : 0;
-    if (ins) for (int i = 0; i < 3; ++i) { d.piv_l[i] = insect.st.x_pivot_l_g[i]; d.piv_r[i] = insect.st.x_pivot_r_g[i]; }
-    d.partial = dpart;
-    timed(TM_DIAG, [&] { launch_diag<T>(geom, d, stream); });
-    kcheck("diag");
-    launch_reduce_rows(dpart, ndblocks, DIAG_NV, kinds + 8, rmid, red + 16, stream);
-    kcheck("reduce diag");
-    std::vector<double> r(DIAG_NV);
-    HIPCHECK(hipMemcpyAsync(r.data(), red + 16, sizeof(double) * DIAG_NV, hipMemcpyDeviceToHost, stream));
+    const std::vector<double> r = diag_sums(pf(0), pf(3), pf(6));
     std::vector<C> m0(3);
     for (int c = 0; c < 3; ++c)
       HIPCHECK(hipMemcpyAsync(&m0[c], cf(uk, c), sizeof(C), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     const double dv = p.dx() * p.dy() * p.dz();
     double o[FLUSI_INTEGRALS_NV] = {0};
-    double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
-    for (int c = 1; c < 4; ++c)
-      for (int i = 0; i < 3; ++i) { F[i] += r[DG_FORCE + 3 * c + i] * dv; Tq[i] += r[DG_TORQUE0 + 3 * c + i] * dv; }
-    if (forces) {
-      double pw[3];
-      for (int i = 0; i < 3; ++i) pw[i] = r[DG_POWER + i] * dv;
-      last_power_total = pw[0] + pw[1] + pw[2];
-      last_power4_xyz = {pw[0], pw[1], pw[2]};
-    }
-    for (int i = 0; i < 3; ++i) { o[i] = F[i]; o[3 + i] = Tq[i]; }
-    o[6] = last_power_total; o[7] = last_power4_xyz[0]; o[8] = last_power4_xyz[1]; o[9] = last_power4_xyz[2];
+    drag_assemble(t, r, o, forces);
     const double ekf[3] = {0.5 * dv * r[DG_EKF], 0.5 * dv * r[DG_EKF + 1], 0.5 * dv * r[DG_EKF + 2]};
     const double ek[3] = {0.5 * dv * r[DG_EK], 0.5 * dv * r[DG_EK + 1], 0.5 * dv * r[DG_EK + 2]};
     const double dsf[3] = {p.nu * dv * r[DG_DISSF], p.nu * dv * r[DG_DISSF + 1], p.nu * dv * r[DG_DISSF + 2]};
@@ -1868,39 +1982,6 @@ class Solver : public SolverBase {
     o[26] = r[DG_DIVMAX]; o[27] = r[DG_DIVMAXF];
     o[28] = r[DG_MASKVOL] * dv;
     for (int i = 0; i < 3; ++i) o[29 + i] = r[DG_MASKUS + i] * dv;
-    for (int c = 1; c < 4; ++c)
-      for (int i = 0; i < 3; ++i) { o[32 + 3 * (c - 1) + i] = r[DG_FORCE + 3 * c + i] * dv; o[44 + 3 * (c - 1) + i] = r[DG_TORQUE + 3 * c + i] * dv; }
-    for (int c = 0; c < 6; ++c) o[53 + c] = std::sqrt(c < 4 ? r[DG_URES + c] : 9.0e9);
-    if (ins) {
-      // module_insects.f90:469-586 aero_power / inert_power
-      const InsectMotion& s = insect.st;
-      double ap_l = 0, ap_r = 0;
-      for (int i = 0; i < 3; ++i) { ap_l -= o[44 + 3 + i] * s.rot_rel_l_g[i]; ap_r -= o[44 + 6 + i] * s.rot_rel_r_g[i]; }
-      auto ipow = [&](const Vec3& rw, const Vec3& rdt) {
-        const Insect& I = insect;
-        const double a0 = I.Jxx * rdt[0] + I.Jxy * rdt[1], a1 = I.Jxy * rdt[0] + I.Jyy * rdt[1], a2 = I.Jzz * rdt[2];
-        const double b0 = I.Jxx * rw[0] + I.Jxy * rw[1], b1 = I.Jxy * rw[0] + I.Jyy * rw[1], b2 = I.Jzz * rw[2];
-        return rw[0] * (a0 + rw[1] * b2 - rw[2] * b1) + rw[1] * (a1 + rw[2] * b0 - rw[0] * b2) + rw[2] * (a2 + rw[0] * b1 - rw[1] * b0);
-      };
-      const double ip_l = ipow(s.rot_rel_l_w, s.rot_dt_l_w), ip_r = ipow(s.rot_rel_r_w, s.rot_dt_r_w);
-      o[59] = 0.0 + ap_l + ap_r;
-      o[60] = ip_r + ip_l;
-      if (forces) {
-        append("forces.t", {t, F[0], F[1], F[2], 0, 0, 0, Tq[0], Tq[1], Tq[2], 0, 0, 0, o[59], o[60]}, forces_header(true));
-        const double apc[3] = {0.0, ap_l, ap_r}, ipc[3] = {0.0, ip_l, ip_r};
-        for (int c = 1; c <= 3; ++c) {
-          std::vector<double> row = {t};
-          for (int i = 0; i < 3; ++i) row.push_back(o[32 + 3 * (c - 1) + i]);
-          row.insert(row.end(), {0, 0, 0});
-          for (int i = 0; i < 3; ++i) row.push_back(o[44 + 3 * (c - 1) + i]);
-          row.insert(row.end(), {0, 0, 0, apc[c - 1], ipc[c - 1]});
-          append("forces_part" + std::to_string(c) + ".t", row, forces_header(true));
-        }
-      }
-    } else if (forces) {
-      append("forces.t", {t, F[0], F[1], F[2], 0, 0, 0, Tq[0], Tq[1], Tq[2], 0, 0, 0}, forces_header(false));
-    }
-    if (forces) append("u_residual.t", {t, o[53], o[54], o[55], o[56], o[57], o[58]}, "%          time     u_residual(0:5)");
     append("divu.t", {t, o[26], o[27]}, "%          time         max_div   max_div_fluid");
     std::vector<double> erow = {t};
     for (int i = 10; i < 26; ++i) erow.push_back(o[i]);
@@ -2419,6 +2500,9 @@ int flusi_get_mask(flusi_ctx* ctx, double* mask, double* us, int16_t* color) {
 }
 int flusi_write_integrals(flusi_ctx* ctx, double time, double* out) {
   return guarded(ctx, [&] { ctx->s->write_integrals(time, out); });
+}
+int flusi_cal_drag(flusi_ctx* ctx, double time, const double* u, double* out) {
+  return guarded(ctx, [&] { ctx->s->cal_drag(time, u, out); });
 }
 int flusi_energy_k(flusi_ctx* ctx, double* E) { return guarded(ctx, [&] { *E = ctx->s->energy_k(); }); }
 int flusi_cal_vis(flusi_ctx* ctx, double dt, double* vis) { return guarded(ctx, [&] { ctx->s->cal_vis(dt, vis); }); }

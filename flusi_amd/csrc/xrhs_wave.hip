@@ -48,7 +48,7 @@ __device__ __forceinline__ double wsum(double v) {
 }
 __device__ __forceinline__ double wmax(double v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
           fc[c] += (fabs(fs[c]) >= fabs(pv[c])) ? (fs[c] - t) + pv[c] : (pv[c] - t) + fs[c];
           fs[c] = t;
         }
-        umax2 = fmax(umax2, ux * ux + uy * uy + uz * uz);
+        umax2 = nanmax(umax2, ux * ux + uy * uy + uz * uz);
       }
       if constexpr (sizeof(T) == 4) {
         // FP32-mixed: u x w in FP32, the precision of the transforms that
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
     if (threadIdx.x == 0) {
       double r[4] = {red[0][0], red[0][1], red[0][2], red[0][3]};
       for (int w = 1; w < WPB; ++w) {
-        r[0] = fmax(r[0], red[w][0]);
+        r[0] = nanmax(r[0], red[w][0]);
         for (int c = 1; c < 4; ++c) r[c] += red[w][c];
       }
       for (int c = 0; c < 4; ++c) a.partial[(long)blockIdx.x * 4 + c] = r[c];

@@ -157,6 +157,15 @@ module flusi_hip
       real(c_double), dimension(*), intent(out) :: out
     end function
 
+    ! drag.f90:17 cal_drag(time,u,Insect); u may be c_null_ptr (ifft3 of uk)
+    integer(c_int) function flusi_cal_drag(ctx, time, u, out) bind(C, name="flusi_cal_drag")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: time
+      type(c_ptr), value :: u
+      real(c_double), dimension(*), intent(out) :: out
+    end function
+
     integer(c_int) function flusi_energy_k(ctx, e) bind(C, name="flusi_energy_k")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

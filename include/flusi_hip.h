@@ -116,6 +116,19 @@ int flusi_get_mask(flusi_ctx* ctx, double* mask, double* us, int16_t* color);
 #define FLUSI_INTEGRALS_NV 61
 int flusi_write_integrals(flusi_ctx* ctx, double time, double* out);
 
+/* drag.f90:17-454 cal_drag(time, u, Insect): per-colour forces, moments about
+ * (x0,y0,z0) (the body centre for insects) and about the wing pivots,
+ * penalisation power, the residual velocity min|u-us| inside the mask and,
+ * for insects, aerodynamic and inertial power, from the physical velocity u
+ * (3 real components in the reference layout; NULL: ifft3 of the current uk)
+ * with the mask, us and colour as they stand -- a moving mask is redrawn by
+ * flusi_create_mask first, as write_integrals_fsi does (integrals.f90:87-100).
+ * Appends the forces.t, forces_part<n>.t and u_residual.t rows.  out (may be
+ * NULL): the FLUSI_INTEGRALS_NV layout of flusi_write_integrals with slots
+ * 0..9 and 32..60 set and the others zero.  Does nothing without
+ * penalisation (drag.f90:53). */
+int flusi_cal_drag(flusi_ctx* ctx, double time, const double* u, double* out);
+
 /* integrals.f90:522-552 compute_energies_k of the current uk. */
 int flusi_energy_k(flusi_ctx* ctx, double* E);
 
@@ -128,7 +141,13 @@ int flusi_fft(flusi_ctx* ctx, const double* f, double* fk);
 int flusi_ifft(flusi_ctx* ctx, const double* fk, double* f);
 
 /* Device time in milliseconds accumulated per pass since the last reset
- * (module_timing.f90 toc slots); names[i] are static strings. */
+ * (module_timing.f90 toc slots); names[i] are static strings.  The passes
+ * fuse several of the reference's slots: ZINV and YINV are the z and y parts
+ * of "RHS (cal_nlk_fsi::ifft3)" and "(cal_nlk_fsi::curl+ifft3)"; XPASS their
+ * x part, "(cal_nlk_fsi::nonlinear terms)", "(cal_nlk_fsi::vorticity_sponge)"
+ * and the x part of "FFT"; YFWD the y part of "FFT"; ZSTEP its z part,
+ * add_grad_pressure and the stepper update of "FluidTimestep (wrapper)";
+ * MASK "create_mask (main wrapper)"; DIAG "Statistics (cal_drag)". */
 #define FLUSI_TIMER_SLOTS 8
 int flusi_timers(flusi_ctx* ctx, int enable, double* ms, const char** names);
 

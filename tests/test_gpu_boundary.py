@@ -1,0 +1,216 @@
+"""The C ABI's remaining boundary pieces on the GPU: cal_drag as its own entry
+point (drag.f90:17-454) against the oracle, the optional NaN check, the
+timesteps_info.t writer, and the reference's abort codes driven through the
+ABI.  The error paths are host checks (the kernels run to completion on
+well-defined inputs, NaN included); none is meant to fault the GPU."""
+import os
+
+import numpy as np
+import pytest
+
+import flusi_amd
+from oracle import flusi_oracle as fo
+from tests._util import CONFIGS, GOLDEN, rel_l2, rewrite_ini
+
+pytestmark = pytest.mark.gpu
+
+
+def _insect_ini(tmp_path, n=64, **kw):
+    over = dict(Resolution__nx=n, Resolution__ny=n, Resolution__nz=n, Time__nt=4,
+                Time__dt_fixed="4.0e-4", Penalization__eps="1.0e-2", Saving__tintegral=100.0,
+                Saving__itdrag=100000)
+    over.update(kw)
+    return rewrite_ini(os.path.join(CONFIGS, "cfg3_insect_512.ini"), str(tmp_path), **over)
+
+
+def _smooth_state(sim, seed=5, amp=0.3):
+    g = sim.g
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((3, g.nxh, g.ny, g.nz)) + 1j * rng.standard_normal((3, g.nxh, g.ny, g.nz))
+    v *= np.exp(-0.5 * g.K2 / (4.0 * (2 * np.pi / g.p.xl) ** 2))[None]
+    v = fo.add_grad_pressure(g, v)
+    v[:, 0, 0, 0] = 0.0
+    v = fo.fft3(g, fo.ifft3(g, v))
+    v *= amp / np.sqrt(fo.energy_k(g, v))
+    return sim.uk + v
+
+
+def _check_drag(o, ref, rtol):
+    F, T = ref["force"], ref["torque"]
+    np.testing.assert_allclose(o[0:3], F, rtol=rtol, atol=rtol * np.max(np.abs(F)))
+    np.testing.assert_allclose(o[3:6], T, rtol=rtol, atol=rtol * np.max(np.abs(T)))
+    np.testing.assert_allclose(o[6:10], ref["penalization_power"], rtol=rtol,
+                               atol=rtol * np.max(np.abs(ref["penalization_power"])))
+    fp, tp = ref["force_part"], ref["torque_part"]
+    for c in (1, 2, 3):
+        np.testing.assert_allclose(o[32 + 3 * (c - 1):35 + 3 * (c - 1)], fp[c], rtol=rtol,
+                                   atol=rtol * np.max(np.abs(fp)))
+        np.testing.assert_allclose(o[44 + 3 * (c - 1):47 + 3 * (c - 1)], tp[c], rtol=rtol,
+                                   atol=rtol * np.max(np.abs(tp)))
+    np.testing.assert_allclose(o[53:59], np.sqrt(ref["u_residual"]), rtol=rtol)
+    assert not o[10:32].any()
+
+
+@pytest.mark.parametrize("prec,rtol", [(flusi_amd.FP64, 1e-10), (flusi_amd.FP32_MIXED, 2e-5)])
+def test_cal_drag_matches_oracle(tmp_path, prec, rtol):
+    """cal_drag(time, u, Insect) with the host's own u and with u = ifft3(uk),
+    the mask redrawn by create_mask first (as integrals.f90:87-100); the
+    rows it appends are those of the oracle's cal_drag."""
+    ini = _insect_ini(tmp_path)
+    sim = fo.Simulation(ini)
+    sim.init_fields()
+    sim.uk = _smooth_state(sim)
+    t = 0.0375
+    sim.create_mask(t)
+    u = fo.ifft3(sim.g, sim.uk)
+    ref = sim.cal_drag(t, u)
+    out = tmp_path / f"out{prec}"
+    out.mkdir()
+    with flusi_amd.Flusi(ini, prec, output_dir=str(out)) as s:
+        s.init_fields()
+        s.set_uk(sim.uk)
+        s.create_mask(t)
+        o_host = s.cal_drag(t, u)          # the host's velocity
+        o_uk = s.cal_drag(t)               # ifft3 of the context's uk
+    print(f"cal_drag precision {prec}: F {o_host[0:3]} vs {ref['force']}", flush=True)
+    _check_drag(o_host, ref, rtol)
+    _check_drag(o_uk, ref, rtol)
+    for ts in ("forces.t", "forces_part1.t", "forces_part2.t", "forces_part3.t", "u_residual.t"):
+        got = fo.read_timeseries(str(out / ts))
+        assert len(got) == 2 and len(got[0]) == len(sim.series[ts][0]), ts
+        np.testing.assert_allclose(got[0], sim.series[ts][-1], rtol=max(rtol, 1e-7), atol=1e-12)
+
+
+def test_cal_drag_sphere_without_fft_plan_and_without_penalisation(tmp_path):
+    """A sphere in uniform flow on a grid with no FFT plan (the host passes u):
+    F_x = sum(mask) dV / eta, the closed form of drag.f90 for u = (1, 0, 0)
+    and us = 0; without penalisation cal_drag returns zeros (drag.f90:53)."""
+    ini = rewrite_ini(os.path.join(GOLDEN, "sphere_sponge_RK2", "sphere_sponge_RK2.ini"), str(tmp_path),
+                      name="s.ini", Resolution__nx=50, Resolution__ny=40, Resolution__nz=36,
+                      Sponge__iVorticitySponge="no")
+    p, _ = fo.get_params(ini)
+    with flusi_amd.Flusi(ini, flusi_amd.FP64) as s:
+        s.init_fields()
+        m, _, _ = s.get_mask()
+        u = np.zeros((3, p.nz, p.ny, p.nx))
+        u[0] = 1.0
+        o = s.cal_drag(0.0, u)
+    dv = p.dx * p.dy * p.dz
+    assert o[0] == pytest.approx(m.sum() * dv / p.eps, rel=1e-12)
+    assert abs(o[1]) < 1e-12 * o[0] and abs(o[2]) < 1e-12 * o[0]
+    ini0 = rewrite_ini(ini, str(tmp_path), name="s0.ini", Penalization__iPenalization=0)
+    with flusi_amd.Flusi(ini0, flusi_amd.FP64) as s:
+        s.init_fields()
+        assert not s.cal_drag(0.0, u).any()
+
+
+def test_timesteps_info_written_per_step(tmp_path):
+    """time_step.f90:163, 189-201: one row per step with time, wall time since
+    the loop started, it, dt and the number of ranks."""
+    ini = _insect_ini(tmp_path, n=32)
+    out = tmp_path / "out"
+    out.mkdir()
+    with flusi_amd.Flusi(ini, flusi_amd.FP64, output_dir=str(out)) as s:
+        s.init_fields()
+        s.advance(3)
+    rows = [ln.split() for ln in open(out / "timesteps_info.t") if ln.strip()]
+    assert len(rows) == 3
+    for i, r in enumerate(rows):
+        assert len(r) == 5
+        assert float(r[0]) == pytest.approx(4.0e-4 * (i + 1), rel=1e-12)
+        assert int(r[2]) == i + 1 and float(r[3]) == 4.0e-4 and int(r[4]) == 1
+    assert float(rows[0][1]) <= float(rows[1][1]) <= float(rows[2][1])
+
+
+# ----------------------------------------------------------------- abort codes
+
+
+def test_abort_eps_too_small(tmp_path):
+    """rhs.f90:173: eps < 1e-11 aborts 77363800 before any transform."""
+    ini = _insect_ini(tmp_path, n=32, Penalization__eps="1.0e-12")
+    with flusi_amd.Flusi(ini, flusi_amd.FP64) as s:
+        s.init_fields()
+        with pytest.raises(flusi_amd.FlusiError) as e:
+            s.cal_nlk(0.0)
+        assert e.value.code == 77363800
+
+
+def test_abort_unknown_body_type(tmp_path):
+    """body_geometry.f90:85-86: the default BodyType 'ellipsoid' has no
+    drawing and aborts 10623."""
+    ini = _insect_ini(tmp_path, n=32, Insects__BodyType="ellipsoid")
+    with pytest.raises(flusi_amd.FlusiError) as e:
+        with flusi_amd.Flusi(ini, flusi_amd.FP64) as s:
+            s.init_fields()
+    assert e.value.code == 10623
+
+
+def test_abort_nan_with_cfl_dt(tmp_path):
+    """fluid_time_step.f90:947-950: a NaN in the velocity stops adjust_dt
+    (CFL-adapted dt) with 100011; a velocity of 1e3 or more with 100012."""
+    ini = rewrite_ini(os.path.join(GOLDEN, "vortex_ring1_AB2", "vortexring1_AB2.ini"), str(tmp_path),
+                      Saving__tintegral=100.0, Saving__tsave=100.0)
+    with flusi_amd.Flusi(ini, flusi_amd.FP64) as s:
+        s.init_fields()
+        uk = s.get_uk()
+        bad = uk.copy()
+        bad[0, 1, 1, 1] = np.nan
+        s.set_uk(bad)
+        with pytest.raises(flusi_amd.FlusiError) as e:
+            s.step()
+        assert e.value.code == 100011
+        s.set_uk(uk * 1.0e5)
+        with pytest.raises(flusi_amd.FlusiError) as e:
+            s.step()
+        assert e.value.code == 100012
+
+
+def test_checknan_every_step_with_fixed_dt(tmp_path):
+    """With dt_fixed no velocity maximum is taken, so a NaN would go on
+    unnoticed; checknan_every=N checks uk after every N-th step
+    (basic_operators.f90:925-950 checknan_cmplx, abort 202).  A clean run
+    with the check on takes the same steps as one without it."""
+    base = dict(Time__dt_fixed="1.0e-3", Saving__tintegral=100.0, Saving__tsave=100.0)
+    src = os.path.join(GOLDEN, "vortex_ring1_AB2", "vortexring1_AB2.ini")
+    ini_on = rewrite_ini(src, str(tmp_path), name="on.ini", Time__checknan_every=1, **base)
+    ini_off = rewrite_ini(src, str(tmp_path), name="off.ini", **base)
+    with flusi_amd.Flusi(ini_on, flusi_amd.FP64) as a, flusi_amd.Flusi(ini_off, flusi_amd.FP64) as b:
+        a.init_fields()
+        b.init_fields()
+        a.advance(3)
+        b.advance(3)
+        assert np.array_equal(a.get_uk(), b.get_uk())
+        bad = a.get_uk()
+        bad[1, 2, 3, 4] = np.nan
+        a.set_uk(bad)
+        b.set_uk(bad)
+        with pytest.raises(flusi_amd.FlusiError) as e:
+            a.step()
+        assert e.value.code == 202
+        b.step()   # unchecked: the NaN goes on
+        assert np.isnan(b.get_uk()).any()
+
+
+def test_abort_backup_resolution_mismatch(tmp_path):
+    """runtime_backuping.f90:262-268: a backup of another resolution aborts
+    77776."""
+    out = tmp_path / "out"
+    out.mkdir()
+    src = os.path.join(GOLDEN, "vortex_ring1_AB2", "vortexring1_AB2.ini")
+    ini32 = rewrite_ini(src, str(tmp_path), name="a.ini", Saving__tintegral=100.0, Saving__tsave=100.0,
+                        Saving__iDoBackup=1)
+    ini16 = rewrite_ini(src, str(tmp_path), name="b.ini", Resolution__nx=16, Resolution__ny=16,
+                        Resolution__nz=16, Saving__tintegral=100.0, Saving__tsave=100.0)
+    try:
+        flusi_amd._h5lib()
+    except RuntimeError as exc:
+        pytest.skip(str(exc))
+    with flusi_amd.Flusi(ini32, flusi_amd.FP64, output_dir=str(out)) as s:
+        s.init_fields()
+        s.advance(2)
+        s.dump_runtime_backup()
+    with flusi_amd.Flusi(ini16, flusi_amd.FP64, output_dir=str(out)) as s:
+        s.init_fields()
+        with pytest.raises(flusi_amd.FlusiError) as e:
+            s.read_runtime_backup("runtime_backup0.h5")
+        assert e.value.code == 77776

@@ -56,7 +56,7 @@ def test_sphere_outlet_sponge_rk2_steps(tmp_path, prec, tol):
         t, it, _, dt1 = s.get_time()
         assert it == sim.it and abs(dt1 - sim.dt1) <= 1e-12 * sim.dt1
         err = rel_l2(s.get_uk(), sim.uk)
-        check_err(err, tol * 50, "uk after 3 RK2 steps")
+        check_err(err, 5e-11 if prec == flusi_amd.FP64 else 4e-8, "uk after 3 RK2 steps")
 
 
 def test_sphere_full_size_rhs_fp64(tmp_path):
@@ -75,7 +75,7 @@ def test_sphere_full_size_rhs_fp64(tmp_path):
     assert err < 1e-11
 
 
-@pytest.mark.parametrize("prec,tol", [(flusi_amd.FP64, 1e-11), (flusi_amd.FP32_MIXED, 1e-5)])
+@pytest.mark.parametrize("prec,tol", [(flusi_amd.FP64, 1e-11), (flusi_amd.FP32_MIXED, 1.1e-6)])
 def test_sphere_sponge_rhs_nz512(tmp_path, prec, tol):
     """One right-hand side of the sphere workload with the outlet_x sponge on a
     64 x 64 x 512 grid: the z step's six-field (sponge) variant at nz = 512,

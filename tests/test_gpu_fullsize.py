@@ -67,7 +67,7 @@ def insect512(tmp_path_factory):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("prec,tol", [(flusi_amd.FP64, 1e-11), (flusi_amd.FP32_MIXED, 2e-5)])
+@pytest.mark.parametrize("prec,tol", [(flusi_amd.FP64, 1e-11), (flusi_amd.FP32_MIXED, 1.5e-6)])
 def test_insect_512_cal_nlk_matches_oracle(insect512, prec, tol):
     """rhs.f90:2-115 at the benchmark's 512^3 with the insect mask drawn at
     t = 0 (create_mask inside cal_nlk, iMoving = 1)."""
@@ -104,7 +104,7 @@ def _insect_ini(tmp_path, nx, ny, nz, **kw):
 
 @pytest.mark.timeout(600)
 @pytest.mark.parametrize("shape", [(64, 64, 512), (512, 64, 64), (64, 64, 64)])
-@pytest.mark.parametrize("prec,tol", [(flusi_amd.FP64, 1e-10), (flusi_amd.FP32_MIXED, 2e-5)])
+@pytest.mark.parametrize("prec,tol", [(flusi_amd.FP64, 1e-12), (flusi_amd.FP32_MIXED, 6e-7)])
 def test_insect_rk4_steps_match_oracle(tmp_path, shape, prec, tol):
     """Three RK4 steps (fluid_time_step.f90:602-671: explicit diffusion,
     dealiasing of every stage, the mask redrawn at t, t+dt/2, t+dt/2, t+dt)
@@ -159,15 +159,17 @@ def test_adjoint_cfg4_full_size(tmp_path):
             fd[int(i)] = (s.adjoint_forward(tp, K, every) - s.adjoint_forward(tm, K, every)) / (2 * h)
     print(f"[adjoint 128^3] J = {J:.12g}; adjoint {[(i, g[i]) for i in fd]}; central differences {fd}", flush=True)
     for i, v in fd.items():
-        assert abs(g[i] - v) <= 1e-6 * abs(v), (i, g[i], v)
+        # measured 2.6e-10 and 5.3e-9: the O(h^2) error of the differences
+        assert abs(g[i] - v) <= 3e-8 * abs(v), (i, g[i], v)
     print("[adjoint 128^3] FP32-mixed forward/backward", flush=True)
     with flusi_amd.Flusi(ini, flusi_amd.FP32_MIXED) as s32:
         J32 = s32.adjoint_forward(th, K, every)
         g32 = s32.adjoint_backward()
     eg = float(np.linalg.norm(g32 - g) / np.linalg.norm(g))
     print(f"[adjoint 128^3] FP32-mixed: J rel err {abs(J32 - J) / abs(J):.2e}, gradient rel L2 {eg:.2e}", flush=True)
-    assert abs(J32 - J) <= 1e-4 * abs(J)
-    assert eg <= 1e-3
+    # measured 2.2e-7 and 2.1e-7
+    assert abs(J32 - J) <= 1.5e-6 * abs(J)
+    assert eg <= 1.5e-6
 
 
 # ------------------------------------------------------------- 48-member ensemble
@@ -190,7 +192,7 @@ def test_ensemble_48_members_full_size(tmp_path):
         e.init_fields()
         ens[mo] = e
     assert sorted(ens[modes[2]].batches()) == sorted([(8, n, n, n) for n in (64, 96, 128) for _ in range(2)])
-    checks = {16: 96, 47: 128}
+    checks = {8: 96, 47: 128}  # spheres of every size first, then the plates
     for step in range(4):
         print(f"[ensemble 48] step {step}", flush=True)
         for mo in modes:
@@ -211,6 +213,6 @@ def test_ensemble_48_members_full_size(tmp_path):
                 sim.step()
                 err = rel_l2(ens[flusi_amd.ENSEMBLE_GRAPH].member(i).get_uk(), sim.uk)
                 print(f"[ensemble 48] member {i} ({n}^3) after one step: rel L2 vs oracle {err:.3e}", flush=True)
-                assert err < 2e-5
+                assert err < 1e-5
     for e in ens.values():
         e.close()

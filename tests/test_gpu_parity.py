@@ -16,6 +16,12 @@ pytestmark = pytest.mark.gpu
 PRECS = [(flusi_amd.FP64, 1e-12), (flusi_amd.FP32_MIXED, 2e-6)]
 
 
+def bnd(prec, fp64, fp32):
+    """The bound of a check: FP64 at reordered-sum round-off, FP32-mixed at
+    about five times the error measured on MI355X."""
+    return fp64 if prec == flusi_amd.FP64 else fp32
+
+
 def _base_ini(tmp_path, **kw):
     defaults = dict(Time__dt_fixed="1.0e-3", Time__iTimeMethodFluid="AB2")
     defaults.update(kw)
@@ -35,14 +41,14 @@ def test_fft_ifft_match_oracle(tmp_path, shape, prec, tol):
     f = rng.standard_normal((nz, ny, nx))
     with flusi_amd.Flusi(ini, prec) as s:
         fk = s.fft(f)
-        check_err(rel_l2(fk, fo.fft(g, f)), tol * 5, 'fk')
+        check_err(rel_l2(fk, fo.fft(g, f)), bnd(prec, 5e-12, 8e-7), 'fk')
         back = s.ifft(fo.fft(g, f))
-        check_err(rel_l2(back, f), tol * 5, 'back')
+        check_err(rel_l2(back, f), bnd(prec, 5e-12, 8e-7), 'back')
         # imaginary parts of the kx=0 / kx=nx/2 planes are ignored like FFTW's c2r
         fk2 = fo.fft(g, f)
         fk2[0] += 1j * 0.37
         fk2[-1] -= 1j * 0.21
-        check_err(rel_l2(s.ifft(fk2), fo.ifft(g, fk2)), tol * 5, 's.ifft(fk2)')
+        check_err(rel_l2(s.ifft(fk2), fo.ifft(g, fk2)), bnd(prec, 5e-12, 1e-5), 's.ifft(fk2)')
 
 
 @pytest.mark.parametrize("prec,tol", PRECS)
@@ -54,11 +60,12 @@ def test_vortex_ring_init_and_cal_nlk(tmp_path, prec, tol):
     nlk_ref, u_ref = sim.cal_nlk(0.0, sim.uk)
     with flusi_amd.Flusi(ini, prec) as s:
         s.init_fields()
-        check_err(rel_l2(s.get_uk(), sim.uk), tol * 10, 's.get_uk()')
+        check_err(rel_l2(s.get_uk(), sim.uk), bnd(prec, 1e-11, 3e-7), 's.get_uk()')
         nlk, u = s.cal_nlk(0.0, want_u=True)
-        check_err(rel_l2(nlk, nlk_ref), tol * 20, 'nlk')
-        check_err(rel_l2(u, u_ref), tol * 10, 'u')
-        check_err(rel_l2(s.get_vorticity(), fo.ifft3(sim.g, fo.curl(sim.g, sim.uk))), tol * 10, 's.get_vorticity()')
+        check_err(rel_l2(nlk, nlk_ref), bnd(prec, 2e-11, 2e-6), 'nlk')
+        check_err(rel_l2(u, u_ref), bnd(prec, 1e-11, 8e-7), 'u')
+        check_err(rel_l2(s.get_vorticity(), fo.ifft3(sim.g, fo.curl(sim.g, sim.uk))), bnd(prec, 1e-11, 9e-7),
+                  's.get_vorticity()')
 
 
 def _perturbed_state(sim, seed=5, amp=0.3):
@@ -90,11 +97,11 @@ def test_sphere_cavity_sponge_cal_nlk(tmp_path, prec, tol):
         assert np.array_equal(col, sim.color.astype(np.int16))
         s.set_uk(uk)
         nlk = s.cal_nlk(0.0)
-        check_err(rel_l2(nlk, nlk_ref), tol * 50, 'nlk')
+        check_err(rel_l2(nlk, nlk_ref), bnd(prec, 5e-11, 7e-7), 'nlk')
         # save_fields' pressure of the same state (sponge and penalisation in the source)
         sim.uk = uk
         pr_ref = sim.pressure_field(0.0)
-        check_err(rel_l2(s.get_pressure(0.0), pr_ref), tol * 50, 's.get_pressure(0.0)')
+        check_err(rel_l2(s.get_pressure(0.0), pr_ref), bnd(prec, 5e-11, 1e-6), 's.get_pressure(0.0)')
 
 
 @pytest.mark.parametrize("nx", [64, 96, 128, 256, 512])
@@ -115,12 +122,12 @@ def test_x_pass_per_wave_grids(tmp_path, nx, prec, tol):
     with flusi_amd.Flusi(ini, prec) as s:
         s.init_fields()
         s.set_uk(uk)
-        check_err(rel_l2(s.cal_nlk(0.0), nlk_ref), tol * 50, 's.cal_nlk(0.0)')
+        check_err(rel_l2(s.cal_nlk(0.0), nlk_ref), bnd(prec, 5e-11, 6e-7), 's.cal_nlk(0.0)')
         sim.step()
         s.step()
         t, it, dt0, dt1 = s.get_time()
         assert it == sim.it and abs(dt1 - sim.dt1) <= 1e-12 * sim.dt1
-        check_err(rel_l2(s.get_uk(), sim.uk), tol * 50, 's.get_uk()')
+        check_err(rel_l2(s.get_uk(), sim.uk), bnd(prec, 5e-11, 3e-8), 's.get_uk()')
 
 
 def _run_both(ini, prec, nsteps, perturb=False):
@@ -216,6 +223,37 @@ def test_bumblebee_masks_match_fixture(tmp_path, prec):
                 assert ok, (nm, t, errs)
 
 
+def _forces_fp32_scaled(got_path, ref_path):
+    """Force series of the FP32-mixed run of the corner-wrapped insect.  In
+    that run the body's y force and the x/z moments are no longer zero by
+    symmetry (as in the centred run, where they are ~1e-15 and below the
+    rule's 1e-4 cut-off) but small differences of O(1) contributions of the
+    two halves: 1.4e-4 .. 6.7e-4 against forces up to 8.  Their FP32 error
+    is that of the O(1) contributions (7e-7), so compare_timeseries' relative
+    rule fails on them (5e-3 on 1.4e-4) while every other entry passes it.
+    The check: the reference's rule on every entry above 1e-2 of the file's
+    largest force or moment, and every entry within 5e-6 of that scale
+    (measured 8.8e-7).  The FP64 run passes the reference's rule unchanged."""
+    got, ref = fo.read_timeseries(got_path), fo.read_timeseries(ref_path)
+    if len(got) < len(ref) - 1 or len(got[0]) != len(ref[0]):
+        return False, f"{len(got)}x{len(got[0])} written, reference {len(ref)}x{len(ref[0])}"
+    n = min(len(got), len(ref))
+    g, r = np.array(got[:n]), np.array(ref[:n])
+    cols = list(range(1, 4)) + list(range(7, 10))
+    scale = float(np.max(np.abs(r[:, cols])))
+    big = np.abs(r[:, cols]) >= 1e-2 * scale
+    diff = np.abs(g[:, cols] - r[:, cols])
+    rel = np.where(big & (diff > 1e-7), diff / np.maximum(np.abs(r[:, cols]), 1e-300), 0.0)
+    worst_scaled = float(np.max(diff) / scale)
+    # the other columns (time, the zero unsteady terms, the powers) under the
+    # reference's rule unchanged
+    rest = [c for c in range(r.shape[1]) if c not in cols]
+    ok_rest, worst_rest = fo.compare_timeseries(g[:, rest].tolist(), r[:, rest].tolist())
+    ok = float(np.max(rel)) <= 1e-4 and worst_scaled <= 5e-6 and ok_rest
+    return ok, (f"relative error of entries above 1e-2 of scale {np.max(rel):.2e}, error / scale "
+                f"{worst_scaled:.2e}, other columns {worst_rest:.2e}")
+
+
 @pytest.mark.parametrize("case,prec", [("insects_flow_BB_tethered_RK4", flusi_amd.FP64),
                                        ("insects_flow_BB_tethered_RK4", flusi_amd.FP32_MIXED),
                                        ("insects_flow_BB_tethered_AB2", flusi_amd.FP64),
@@ -241,7 +279,10 @@ def test_bumblebee_fixture_runs(tmp_path, prec, case):
     assert abs(t - 0.25) < 1e-12
     bad = []
     for ts in ("forces.t", "forces_part1.t", "forces_part2.t", "forces_part3.t", "kinematics.t", "energy.t", "dt.t"):
-        ok, worst, msg = series_ok(str(out / ts), os.path.join(d, ts))
+        if prec == flusi_amd.FP32_MIXED and case.endswith("_periodic") and ts.startswith("forces"):
+            ok, msg = _forces_fp32_scaled(str(out / ts), os.path.join(d, ts))
+        else:
+            ok, worst, msg = series_ok(str(out / ts), os.path.join(d, ts))
         print(f"{case} precision {prec} {ts}: {msg}", flush=True)
         if not ok:
             bad.append((ts, msg))
@@ -308,7 +349,7 @@ def test_pressure_moving_insect_rk4(tmp_path, prec, tol):
         s.init_fields()
         s.set_uk(sim.uk)
         pr = s.get_pressure(0.0125)
-    check_err(rel_l2(pr, pr_ref), tol * 50, 'pr')
+    check_err(rel_l2(pr, pr_ref), bnd(prec, 5e-11, 9e-7), 'pr')
 
 
 def test_vortex_ring4_sponge_fixture_run_fp64(tmp_path):
@@ -375,8 +416,7 @@ def test_mixed_precision_force_error_sphere(tmp_path):
             res[prec] = s.write_integrals(t)
     f64, f32 = res[flusi_amd.FP64][0:3], res[flusi_amd.FP32_MIXED][0:3]
     rel = abs(f32[0] - f64[0]) / abs(f64[0])
-    print("sphere drag rel err FP32-mixed vs FP64:", rel)
-    assert rel < 1e-4
+    check_err(rel, 4e-6, "sphere drag FP32-mixed vs FP64")
 
 
 def test_fortran_host_runs_vortex_ring_fixture(tmp_path):
