@@ -103,6 +103,64 @@ def run_gpu(ini, prec, warmup, steps, timers):
     return t1 - t0, ms, integ
 
 
+def accuracy_series(ini, steps, every):
+    """Forces and kinetic energy every `every` steps over `steps` RK4 steps
+    from rest in FP32-mixed and in FP64 (identical fixed steps, the mask
+    redrawn at each sample time as write_integrals does); per component the
+    largest error over the series, against the component's largest value
+    (`max_err_over_scale`) and pointwise where the FP64 value is at least 1 %
+    of that scale (`max_rel_err`)."""
+    import flusi_amd
+    ser = {}
+    for prec in (flusi_amd.FP32_MIXED, flusi_amd.FP64):
+        s = flusi_amd.Flusi(ini, prec)
+        s.init_fields()
+        rows = []
+        for _ in range(steps // every):
+            s.advance(every)
+            t = s.get_time()[0]
+            o = s.write_integrals(t)
+            rows.append([t, o[0], o[1], o[2], o[18]])  # time, Fx, Fy, Fz, E_kin
+        s.close()
+        ser[prec] = np.array(rows)
+        print(f"accuracy series precision {prec}: {len(rows)} samples to t={rows[-1][0]:.4g}", file=sys.stderr,
+              flush=True)
+    a, b = ser[flusi_amd.FP32_MIXED], ser[flusi_amd.FP64]
+
+    def comp(i):
+        d = np.abs(a[:, i] - b[:, i])
+        ref = np.abs(b[:, i])
+        scale = float(ref.max())
+        big = ref >= 1e-2 * scale
+        return {"max_err_over_scale": float(d.max() / scale) if scale > 0 else 0.0,
+                "max_rel_err": float((d[big] / ref[big]).max()) if big.any() else 0.0, "scale": scale}
+
+    return {"steps": steps, "every": every, "samples": int(len(b)), "t_end": float(b[-1, 0]),
+            "drag_Fx": comp(1), "side_Fy": comp(2), "lift_Fz": comp(3), "energy": comp(4)}
+
+
+def host_info():
+    """CPU model, logical CPUs usable by this process and RAM of the host."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    ram = None
+    try:
+        for ln in open("/proc/meminfo"):
+            if ln.startswith("MemTotal"):
+                ram = round(int(ln.split()[1]) / 2 ** 20, 1)
+                break
+    except OSError:
+        pass
+    cpus = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "logical_cpus": cpus, "ram_gib": ram}
+
+
 def cpu_baseline(n, workers):
     """One right-hand side (cal_nlk incl. the insect mask) of the same workload
     with the FP64 oracle in a child process; steps/s = 1 / (4 t_RHS)."""
@@ -292,6 +350,9 @@ def main():
     ap.add_argument("--workload", choices=["insect", "ensemble", "adjoint"], default="insect")
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--fd", type=int, default=2, help="central-difference checks of the largest gradient components")
+    ap.add_argument("--series-steps", type=int, default=1000,
+                    help="RK4 steps of the FP32-mixed vs FP64 force/energy series (0: none)")
+    ap.add_argument("--series-every", type=int, default=10, help="steps between samples of that series")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for HBM bytes)")
     args = ap.parse_args()
@@ -343,24 +404,29 @@ def main():
     out["step_roofline_frac"] = B_PT_RK4[4] * gps32 / HBM_PEAK
 
     if not args.no_fp64:
-        dt64, ms64, integ64 = run_gpu(ini, flusi_amd.FP64, args.warmup, args.steps, False)
+        dt64, ms64, integ64 = run_gpu(ini, flusi_amd.FP64, args.warmup, args.steps, True)
         out["fp64_value"] = N * args.steps / dt64
         out["fp32_over_fp64"] = gps32 / out["fp64_value"]
-        f32, f64 = integ32[0:3], integ64[0:3]
-        rel = max(abs(a - b) for a, b in zip(f32, f64)) / max(1e-300, max(abs(b) for b in f64))
-        out["force_rel_err_vs_fp64"] = rel
-        out["forces_fp64"] = list(f64)
-        out["forces_fp32"] = list(f32)
+        out["fp64_pass_ms_per_step"] = {k: v / args.steps for k, v in ms64.items() if v > 0} if ms64 else None
+        out["forces_fp64"] = list(integ64[0:3])
+        out["forces_fp32"] = list(integ32[0:3])
+        if args.series_steps > 0:
+            sini = make_ini(tempfile.mkdtemp(), n, args.series_steps + 10)
+            acc = accuracy_series(sini, args.series_steps, args.series_every)
+            out["accuracy_vs_fp64"] = acc
+            # the headline figure: the worst of drag and lift over the series
+            out["force_rel_err_vs_fp64"] = max(acc["drag_Fx"]["max_rel_err"], acc["lift_Fz"]["max_rel_err"])
 
     if not args.no_cpu:
-        workers = min(16, os.cpu_count() or 1)
+        host = host_info()
+        workers = max(1, min(16, host["logical_cpus"] or 1))
         t_rhs, err = cpu_baseline(n, workers)
         if t_rhs:
             out["cpu_baseline"] = {"value": N / (4.0 * t_rhs), "unit": "grid-pt·steps/s", "cores": workers,
-                                   "kind": "port",
-                                   "sample": f"one FP64 right-hand side (cal_nlk with insect mask) of the {n}^3 "
-                                             f"workload by the numpy/scipy oracle, t_RHS={t_rhs:.2f}s; "
-                                             f"steps/s = 1/(4 t_RHS)"}
+                                   "kind": "port", "host": host,
+                                   "sample": f"one FP64 right-hand side (cal_nlk with the insect mask drawn) of the "
+                                             f"{n}^3 workload by the numpy/scipy oracle on {workers} threads, "
+                                             f"t_RHS={t_rhs:.2f}s; steps/s = 1/(4 t_RHS) (an RK4 step is four)"}
         else:
             out["cpu_baseline"] = {"value": None, "error": err}
     print(json.dumps(out), flush=True)

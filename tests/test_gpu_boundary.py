@@ -87,7 +87,7 @@ def test_cal_drag_sphere_without_fft_plan_and_without_penalisation(tmp_path):
     and us = 0; without penalisation cal_drag returns zeros (drag.f90:53)."""
     ini = rewrite_ini(os.path.join(GOLDEN, "sphere_sponge_RK2", "sphere_sponge_RK2.ini"), str(tmp_path),
                       name="s.ini", Resolution__nx=50, Resolution__ny=40, Resolution__nz=36,
-                      Sponge__iVorticitySponge="no")
+                      Sponge__iVorticitySponge="no", InitialCondition__inicond="quiescent")
     p, _ = fo.get_params(ini)
     with flusi_amd.Flusi(ini, flusi_amd.FP64) as s:
         s.init_fields()

@@ -181,7 +181,10 @@ def test_ensemble_48_members_full_size(tmp_path):
     members each at 64^3, 96^3 and 128^3, spheres then plates per size, RK2,
     FP32-mixed): member-by-member, batched and graph-replayed advances agree
     bit for bit after the eager first step, the capture and two replays; one
-    96^3 sphere and one 128^3 plate match the oracle after the first step."""
+    96^3 sphere and one 128^3 plate match the oracle after the second step
+    (the first RK2 step from a uniform flow changes only its zero mode, since
+    at it = 0 the stale integrating factor drops N0 and N1 = N0,
+    fluid_time_step.f90:554-597)."""
     from flusi_amd.ensemble_configs import write_ensemble
     paths = write_ensemble(str(tmp_path / "ens"))
     assert len(paths) == 48
@@ -205,14 +208,15 @@ def test_ensemble_48_members_full_size(tmp_path):
                 m = ens[mo].member(i)
                 assert m.get_time()[:2] == t0[:2], (step, i, mo)
                 assert np.array_equal(m.get_uk(), u0), (step, i, mo)
-        if step == 0:
+        if step == 1:
             for i, n in checks.items():
                 sim = fo.Simulation(paths[i], workers=WORKERS)
                 sim.init_fields()
                 assert sim.g.nx == n
                 sim.step()
+                sim.step()
                 err = rel_l2(ens[flusi_amd.ENSEMBLE_GRAPH].member(i).get_uk(), sim.uk)
-                print(f"[ensemble 48] member {i} ({n}^3) after one step: rel L2 vs oracle {err:.3e}", flush=True)
+                print(f"[ensemble 48] member {i} ({n}^3) after two steps: rel L2 vs oracle {err:.3e}", flush=True)
                 assert err < 1e-5
     for e in ens.values():
         e.close()
