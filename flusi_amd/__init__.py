@@ -64,6 +64,7 @@ EXPORTED: Dict[str, tuple] = {
     "flusi_write_integrals": (_I, [_P, ctypes.c_double, _D]),
     "flusi_cal_drag": (_I, [_P, ctypes.c_double, _D, _D]),
     "flusi_energy_k": (_I, [_P, _D]),
+    "flusi_energy_k_noalloc": (_I, [_P, _D]),
     "flusi_cal_vis": (_I, [_P, ctypes.c_double, _D]),
     "flusi_fft": (_I, [_P, _D, _D]),
     "flusi_ifft": (_I, [_P, _D, _D]),
@@ -330,6 +331,12 @@ class Flusi:
     def energy_k(self) -> float:
         e = ctypes.c_double()
         self._check(self._lib.flusi_energy_k(self._ctx, ctypes.byref(e)))
+        return e.value
+
+    def energy_k_noalloc(self) -> float:
+        """energy_k without device allocation (scratch in a fixed device array)."""
+        e = ctypes.c_double()
+        self._check(self._lib.flusi_energy_k_noalloc(self._ctx, ctypes.byref(e)))
         return e.value
 
     def cal_vis(self, dt: float) -> np.ndarray:

@@ -36,6 +36,37 @@ static void kcheck(const char* what) {
   if (e != hipSuccess) throw FlusiError(900002, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Scratch of the allocation-free entry point flusi_energy_k_noalloc: the
+// Parseval partial sums, the row reducer's middle rows and the result.  Each
+// GPU holds its own copy until the process ends.  In host code the name
+// `scratch` is a stand-in, not a device address: the address comes from
+// hipGetSymbolAddress, once per device, and only that address is passed on.
+// Every call on a GPU shares its copy (calls that overlap on one GPU race);
+// each call writes every element it reads before reading it.
+constexpr size_t kNoAllocScratchDoubles = 4096;  // 32 KiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* scratch_dev[kMaxDevices] = {};
+static std::mutex scratch_mu;
+
+// the device address of `scratch` on the current device
+static double* scratch_address() {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    throw FlusiError(900020, "device " + std::to_string(dev) + " beyond the " + std::to_string(kMaxDevices) +
+                                 " devices the allocation-free scratch serves");
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw FlusiError(900021, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  return scratch_dev[dev];
+}
+
 // fluid_time_step.f90:1051-1060 truncate: write(str,'(es7.1)') a; read(str,*) a
 static double truncate_dt(double a) {
   char buf[64];
@@ -105,6 +136,7 @@ class SolverBase {
   virtual void write_integrals(double time, double* out) = 0;
   virtual void cal_drag(double time, const double* u, double* out) = 0;
   virtual double energy_k() = 0;
+  virtual double energy_k_noalloc() = 0;
   virtual void cal_vis(double dt, double* vis) = 0;
   virtual void fft1(const double* f, double* fk) = 0;
   virtual void ifft1(const double* fk, double* f) = 0;
@@ -257,6 +289,7 @@ class Solver : public SolverBase {
   bool xwave = false;       // right-hand-side x pass with one wavefront per line
   double* zpart = nullptr; int nzblocks = 0;
   double* dpart = nullptr; int ndblocks = 0;
+  double* noalloc_scratch = nullptr;  // device address of `scratch` on this context's GPU
   double* red = nullptr;    // small results
   double* rmid = nullptr;   // reduction scratch
   int* kinds = nullptr;     // reduction kinds: [0..3] x pass, [4] energy, [8..8+DIAG_NV) diag
@@ -321,6 +354,7 @@ class Solver : public SolverBase {
     nc = (long)nxh * ny * nz;
     geom = GridGeom{nx, ny, nz, nxh, npts, nc, p.dx(), p.dy(), p.dz(), p.xl, p.yl, p.zl, p.periodic ? 1 : 0};
     HIPCHECK(hipStreamCreate(&stream));
+    noalloc_scratch = scratch_address();  // once per device, with this context's device current
     const size_t cs = sizeof(C) * nc;
     mask = dalloc(sizeof(T) * npts);
     us = dalloc(3 * sizeof(T) * npts);
@@ -1837,6 +1871,28 @@ class Solver : public SolverBase {
     return e * p.xl * p.yl * p.zl;
   }
 
+  // energy_k() without device allocation: the partial sums, the reducer's
+  // middle rows and the result in the file-scope scratch (energy_k() keeps
+  // its partials in memory it allocates on its first call); the same kernels
+  // in the same order, so the result is the same to the bit
+  double energy_k_noalloc() override {
+    need_fft();
+    const size_t nb = (size_t)energy_k_blocks(), nmid = reduce_rows_scratch(1);
+    if (nb + nmid + 1 > kNoAllocScratchDoubles)
+      throw FlusiError(900022, "energy_k_noalloc needs " + std::to_string(nb + nmid + 1) +
+                                   " doubles of scratch, the maximum is " + std::to_string(kNoAllocScratchDoubles));
+    double* part = noalloc_scratch;
+    double* mid = part + nb;
+    double* out = mid + nmid;
+    launch_energy_k<T>(geom, uk, part, stream);
+    launch_reduce_rows(part, (int)nb, 1, kinds + 4, mid, out, stream);
+    kcheck("energy_k_noalloc");
+    double e = 0.0;
+    HIPCHECK(hipMemcpyAsync(&e, out, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    return e * p.xl * p.yl * p.zl;
+  }
+
   // vis.f90:9-66 cal_vis into a host table (the boundary copy; the steppers
   // evaluate the same factors inside the z step)
   double* visd = nullptr;
@@ -2505,6 +2561,9 @@ int flusi_cal_drag(flusi_ctx* ctx, double time, const double* u, double* out) {
   return guarded(ctx, [&] { ctx->s->cal_drag(time, u, out); });
 }
 int flusi_energy_k(flusi_ctx* ctx, double* E) { return guarded(ctx, [&] { *E = ctx->s->energy_k(); }); }
+int flusi_energy_k_noalloc(flusi_ctx* ctx, double* E) {
+  return guarded(ctx, [&] { *E = ctx->s->energy_k_noalloc(); });
+}
 int flusi_cal_vis(flusi_ctx* ctx, double dt, double* vis) { return guarded(ctx, [&] { ctx->s->cal_vis(dt, vis); }); }
 int flusi_fft(flusi_ctx* ctx, const double* f, double* fk) { return guarded(ctx, [&] { ctx->s->fft1(f, fk); }); }
 int flusi_ifft(flusi_ctx* ctx, const double* fk, double* f) { return guarded(ctx, [&] { ctx->s->ifft1(fk, f); }); }

@@ -131,6 +131,17 @@ int flusi_cal_drag(flusi_ctx* ctx, double time, const double* u, double* out);
 
 /* integrals.f90:522-552 compute_energies_k of the current uk. */
 int flusi_energy_k(flusi_ctx* ctx, double* E);
+/* flusi_energy_k without any device allocation (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * flusi_energy_k allocates its partial sums on its first call; this variant
+ * keeps them (512 doubles), the reduction rows and the result in a
+ * file-scope device array of 4096 doubles (32 KiB) that each GPU holds until
+ * the process ends, and returns the same value to the bit.  Calls on one GPU
+ * share that array: calls that overlap on one GPU (contexts driven from
+ * different host threads) race.  900022 if the scratch would exceed its
+ * maximum. */
+int flusi_energy_k_noalloc(flusi_ctx* ctx, double* E);
 
 /* vis.f90:9-66 cal_vis (real, spectral layout). */
 int flusi_cal_vis(flusi_ctx* ctx, double dt, double* vis);
