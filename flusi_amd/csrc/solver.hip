@@ -1641,20 +1641,36 @@ class Solver : public SolverBase {
   // flusi_hdf5_interface.f90:21-81 save_field_hdf5: one field per file
   // "<name>.h5", dataset = the name up to its first '_', with the
   // attributes time, viscosity, epsi, domain_size, origin and nxyz
+  // flusi_hdf5_interface.f90:21-81 save_field_hdf5; with striding > 1 the
+  // field at ix, iy, iz = 0, s, 2s, ... (every index below n) and nxyz =
+  // (nx, ny, nz) / s (integer division), as save_field_hdf5_strided (:95-169)
   void save_field_h5(const std::string& name, const double* f) {
     const std::string file = out_path(name + ".h5");
     const std::string dset = name.substr(0, name.find('_'));
-    h5_write(file, dset, nx, ny, nz, f, p.field_precision != "double", true,
+    const int st = p.striding;
+    if (st < 1) throw FlusiError(8881, "Striding value is bad, exit!");
+    std::vector<double> red;
+    int mx = nx, my = ny, mz = nz;
+    if (st > 1) {
+      mx = (nx - 1) / st + 1; my = (ny - 1) / st + 1; mz = (nz - 1) / st + 1;
+      red.resize((size_t)mx * my * mz);
+      for (int iz = 0; iz < mz; ++iz)
+        for (int iy = 0; iy < my; ++iy)
+          for (int ix = 0; ix < mx; ++ix)
+            red[((size_t)iz * my + iy) * mx + ix] = f[((size_t)(st * iz) * ny + st * iy) * nx + st * ix];
+      f = red.data();
+    }
+    h5_write(file, dset, mx, my, mz, f, p.field_precision != "double", true,
              {{"time", {time}, 0}, {"viscosity", {p.nu}, 0}, {"epsi", {p.eps}, 0},
               {"domain_size", {p.xl, p.yl, p.zl}, 0}, {"origin", {0.0, 0.0, 0.0}, 0},
-              {"nxyz", {(double)nx, (double)ny, (double)nz}, 1}});
+              {"nxyz", {(double)(nx / st), (double)(ny / st), (double)(nz / st)}, 1}});
   }
 
   // save_fields.f90:46-237 save_fields_fsi at the current time
   void save_fields() override {
     need_fft();
     if (!fields_initialized) throw FlusiError(900004, "fields not initialised (call flusi_init_fields)");
-    if (p.striding != 1) throw FlusiError(8881, "striding > 1 is not supported by the HDF5 field writer");
+    if (p.striding < 1) throw FlusiError(8881, "Striding value is bad, exit!");
     char name[32];
     if (p.naming == "timestep") std::snprintf(name, sizeof(name), "%06d", it);
     else if (p.save_only_one_period == "yes")  // save_fields.f90:86 (t minus the period count)

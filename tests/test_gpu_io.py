@@ -127,3 +127,29 @@ def test_snapshot_files_hold_the_fields(tmp_path, prec):
         np.testing.assert_allclose(got, f32(ref), rtol=0, atol=1e-6 * max(1.0, np.abs(ref).max()), err_msg=name)
         assert flusi_amd.read_h5_attr(fpath, name, "time", 1)[0] == t
         np.testing.assert_array_equal(flusi_amd.read_h5_attr(fpath, name, "nxyz", 3), [32, 32, 32])
+
+
+def test_strided_snapshots(tmp_path):
+    """flusi_hdf5_interface.f90:95-169 save_field_hdf5_strided: with striding =
+    3 on a 32 x 32 x 24 grid the file holds the points 0, 3, ..., 30 (11) in x
+    and y and 0, 3, ..., 21 (8) in z, with nxyz = (32, 32, 24) / 3 = (10, 10,
+    8) as the reference writes it."""
+    src = os.path.join(GOLDEN, "insects_flow_BB_tethered_RK4", "insect_test.ini")
+    out = str(tmp_path / "snap")
+    path = rewrite_ini(src, out, "case.ini", Resolution__nx="32", Resolution__ny="32", Resolution__nz="24",
+                       Saving__iSaveVelocity="1", Saving__iSaveMask="1", Saving__striding="3",
+                       Saving__iDoBackup="0", Saving__field_precision="double")
+    with flusi_amd.Flusi(path, flusi_amd.FP64, output_dir=out) as s:
+        s.init_fields()
+        s.advance(2)
+        t = s.get_time()[0]
+        s.save_fields()
+        u = s.get_u()
+        mask, _, _ = s.get_mask()
+    tag = "%06d" % int(np.floor(t * 1000.0 + 0.5))
+    for name, ref in (("ux", u[0]), ("uz", u[2]), ("mask", mask)):
+        fpath = os.path.join(out, f"{name}_{tag}.h5")
+        got = flusi_amd.read_h5_field(fpath)
+        assert got.shape == (8, 11, 11)
+        np.testing.assert_array_equal(got, ref[::3, ::3, ::3], err_msg=name)
+        np.testing.assert_array_equal(flusi_amd.read_h5_attr(fpath, name, "nxyz", 3), [10, 10, 8])
