@@ -133,13 +133,16 @@ template <int R, int Ns> __device__ __forceinline__ int stage_out(int j, int q) 
   return (j / Ns) * Ns * R + (j % Ns) + q * Ns;
 }
 
-// One Stockham stage over NL lines (line i at i*LSP) held in LDS.
+// One Stockham stage over NL lines (line i at i*LSP) held in LDS.  tid is
+// the thread's index in the block; a caller that runs several transforms in
+// one kernel can pass a copy the compiler cannot see through, so that the
+// addresses and twiddles derived from it are recomputed per transform rather
+// than kept live (in registers, or spilled) from the first to the last.
 template <typename T, int L, int NL, int NT, int R, int S, int Ns, int LSP>
-__device__ __forceinline__ void lds_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+__device__ __forceinline__ void lds_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw, int tid) {
   constexpr int NB = NL * (L / R);
   constexpr int PER = (NB + NT - 1) / NT;
   cplx<T> v[PER][R];
-  const int tid = threadIdx.x;
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
@@ -171,15 +174,15 @@ template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE
 struct Stages;
 template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE, int LAST, int NS>
 struct Stages<T, L, NL, NT, KIND, S, LSP, STAGE, LAST, NS, false> {
-  static __device__ __forceinline__ void run(cplx<T>* buf, const cplx<T>* tw) {
+  static __device__ __forceinline__ void run(cplx<T>* buf, const cplx<T>* tw, int tid = threadIdx.x) {
     constexpr int R = plan_radix_k(KIND, L, STAGE);
-    lds_stage<T, L, NL, NT, R, S, NS, LSP>(buf, tw);
-    Stages<T, L, NL, NT, KIND, S, LSP, STAGE + 1, LAST, NS * R>::run(buf, tw);
+    lds_stage<T, L, NL, NT, R, S, NS, LSP>(buf, tw, tid);
+    Stages<T, L, NL, NT, KIND, S, LSP, STAGE + 1, LAST, NS * R>::run(buf, tw, tid);
   }
 };
 template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE, int LAST, int NS>
 struct Stages<T, L, NL, NT, KIND, S, LSP, STAGE, LAST, NS, true> {
-  static __device__ __forceinline__ void run(cplx<T>*, const cplx<T>*) {}
+  static __device__ __forceinline__ void run(cplx<T>*, const cplx<T>*, int = 0) {}
 };
 
 // product of the radices of stages [0, s) of a plan

@@ -151,6 +151,7 @@ template <typename T> void launch_tile_pass(const TilePassArgs& a, int L, int di
 // the same passes with each line transformed inside one wavefront (tiles_wave.hip)
 template <typename T> void launch_tile_wave(const TilePassArgs& a, int L, int dir, bool contig_in, hipStream_t s);
 bool tile_wave_supported(int L);
+bool zstep_wave_supported(int nz);  // in-wave z step (else the block kernel)
 template <typename T> void launch_x_pass(const XPassArgs& a, int mode, hipStream_t s);
 template <typename T> void launch_zstep(const ZStepArgs& a, hipStream_t s);
 int x_pass_blocks(int nx, int nlines, bool fp64);

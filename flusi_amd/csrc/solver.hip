@@ -726,7 +726,7 @@ class Solver : public SolverBase {
   std::vector<Variant> candidates() const {
     const bool tw_any = tile_wave_supported(ny) || tile_wave_supported(nz);
     const bool tw_all = tile_wave_supported(ny) && tile_wave_supported(nz);
-    const bool xw = x_rhs_wave_supported(nx), zw = tile_wave_supported(nz);
+    const bool xw = x_rhs_wave_supported(nx), zw = zstep_wave_supported(nz);
     std::vector<Variant> c;
     for (int tw : {1, 0}) {
       if (tw && !tw_any) continue;

@@ -12,6 +12,7 @@
 #include "pass_common.h"
 #include "wave_fft.h"
 #include <cstdlib>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 
@@ -54,7 +55,11 @@ template <typename T> __device__ __forceinline__ MOff member_off(const ZStepArgs
 // reference's FP64 comparison (so that the same modes are removed), the
 // integrating factors exp(-nu k^2 dt) are evaluated in FP64, and the
 // Parseval sum is accumulated in compensated FP64.
-template <typename T, bool SPONGE, int MG>
+//
+// FOLDED: rv[f] already holds the scaled right-hand side with the sponge term
+// added (sponge_fold below), so that a kernel need not keep the sponge
+// fields' values live until the epilogue.
+template <typename T, bool SPONGE, int MG, bool FOLDED = false>
 __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, int kx, int ky, int kzi, const cplx<T>* rv,
                                           double kxv_d, double kxt2, double wE,
                                           bool final_stage, double& esum, double& ecomp) {
@@ -74,8 +79,11 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
   const R scale = (R)a.scale, nu = (R)a.nu, dt = (R)a.dt;
   R nr[3], ni[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) { nr[c] = (R)rv[c].x * scale; ni[c] = (R)rv[c].y * scale; }
-  if constexpr (SPONGE) {
+  for (int c = 0; c < 3; ++c) {
+    nr[c] = FOLDED ? (R)rv[c].x : (R)rv[c].x * scale;
+    ni[c] = FOLDED ? (R)rv[c].y : (R)rv[c].y * scale;
+  }
+  if constexpr (SPONGE && !FOLDED) {
     // sponge.f90:22-83: i k x (w/k^2), zero at k = 0
     if (k2d != 0.0) {
       R sr[3], sm[3];
@@ -248,6 +256,46 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
   }
 }
 
+// sponge.f90:22-83 at (kx, ky, kz), for the kernels that transform the three
+// sponge fields first: s0..s2 (their unscaled transforms) are replaced by the
+// term added to fields 0..2, i k x (w/k^2) scaled, zero at k = 0 and at
+// padding lines (ky >= ny).  The operations are those of zepilogue's own
+// sponge branch.
+template <typename T>
+__device__ __forceinline__ void sponge_term(const ZStepArgs& a, const MOff& mo, double kxv_d, int ky, int kzi,
+                                            cplx<T>& s0, cplx<T>& s1, cplx<T>& s2) {
+  using R = T;
+  double k2d = 0.0, kyv_d = 0.0, kzv_d = 0.0;
+  if (ky < a.ny) {
+    kyv_d = a.ky[ky + mo.md];
+    kzv_d = a.kz[kzi + mo.md];
+    k2d = (kxv_d * kxv_d + kyv_d * kyv_d) + kzv_d * kzv_d;
+  }
+  if (k2d == 0.0) {
+    s0 = s1 = s2 = mk<T>(T(0), T(0));
+    return;
+  }
+  const R kxv = (R)kxv_d, kyv = (R)kyv_d, kzv = (R)kzv_d, k2 = (R)k2d, scale = (R)a.scale;
+  const R sr[3] = {(R)s0.x * scale / k2, (R)s1.x * scale / k2, (R)s2.x * scale / k2};
+  const R sm[3] = {(R)s0.y * scale / k2, (R)s1.y * scale / k2, (R)s2.y * scale / k2};
+  const R cr0 = kyv * sr[2] - kzv * sr[1], ci0 = kyv * sm[2] - kzv * sm[1];
+  const R cr1 = kzv * sr[0] - kxv * sr[2], ci1 = kzv * sm[0] - kxv * sm[2];
+  const R cr2 = kxv * sr[1] - kyv * sr[0], ci2 = kxv * sm[1] - kyv * sm[0];
+  s0 = mk<T>((T)-ci0, (T)cr0);
+  s1 = mk<T>((T)-ci1, (T)cr1);
+  s2 = mk<T>((T)-ci2, (T)cr2);
+}
+// field value o (unscaled transform) scaled, plus its sponge term c.  The
+// sum is pinned where it is formed: left free, the compiler sinks the add
+// into the epilogue and keeps both operands live until then.
+template <typename T> __device__ __forceinline__ cplx<T> sponge_fold(cplx<T> o, cplx<T> c, T scale) {
+  T x = o.x * scale, y = o.y * scale;
+  x += c.x;
+  y += c.y;
+  asm volatile("" : "+v"(x), "+v"(y));
+  return mk<T>(x, y);
+}
+
 // Lines per workgroup of the block kernel.  FP32 at nz = 512 takes 8 lines
 // (512 threads, 64-byte Q segments) and is held to 128 VGPRs, so that two
 // workgroups share a CU and one's transform stages (LDS work between block
@@ -301,8 +349,15 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
   constexpr bool PARK = ZBShape<T, L, TL>::park;
   // park: field 0 by (ky line, kz), written and read back by the same thread
   cplx<T>* park = buf + TL * LSP;
-  // r[f][p][q]: field f at (ky = b0 + bb, kz = j + q L/RL) of last-stage butterfly p
-  cplx<T> r[NF][PERL][RL];
+  // r[f][p][q]: field f (0..2) at (ky = b0 + bb, kz = j + q L/RL) of
+  // last-stage butterfly p.  With the sponge the three sponge fields are
+  // transformed first into r, turned in place into their curl terms
+  // i k x (w/k^2) (sponge.f90:22-83) per position, and each term is added to
+  // its field as that field is transformed, so that no more than three
+  // fields' values are live (six would spill at nz >= 384).
+  cplx<T> r[3][PERL][RL];
+  // field of the i-th transform
+  constexpr auto fld = [](int i) { return SPONGE ? (i < 3 ? 3 + i : i - 3) : i; };
   // stage-0 operands of a field; the next field's are in flight while the
   // current one goes through its LDS stages
   cplx<T> v[PER0][R0];
@@ -319,37 +374,111 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
       }
     }
   };
-  loadq(0);
+  loadq(fld(0));
+  if constexpr (!SPONGE) {
 #pragma unroll
-  for (int f = 0; f < NF; ++f) {
+    for (int f = 0; f < NF; ++f) {
 #pragma unroll
-    for (int p = 0; p < PER0; ++p) {
-      const int beta = tid + p * NT;
-      if (NB0 % NT == 0 || beta < NB0) {
-        const int bb = beta % TL, j = beta / TL;
-        Bfly<R0, -1, T>::run(v[p]);
+      for (int p = 0; p < PER0; ++p) {
+        const int beta = tid + p * NT;
+        if (NB0 % NT == 0 || beta < NB0) {
+          const int bb = beta % TL, j = beta / TL;
+          Bfly<R0, -1, T>::run(v[p]);
 #pragma unroll
-        for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
-      }
-    }
-    __syncthreads();
-    if (f + 1 < NF) loadq(f + 1);
-    Stages<T, L, TL, NT, PLAN_TILE, -1, LSP, 1, NSTG - 1, R0>::run(buf, tw);
-#pragma unroll
-    for (int p = 0; p < PERL; ++p) {
-      const int beta = tid + p * NT;
-      if (NBL % NT == 0 || beta < NBL) {
-        const int bb = beta / (L / RL), j = beta - bb * (L / RL);
-#pragma unroll
-        for (int q = 0; q < RL; ++q) r[f][p][q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
-        twiddle_bfly<T, L, RL, -1, NSL>(r[f][p], j, tw);
-        if (PARK && f == 0) {
-#pragma unroll
-          for (int q = 0; q < RL; ++q) park[bb * LSP + lp<T>(j + q * (L / RL))] = r[f][p][q];
+          for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
         }
       }
+      __syncthreads();
+      if (f + 1 < NF) loadq(f + 1);
+      Stages<T, L, TL, NT, PLAN_TILE, -1, LSP, 1, NSTG - 1, R0>::run(buf, tw);
+#pragma unroll
+      for (int p = 0; p < PERL; ++p) {
+        const int beta = tid + p * NT;
+        if (NBL % NT == 0 || beta < NBL) {
+          const int bb = beta / (L / RL), j = beta - bb * (L / RL);
+#pragma unroll
+          for (int q = 0; q < RL; ++q) r[f][p][q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
+          twiddle_bfly<T, L, RL, -1, NSL>(r[f][p], j, tw);
+          if (PARK && f == 0) {
+#pragma unroll
+            for (int q = 0; q < RL; ++q) park[bb * LSP + lp<T>(j + q * (L / RL))] = r[f][p][q];
+          }
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
+  } else {
+    // Transform number i as straight-line code per i (with a loop left to
+    // the unroller the allocation spills at nz = 512), each on its own copy
+    // of the thread index (see lds_stage).
+    auto xform = [&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int f = fld(i);
+      constexpr int slot = i < 3 ? i : f;  // register slot of this transform's output
+      int tl = tid;
+      asm volatile("" : "+v"(tl));
+#pragma unroll
+      for (int p = 0; p < PER0; ++p) {
+        const int beta = tl + p * NT;
+        if (NB0 % NT == 0 || beta < NB0) {
+          const int bb = beta % TL, j = beta / TL;
+          Bfly<R0, -1, T>::run(v[p]);
+#pragma unroll
+          for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
+        }
+      }
+      __syncthreads();
+      if constexpr (i + 1 < NF) loadq(fld(i + 1));
+      Stages<T, L, TL, NT, PLAN_TILE, -1, LSP, 1, NSTG - 1, R0>::run(buf, tw, tl);
+#pragma unroll
+      for (int p = 0; p < PERL; ++p) {
+        const int beta = tl + p * NT;
+        if (NBL % NT == 0 || beta < NBL) {
+          const int bb = beta / (L / RL), j = beta - bb * (L / RL);
+          cplx<T> o[RL];
+#pragma unroll
+          for (int q = 0; q < RL; ++q) o[q] = buf[bb * LSP + lp<T>(j + q * (L / RL))];
+          twiddle_bfly<T, L, RL, -1, NSL>(o, j, tw);
+          // slot 0 (the first sponge field, then its term, then field 0)
+          // waits in the park tile where the LDS allows
+          cplx<T>* pk = park + bb * LSP;
+          if constexpr (i >= 3) {
+            // scale, and add the field's sponge term
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+              const cplx<T> c = (PARK && slot == 0) ? pk[lp<T>(j + q * (L / RL))] : r[slot][p][q];
+              o[q] = sponge_fold<T>(o[q], c, (T)a.scale);
+            }
+          }
+          if constexpr (PARK && slot == 0) {
+#pragma unroll
+            for (int q = 0; q < RL; ++q) pk[lp<T>(j + q * (L / RL))] = o[q];
+          } else {
+#pragma unroll
+            for (int q = 0; q < RL; ++q) r[slot][p][q] = o[q];
+          }
+          if constexpr (i == 2) {
+            // all three sponge fields are in: turn them into the terms
+            const double kxs = a.kx[kx + mo.md];
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+              cplx<T> s0 = PARK ? pk[lp<T>(j + q * (L / RL))] : r[0][p][q];
+              sponge_term<T>(a, mo, kxs, b0 + bb, j + q * (L / RL), s0, r[1][p][q], r[2][p][q]);
+              if constexpr (PARK) pk[lp<T>(j + q * (L / RL))] = s0;
+              else r[0][p][q] = s0;
+              __builtin_amdgcn_sched_barrier(0);  // one position at a time
+            }
+          }
+        }
+      }
+      __syncthreads();
+    };
+    xform(std::integral_constant<int, 0>{});
+    xform(std::integral_constant<int, 1>{});
+    xform(std::integral_constant<int, 2>{});
+    xform(std::integral_constant<int, 3>{});
+    xform(std::integral_constant<int, 4>{});
+    xform(std::integral_constant<int, 5>{});
   }
 
   const double kxv = a.kx[kx + mo.md];
@@ -367,10 +496,10 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
     const int kzi = j + q * (L / RL);
     const int ky = b0 + bb;
     if (ky >= a.ny) continue;
-    cplx<T> rv[NF];
+    cplx<T> rv[3];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) rv[f] = (PARK && f == 0) ? park[bb * LSP + lp<T>(kzi)] : r[f][p][q];
-    zepilogue<T, SPONGE, MG>(a, mo, kx, ky, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
+    for (int f = 0; f < 3; ++f) rv[f] = (PARK && f == 0) ? park[bb * LSP + lp<T>(kzi)] : r[f][p][q];
+    zepilogue<T, false, MG, SPONGE>(a, mo, kx, ky, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
     if constexpr (EU > 0)
       if ((p * RL + q + 1) % EU == 0) __builtin_amdgcn_sched_barrier(0);
   }
@@ -437,22 +566,74 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
       x[k] = ok ? nq[mo.mc + f * a.fs + ((long)kx * a.nz + aa) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
     }
   };
-  load(0);
-  cplx<T> r[NF][8];
+  cplx<T> r[3][8];
+  if constexpr (!SPONGE) {
+    load(0);
 #pragma unroll
-  for (int f = 0; f < NF; ++f) {
+    for (int f = 0; f < NF; ++f) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int e = tid + k * NT, bb = e % TL, aa = e / TL;
-      tile[bb * LS + lp<T>(aa)] = x[k];
+      for (int k = 0; k < K; ++k) {
+        const int e = tid + k * NT, bb = e % TL, aa = e / TL;
+        tile[bb * LS + lp<T>(aa)] = x[k];
+      }
+      __syncthreads();
+      if (f + 1 < NF) load(f + 1);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) r[f][q] = ln[lp<T>(l + q * G)];
+      wave_sync();
+      twfft<T, L, -1>(r[f], tw, ln, l);
+      __syncthreads();
     }
-    __syncthreads();
-    if (f + 1 < NF) load(f + 1);
+  } else {
+    // transform order, folding of the sponge terms and per-transform copies
+    // of the thread index as in zstep_k
+    constexpr auto fld = [](int i) { return i < 3 ? 3 + i : i - 3; };
+    auto loadt = [&](int f, int t) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) r[f][q] = ln[lp<T>(l + q * G)];
-    wave_sync();
-    twfft<T, L, -1>(r[f], tw, ln, l);
-    __syncthreads();
+      for (int k = 0; k < K; ++k) {
+        const int e = t + k * NT, bb = e % TL, aa = e / TL;
+        const bool ok = b0 + bb < a.ny;
+        x[k] = ok ? nq[mo.mc + f * a.fs + ((long)kx * a.nz + aa) * a.ny + b0 + bb] : mk<T>(T(0), T(0));
+      }
+    };
+    loadt(fld(0), tid);
+    auto xform = [&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int f = fld(i);
+      int t = tid;
+      asm volatile("" : "+v"(t));
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int e = t + k * NT, bb = e % TL, aa = e / TL;
+        tile[bb * LS + lp<T>(aa)] = x[k];
+      }
+      __syncthreads();
+      if constexpr (i + 1 < NF) loadt(fld(i + 1), t);
+      cplx<T> o[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) o[q] = ln[lp<T>(l + q * G)];
+      wave_sync();
+      twfft<T, L, -1>(o, tw, ln, l);
+      if constexpr (i < 3) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r[i][q] = o[q];
+        if constexpr (i == 2) {
+          const double kxs = a.kx[kx + mo.md];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) sponge_term<T>(a, mo, kxs, b0 + line, l + q * G, r[0][q], r[1][q], r[2][q]);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r[f][q] = sponge_fold<T>(o[q], r[f][q], (T)a.scale);
+      }
+      __syncthreads();
+    };
+    xform(std::integral_constant<int, 0>{});
+    xform(std::integral_constant<int, 1>{});
+    xform(std::integral_constant<int, 2>{});
+    xform(std::integral_constant<int, 3>{});
+    xform(std::integral_constant<int, 4>{});
+    xform(std::integral_constant<int, 5>{});
   }
 
   const double kxv = a.kx[kx + mo.md];
@@ -464,10 +645,10 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
   if (ky < a.ny) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      cplx<T> rv[NF];
+      cplx<T> rv[3];
 #pragma unroll
-      for (int f = 0; f < NF; ++f) rv[f] = r[f][q];
-      zepilogue<T, SPONGE, MG>(a, mo, kx, ky, l + q * G, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
+      for (int f = 0; f < 3; ++f) rv[f] = r[f][q];
+      zepilogue<T, false, MG, SPONGE>(a, mo, kx, ky, l + q * G, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
     }
   }
   if (final_stage && a.energy_partial) {
@@ -482,10 +663,13 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
   }
 }
 
-static bool zwave_supported(int nz) { return nz == 64 || nz == 128 || nz == 256 || nz == 512; }
+// The in-wave kernel holds eight positions of three fields per lane; at
+// nz = 512 that exceeds its register budget (it spilled 48-130 bytes per lane
+// in every stepper family), so nz = 512 always takes the block kernel.
+bool zstep_wave_supported(int nz) { return nz == 64 || nz == 128 || nz == 256; }
 
 int zstep_blocks(int nz, int ny, int nxh, bool fp64, bool wave) {
-  const int TL = (wave && zwave_supported(nz)) ? (fp64 ? Tile<double>::TL : Tile<float>::TL)
+  const int TL = (wave && zstep_wave_supported(nz)) ? (fp64 ? Tile<double>::TL : Tile<float>::TL)
                                                : (fp64 ? zb_tl<double>(nz) : zb_tl<float>(nz));
   return ((ny + TL - 1) / TL) * nxh;
 }
@@ -503,7 +687,7 @@ static void run_zb(const ZStepArgs& a, hipStream_t s) {
 
 template <typename T, int L, bool SP, int MG>
 static void run_z(const ZStepArgs& a, hipStream_t s) {
-  if constexpr (L == 64 || L == 128 || L == 256 || L == 512) {
+  if constexpr (L == 64 || L == 128 || L == 256) {
     if (a.wave) {
       using Sh = ZWShape<T, L>;
       static bool winit = false;
