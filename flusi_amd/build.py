@@ -61,7 +61,23 @@ def build(force: bool = True, jobs: int = 4) -> str:
     os.replace(tmp, LIB)
     build_h5io()
     build_fortran()
+    build_probe()
     return LIB
+
+
+PROBE_LIB = os.path.join(HERE, "libflusi_probe.so")
+
+
+def build_probe() -> str:
+    """libflusi_probe.so: the HBM stream probe bench.py compares the passes
+    with (copy, read-only and write-only bandwidth of this device)."""
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-shared", os.path.join(CSRC, "probe.hip"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
 
 
 HDF5_DIR = os.environ.get("FLUSI_HDF5_DIR", "/opt/conda")
