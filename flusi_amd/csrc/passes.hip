@@ -87,9 +87,14 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
     for (int o = 0; o < nout; ++o) {
       const int op = o ? a.jobs[jb].op2 : a.jobs[jb].op;
       cplx<T>* out = reinterpret_cast<cplx<T>*>(mshift(o ? a.jobs[jb].out2 : a.jobs[jb].out, moff));
+      // FP32 at L >= 384: a copy of the thread index per transform, so that
+      // the LDS addresses and twiddles are recomputed in each iteration of
+      // the job loop instead of being kept live (spilled) across it
+      int tl = tid;
+      asm volatile("" : "+v"(tl));
 #pragma unroll
       for (int p = 0; p < PER0; ++p) {
-        const int beta = tid + p * NT;
+        const int beta = tl + p * NT;
         if (NB0 % NT == 0 || beta < NB0) {
           int bb, j;
           if (CONTIG_IN) { bb = beta / (L / R0); j = beta - bb * (L / R0); }
@@ -122,11 +127,11 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
       }
       if (PF && o == nout - 1 && jb + 1 < a.njobs) load(jb + 1, 0);
       __syncthreads();
-      Stages<T, L, TL, NT, PLAN_TILE, S, LSP, 1, NSTG - 1, R0>::run(buf, tw);
+      Stages<T, L, TL, NT, PLAN_TILE, S, LSP, 1, NSTG - 1, R0>::run(buf, tw, tl);
       constexpr int NSL = L / RL;
 #pragma unroll
       for (int p = 0; p < PERL; ++p) {
-        const int beta = tid + p * NT;
+        const int beta = tl + p * NT;
         if (NBL % NT == 0 || beta < NBL) {
           int bb, j;
           if (!CONTIG_IN) { bb = beta / (L / RL); j = beta - bb * (L / RL); }
