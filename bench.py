@@ -72,9 +72,9 @@ def pass_bytes(n, s, stepper="RK4"):
 
 def measured_traffic(n, kernel):
     """HBM bytes per launch of a pass from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the same counters
-    read for a device copy of known size); None when not measured for this
-    grid.  Lines of the file: `traffic <grid> <PASS> <bytes>`."""
+    (reads from the TCC request-size counters, writes from WRITE_SIZE, both
+    checked against the probe kernels of known size); None when not measured
+    for this grid.  Lines of the file: `traffic <grid> <PASS> <bytes>`."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_insect_fp32.md")
     if not os.path.exists(path):
         return None
@@ -378,7 +378,7 @@ def main():
                     help="RK4 steps of the FP32-mixed vs FP64 force/energy series (0: none)")
     ap.add_argument("--series-every", type=int, default=10, help="steps between samples of that series")
     ap.add_argument("--copy-calib", action="store_true",
-                    help="only run a 1 GiB device-to-device copy (counter calibration for HBM bytes)")
+                    help="only run the HBM probe on 1 GiB buffers (counter calibration for HBM bytes)")
     args = ap.parse_args()
     if args.copy_calib:
         # counter calibration: probe_read_k and probe_write_k stream exactly 1
