@@ -307,7 +307,7 @@ def run_ensemble(args):
                                              "sum(16 * 2 * t_RHS)"}
         else:
             out["cpu_baseline"] = {"value": None, "error": t}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_adjoint(args):
@@ -360,7 +360,31 @@ def run_adjoint(args):
         "fp32": {"forward_s": out["fp32"]["forward_s"], "backward_s": out["fp32"]["backward_s"]},
         "fp64": {"forward_s": out["fp64"]["forward_s"], "backward_s": out["fp64"]["backward_s"]},
     }
-    print(json.dumps(res), flush=True)
+    return res
+
+
+def other_forms():
+    """The ensemble and adjoint workloads in brief, so that the default run
+    records them too: the 48-member ensemble (5 graph-replayed RK2 steps after
+    2 warm-up steps) and the 128^3 adjoint gradient (400 RK4 steps forward and
+    backward in both precisions, central differences of its two largest
+    components).  Their own modes (--workload ensemble|adjoint) report more."""
+    sub = argparse.Namespace(steps=5, warmup=2, no_cpu=True, adj_steps=400, fd=2)
+    e = run_ensemble(sub)
+    a = run_adjoint(sub)
+    return {
+        "ensemble_48": {
+            "metric": e["metric"], "value": e["value"], "unit": e["unit"], "ms_per_step": e["ms_per_step"],
+            "steps": e["steps"], "workload": e["config"]["workload"],
+            "roofline": {k: e["roofline"][k] for k in ("kernel", "achieved", "frac", "avg_launch_ms")},
+            "step_roofline_frac": e["step_roofline_frac"], "batched_eager_gps": e["batched_eager_gps"],
+            "members_alone_gps": e["members_alone_gps"]},
+        "adjoint_128": {
+            "metric": a["metric"], "value": a["value"], "unit": a["unit"], "ms_per_step": a["ms_per_step"],
+            "steps": a["steps"], "workload": a["config"]["workload"], "J_fp64": a["J_fp64"], "J_fp32": a["J_fp32"],
+            "grad_fp32_rel_l2_vs_fp64": a["grad_fp32_rel_l2_vs_fp64"],
+            "fd_max_rel_err": (a["fd_check"] or {}).get("max_rel_err"), "fp64": a["fp64"], "fp32": a["fp32"]},
+    }
 
 
 def main():
@@ -377,6 +401,8 @@ def main():
     ap.add_argument("--series-steps", type=int, default=1000,
                     help="RK4 steps of the FP32-mixed vs FP64 force/energy series (0: none)")
     ap.add_argument("--series-every", type=int, default=10, help="steps between samples of that series")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the brief ensemble and adjoint runs of the default workload")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run the HBM probe on 1 GiB buffers (counter calibration for HBM bytes)")
     args = ap.parse_args()
@@ -390,10 +416,10 @@ def main():
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
-        run_ensemble(args)
+        print(json.dumps(run_ensemble(args)), flush=True)
         return
     if args.workload == "adjoint":
-        run_adjoint(args)
+        print(json.dumps(run_adjoint(args)), flush=True)
         return
     import flusi_amd
     n = args.grid
@@ -445,6 +471,9 @@ def main():
             # the headline figure: the worst of drag and lift over the series
             rel = [acc[c]["max_rel_err"] for c in ("drag_Fx", "lift_Fz") if acc[c]["max_rel_err"] is not None]
             out["force_rel_err_vs_fp64"] = max(rel) if rel else None
+
+    if not args.no_forms:
+        out["forms"] = other_forms()
 
     if not args.no_cpu:
         host = host_info()

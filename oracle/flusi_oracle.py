@@ -25,8 +25,10 @@ reference's own fixtures) and tests/test_oracle_pinning.py (the branches no
 fixture runs: the outlet_x sponge region against the fixture-pinned cavity
 predicate, its first column and its single-precision mask threshold; the
 ellipsoid_plain body against the insects_mask_jerry keys and the ellipsoid's
-volume; the sinusoidal mean flow's zero mode against the step's start time).
-No function is left without an independent check.
+volume; the sinusoidal mean flow's zero mode against the step's start time;
+the adjoint's differentiable insect drawing composed from single-part
+drawings of the reference's own mode).  No function is left without an
+independent check.
 """
 from __future__ import annotations
 
@@ -730,6 +732,9 @@ class Insect:
         self.body_moves = "yes"
         self.body_already_drawn = False
         self.cleaned_already_once = False
+        # differentiable drawing of the adjoint (build extension, no reference
+        # counterpart): see draw()
+        self.smooth_mode = False
         self.time = 0.0
         if self.BodyMotion not in ("tethered",):
             raise FlusiAbort(10625, f"BodyMotion {self.BodyMotion} not in scope")
@@ -819,11 +824,27 @@ class Insect:
 
     # ---- drawing -------------------------------------------------------
     def draw(self, g: Grid, mask, color, us):
-        """module_insects.f90:338-399 Draw_Insect, in place on (mask, color, us)."""
+        """module_insects.f90:338-399 Draw_Insect, in place on (mask, color, us).
+
+        With smooth_mode (the adjoint's differentiable drawing, a build
+        extension): everything is cleared and redrawn at every call, the wing
+        root taper is min(steps(-y, 0), steps(y, L_span)) instead of the
+        reference's switch at y = 0, and wherever a wing has a positive value
+        us is the part-weighted mean sum_w m_w us_w / (m_body + sum_w m_w)
+        (before the body-motion transform) instead of the winning part's."""
         p = self.p
         ddx = (g.dx, g.dy, g.dz)
         self.smooth = 1.0 * max(ddx)
         self.safety = 3.5 * self.smooth
+        if self.smooth_mode:
+            mask[...] = 0.0
+            color[...] = 0
+            us[...] = 0.0
+            self.cleaned_already_once = True
+            self._draw_body(g, mask, color, us)
+            self.body_already_drawn = True
+            self._draw_wings(g, mask, color, us)
+            return
         # delete_old_mask (module_insects.f90:844-883)
         if self.body_moves == "no" and self.cleaned_already_once:
             w = (color == self.color_l) | (color == self.color_r)
@@ -1038,11 +1059,20 @@ class Insect:
         """wings_geometry.f90:5-110 draw_insect_wings (stage I right then left
         wing, stage II transform of us to the global frame)."""
         s = self.st
+        if self.smooth_mode:
+            mbody = mask.copy()
+            self._wsum = np.zeros_like(mask)
+            self._usw = np.zeros((3,) + mask.shape)
         if self.RightWing == "yes":
             self._draw_wing(g, mask, color, us, self.color_r, s["M_wing_r"], self.x_pivot_r_b, s["rot_rel_r_w"])
         if self.LeftWing == "yes":
             self._draw_wing(g, mask, color, us, self.color_l, s["M_wing_l"], self.x_pivot_l_b, s["rot_rel_l_w"])
-        sel = ((color == self.color_l) | (color == self.color_r)) & (mask > 0.0)
+        if self.smooth_mode:
+            sel = self._wsum > 0.0
+            for i in range(3):
+                us[i][sel] = self._usw[i][sel] / (mbody[sel] + self._wsum[sel])
+        else:
+            sel = ((color == self.color_l) | (color == self.color_r)) & (mask > 0.0)
         if not sel.any():
             return
         Mb = s["M_body"]
@@ -1081,15 +1111,23 @@ class Insect:
         else:
             raise FlusiAbort(10626, f"WingShape {self.WingShape} not in scope")
         sel &= (xw[0] > x_bot - self.safety) & (xw[0] < x_top + self.safety)
-        y_tmp = np.where(xw[1] < 0.0, steps(-xw[1], 0.0, self.smooth), steps(xw[1], self.L_span, self.smooth))
+        if self.smooth_mode:
+            y_tmp = np.minimum(steps(-xw[1], 0.0, self.smooth), steps(xw[1], self.L_span, self.smooth))
+        else:
+            y_tmp = np.where(xw[1] < 0.0, steps(-xw[1], 0.0, self.smooth), steps(xw[1], self.L_span, self.smooth))
         z_tmp = steps(np.abs(xw[2]), 0.5 * self.WingThickness, self.smooth)
         x_tmp = np.where(xw[0] < 0.0, steps(-xw[0], -x_bot, self.smooth), steps(xw[0], x_top, self.smooth))
         mt = z_tmp * y_tmp * x_tmp
+        v = [rot[1] * xw[2] - rot[2] * xw[1], rot[2] * xw[0] - rot[0] * xw[2], rot[0] * xw[1] - rot[1] * xw[0]]
+        MT = Mw.T
+        if self.smooth_mode:
+            acc = sel & (mt > 0.0)
+            self._wsum[acc] += mt[acc]
+            for i in range(3):
+                self._usw[i][acc] += (mt * (MT[i, 0] * v[0] + MT[i, 1] * v[1] + MT[i, 2] * v[2]))[acc]
         sel &= (mask < mt) & (mt > 0.0)
         mask[sel] = mt[sel]
         color[sel] = cw
-        v = [rot[1] * xw[2] - rot[2] * xw[1], rot[2] * xw[0] - rot[0] * xw[2], rot[0] * xw[1] - rot[1] * xw[0]]
-        MT = Mw.T
         for i in range(3):
             us[i][sel] = (MT[i, 0] * v[0] + MT[i, 1] * v[1] + MT[i, 2] * v[2])[sel]
 
@@ -1118,6 +1156,23 @@ class Insect:
         MT = Mw.T
         for i in range(3):
             us[i][idx] = MT[i, 0] * v[0] + MT[i, 1] * v[1] + MT[i, 2] * v[2]
+
+    def set_controls(self, theta):
+        """The adjoint's controls (build extension): the Fourier coefficients
+        of from_file kinematics, shared by both wings.  For angle a in (phi,
+        alpha, theta) and n = nfft, control a*2n + i is ai(i) and
+        a*2n + n + i is bi(i), in the file's units."""
+        for k in (self.kine_r, self.kine_l):
+            if k.protocol != "from_file":
+                continue
+            if not k.initialized:
+                k._load_file(self.base_dir)
+            n = len(k.ai["phi"])
+            if len(theta) != 6 * n:
+                raise ValueError(f"{len(theta)} controls for nfft = {n}")
+            for a, ang in enumerate(("phi", "alpha", "theta")):
+                k.ai[ang] = [float(theta[a * 2 * n + i]) for i in range(n)]
+                k.bi[ang] = [float(theta[a * 2 * n + n + i]) for i in range(n)]
 
     def aero_inert_power(self, torque_part):
         """module_insects.f90:469-586 aero_power / inert_power."""
@@ -1406,6 +1461,25 @@ class Simulation:
             self.insect = Insect(self.ini, p, self.base_dir)
         if p.iPenalization == 1:
             self.create_mask(self.time)
+
+    def adjoint_mean_lift(self, theta, nsteps: int) -> float:
+        """The discrete adjoint's objective (build extension, no reference
+        counterpart): J = (1/K) sum_{n=1..K} F_z(t_n) over K steps from the
+        initial condition with the kinematics controls theta
+        (Insect.set_controls), F_z the z component of cal_drag's force
+        (drag.f90:17-454) with the mask that the step's last stage drew at
+        t_n and u = ifft3(uk(t_n)), the insect drawn in its differentiable
+        mode (Insect.draw)."""
+        self.init_fields()
+        self.insect.smooth_mode = True
+        self.insect.set_controls(theta)
+        self.create_mask(self.time)  # the initial mask with these controls
+        J = 0.0
+        for _ in range(nsteps):
+            self.step()
+            self.cal_drag(self.time, ifft3(self.g, self.uk))
+            J += float(self.last_integrals["force"][2]) / nsteps
+        return J
 
     def set_mean_flow(self, time):
         """fluid_time_step.f90:1066-1090.  The fixed mean flow is pinned by the

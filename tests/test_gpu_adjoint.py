@@ -1,14 +1,18 @@
 """The discrete adjoint on the GPU: d(mean lift)/d(wing kinematics
 coefficients) against central finite differences of the solver's own mean
-lift, checkpoint-interval independence, FP32-mixed against FP64, and the
-torch.autograd binding.  There is no reference counterpart: the finite
-differences pin it."""
+lift, the objective and two gradient components against the oracle's
+restatement of the same objective, checkpoint-interval independence,
+FP32-mixed against FP64, and the torch.autograd binding.  The reference has
+no adjoint; the oracle's objective is built from its fixture-pinned steppers
+and cal_drag and the differentiable drawing pinned in
+tests/test_oracle_pinning.py."""
 import os
 
 import numpy as np
 import pytest
 
 import flusi_amd
+from oracle import flusi_oracle as fo
 from tests._util import CONFIGS, rewrite_ini
 
 pytestmark = pytest.mark.gpu
@@ -39,6 +43,29 @@ def test_adjoint_gradient_matches_central_differences(tmp_path):
     assert np.max(np.abs(fd)) > 0.0
     assert np.max(np.abs(g - fd)) <= 1e-6 * np.max(np.abs(fd)), np.abs(g - fd) / np.max(np.abs(fd))
     s.close()
+
+
+def test_adjoint_objective_and_gradient_match_oracle(tmp_path):
+    """J of the GPU's forward sweep against the oracle's adjoint_mean_lift
+    with the same controls, and the adjoint gradient's two largest
+    components against central differences of the oracle's J."""
+    ini = _ini(tmp_path)
+    s = flusi_amd.Flusi(ini, flusi_amd.FP64)
+    th = s.adjoint_controls()
+    J = s.adjoint_forward(th, K, EVERY)
+    g = s.adjoint_backward()
+    s.close()
+    Jo = fo.Simulation(ini).adjoint_mean_lift(th, K)
+    print(f"J gpu {J!r} oracle {Jo!r} rel {abs(J - Jo) / abs(Jo):.3e}", flush=True)
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    h = 1.0e-4
+    for i in np.argsort(-np.abs(g))[:2]:
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (fo.Simulation(ini).adjoint_mean_lift(tp, K) - fo.Simulation(ini).adjoint_mean_lift(tm, K)) / (2 * h)
+        print(f"control {i}: adjoint {g[i]!r} oracle central difference {fd!r}", flush=True)
+        assert abs(g[i] - fd) <= 1e-6 * np.max(np.abs(g))
 
 
 def test_adjoint_checkpoint_interval_and_precision(tmp_path):

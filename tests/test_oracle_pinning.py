@@ -192,3 +192,89 @@ def test_sinusoidal_mean_flow_zero_mode(tmp_path, perturb):
             rest[:, 0, 0, 0] = 0.0
             assert not rest.any()
             assert sim.uk[1, 0, 0, 0] == 0.0
+
+
+# ------------------------------------------------- adjoint's smooth drawing
+def _cfg4_draw(tmp_path, name, t, smooth, **over):
+    """One drawing of the cfg4 insect (ellipsoid_plain body, two TwoEllipses
+    wings, from_file kinematics) at 40^3 and time t."""
+    ini = rewrite_ini(os.path.join(CONFIGS, "cfg4_insect_adjoint_128.ini"), str(tmp_path), name=name,
+                      Resolution__nx=40, Resolution__ny=40, Resolution__nz=40, **over)
+    sim = fo.Simulation(ini)
+    sim.init_fields()
+    sim.insect.smooth_mode = smooth
+    sim.create_mask(t)
+    return sim
+
+
+@pytest.mark.parametrize("t", [0.0, 0.0375, 0.21])
+def test_smooth_drawing_composes_single_parts(tmp_path, t):
+    """The differentiable drawing against the reference's own drawing of each
+    part alone.  Its mask is the largest part value, where each wing's value
+    is the reference's times the root factor steps(-y, 0) in [1/2, 1] (1 off
+    the root slab); its us is sum_w m_w us_w / (m_body + sum_w m_w) with us_w
+    the reference's single-wing velocity (the body is tethered at rest, so
+    the body-motion transform is the identity here).  A wrong weight, a
+    missing body term or a taper on the wrong side fails one of these."""
+    full = _cfg4_draw(tmp_path, "full.ini", t, True)
+    body = _cfg4_draw(tmp_path, "body.ini", t, False, Insects__LeftWing="no", Insects__RightWing="no")
+    right = _cfg4_draw(tmp_path, "right.ini", t, False, Insects__LeftWing="no", Insects__BodyType="nobody")
+    left = _cfg4_draw(tmp_path, "left.ini", t, False, Insects__RightWing="no", Insects__BodyType="nobody")
+    right_s = _cfg4_draw(tmp_path, "right_s.ini", t, True, Insects__LeftWing="no", Insects__BodyType="nobody")
+    left_s = _cfg4_draw(tmp_path, "left_s.ini", t, True, Insects__RightWing="no", Insects__BodyType="nobody")
+    ins = full.insect
+    assert np.all(ins.st["M_body"] == np.eye(3)) and not np.any(ins.st["vc"]) and not np.any(ins.st["rot_body_b"])
+    mb = body.mask
+    for std, sm in ((right, right_s), (left, left_s)):
+        # root factor: equal off the root slab, between 1/2 and 1 on it, and
+        # the slab is thin (about one grid spacing of the span)
+        r = np.where(std.mask > 0, sm.mask / np.where(std.mask > 0, std.mask, 1.0), 1.0)
+        assert np.all(sm.mask[std.mask == 0] == 0)
+        assert r.min() >= 0.5 - 1e-12 and r.max() <= 1.0 + 1e-12
+        changed = (r < 1.0) & (std.mask > 0)
+        assert changed.any() and changed.sum() < 0.2 * (std.mask > 0).sum()
+        # a wing drawn alone moves its points with the reference's velocity
+        # (to rounding: (m v) / m)
+        on = sm.mask > 0
+        np.testing.assert_allclose(sm.us[:, on], std.us[:, on], rtol=1e-14, atol=1e-15)
+    mr, ml = right_s.mask, left_s.mask
+    np.testing.assert_array_equal(full.mask, np.maximum(mb, np.maximum(mr, ml)))
+    wsum = mr + ml
+    on = wsum > 0
+    assert on.any() and (on & (mb > 0)).any()
+    want = np.zeros_like(full.us)
+    for i in range(3):
+        want[i][on] = (mr[on] * right.us[i][on] + ml[on] * left.us[i][on]) / (mb[on] + wsum[on])
+    np.testing.assert_allclose(full.us, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    # where no wing reaches, us is the tethered body's: zero
+    assert not full.us[:, ~on].any()
+
+
+def test_adjoint_objective_is_mean_of_cal_drag_lift(tmp_path):
+    """adjoint_mean_lift with the file's own controls equals the mean over
+    the steps of cal_drag's z force taken after each step by a plain stepping
+    loop with the same differentiable drawing (the objective adds nothing to
+    the fixture-pinned cal_drag and steppers), and the controls reach the
+    kinematics: moving one coefficient moves J."""
+    ini = rewrite_ini(os.path.join(CONFIGS, "cfg4_insect_adjoint_128.ini"), str(tmp_path),
+                      Resolution__nx=24, Resolution__ny=24, Resolution__nz=24,
+                      Time__dt_fixed="4.0e-4", Penalization__eps="1.0e-2")
+    sim = fo.Simulation(ini)
+    sim.init_fields()
+    k = sim.insect.kine_r
+    th = np.array([v for ang in ("phi", "alpha", "theta") for v in (k.ai[ang] + k.bi[ang])])
+    assert th.size == 12
+    J = fo.Simulation(ini).adjoint_mean_lift(th, 3)
+    ref = fo.Simulation(ini)
+    ref.init_fields()
+    ref.insect.smooth_mode = True
+    ref.create_mask(ref.time)
+    fz = []
+    for _ in range(3):
+        ref.step()
+        ref.cal_drag(ref.time, fo.ifft3(ref.g, ref.uk))
+        fz.append(ref.last_integrals["force"][2])
+    assert J == pytest.approx(sum(fz) / 3, rel=1e-14)
+    th2 = th.copy()
+    th2[0] += 5.0  # phi amplitude of the first harmonic, degrees
+    assert abs(fo.Simulation(ini).adjoint_mean_lift(th2, 3) - J) > 1e-6 * abs(J)
