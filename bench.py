@@ -1,7 +1,8 @@
 """Benchmark: grid-point-steps/s of the tethered flapping-insect workload
 (512^3, RK4 with explicit diffusion, insect mask and us regenerated in every
-stage) on one MI355X, FP32-mixed, with the FP64 path and the FP64 CPU oracle
-beside it.
+stage) on one MI355X, FP32-mixed, with the FP64 path and the oracle's FP64
+C++ OpenMP restatement on the host cores beside it, plus brief runs of the
+ensemble and adjoint workloads.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 512] [--no-cpu] [--no-fp64]
     python bench.py --workload ensemble [--steps K] [--warmup W] [--no-cpu]
@@ -185,28 +186,24 @@ def host_info():
     return {"cpu_model": model, "logical_cpus": cpus, "ram_gib": ram}
 
 
-def cpu_baseline(n, workers):
-    """One right-hand side (cal_nlk incl. the insect mask) of the same workload
-    with the FP64 oracle in a child process; steps/s = 1 / (4 t_RHS)."""
-    code = f"""
-import os, sys, time, json
-sys.path.insert(0, {ROOT!r})
-from oracle import flusi_oracle as fo
-import bench
-d = bench.tempfile.mkdtemp()
-ini = bench.make_ini(d, {n}, 0)
-sim = fo.Simulation(ini, workers={workers})
-sim.init_fields()
-t0 = time.perf_counter()
-sim.cal_nlk(0.0, sim.uk)
-t1 = time.perf_counter()
-print(json.dumps({{"t_rhs": t1 - t0}}))
-"""
-    env = dict(os.environ)
-    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=900)
+def cpu_baseline(n, threads):
+    """RK4 steps of the same workload by the oracle's C++ OpenMP restatement
+    (oracle/cpu/flusi_cpu: own FFT, FP64, the reference's operation order) on
+    `threads` host threads, from the same quiescent start with the mask drawn
+    at every stage; returns (seconds per step, set-up seconds, steps) or
+    (None, error)."""
+    exe = os.path.join(ROOT, "oracle", "cpu", "flusi_cpu")
+    if not os.path.exists(exe):
+        return None, "oracle/cpu/flusi_cpu is not built"
+    d = tempfile.mkdtemp()
+    ini = make_ini(d, n, 0)
+    steps = 1
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([exe, ini, str(steps), str(threads)], capture_output=True, text=True, env=env, timeout=900)
     if r.returncode != 0:
         return None, r.stderr[-400:]
-    return json.loads(r.stdout.strip().splitlines()[-1])["t_rhs"], None
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    return (info["s_per_step"], info["setup_s"], steps), None
 
 
 def ensemble_cpu_baseline(sizes, per_size, workers):
@@ -477,14 +474,16 @@ def main():
 
     if not args.no_cpu:
         host = host_info()
-        workers = max(1, min(16, host["logical_cpus"] or 1))
-        t_rhs, err = cpu_baseline(n, workers)
-        if t_rhs:
-            out["cpu_baseline"] = {"value": N / (4.0 * t_rhs), "unit": "grid-pt·steps/s", "cores": workers,
+        threads = max(1, host["logical_cpus"] or 1)
+        res, err = cpu_baseline(n, threads)
+        if res:
+            sps, setup, k = res
+            out["cpu_baseline"] = {"value": N / sps, "unit": "grid-pt·steps/s", "cores": threads,
                                    "kind": "port", "host": host,
-                                   "sample": f"one FP64 right-hand side (cal_nlk with the insect mask drawn) of the "
-                                             f"{n}^3 workload by the numpy/scipy oracle on {workers} threads, "
-                                             f"t_RHS={t_rhs:.2f}s; steps/s = 1/(4 t_RHS) (an RK4 step is four)"}
+                                   "sample": f"{k} FP64 RK4 step of the {n}^3 workload (four right-hand sides, the "
+                                             f"insect mask drawn at each stage) by the oracle's C++ OpenMP "
+                                             f"restatement (oracle/cpu/flusi_cpu.cpp, own FFT) on {threads} threads: "
+                                             f"{sps:.2f} s per step after {setup:.1f} s of set-up"}
         else:
             out["cpu_baseline"] = {"value": None, "error": err}
     print(json.dumps(out), flush=True)
