@@ -39,6 +39,25 @@ template <int S, typename C> __device__ __forceinline__ C mul_si(C a) {
 }
 template <typename C, typename R> __device__ __forceinline__ C scl(C a, R s) { C r; r.x = a.x * s; r.y = a.y * s; return r; }
 
+// ------------------------------------------------------------ global memory
+// Pointers in the global address space.  A device pointer that reaches a
+// kernel through an argument struct (with batch-member offsets applied) stays
+// generic to the compiler, and generic ("flat") accesses count against the
+// LDS wait counter as well as the vector-memory one: every LDS wait of an
+// in-wave transform would then also wait for the pass's outstanding loads and
+// stores.  Accesses through these types are global loads and stores.
+template <typename X> using gptr = X __attribute__((address_space(1)))*;
+template <typename T> struct GWord { using t = T __attribute__((ext_vector_type(2))); };
+template <> struct GWord<float> { using t = double; };
+// complex loads and stores (the vector types' own assignment does not bind
+// across address spaces)
+template <typename T> __device__ __forceinline__ cplx<T> gld(gptr<const cplx<T>> p) {
+  return __builtin_bit_cast(cplx<T>, *reinterpret_cast<gptr<const typename GWord<T>::t>>(p));
+}
+template <typename T> __device__ __forceinline__ void gst(gptr<cplx<T>> p, cplx<T> v) {
+  *reinterpret_cast<gptr<typename GWord<T>::t>>(p) = __builtin_bit_cast(typename GWord<T>::t, v);
+}
+
 // ---------------------------------------------------------------- LDS padding
 template <typename T> struct Pad { static constexpr int PS = (sizeof(T) == 4) ? 4 : 3; };
 template <typename T> __host__ __device__ constexpr int lp(int a) { return a + (a >> Pad<T>::PS); }

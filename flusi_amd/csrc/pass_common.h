@@ -33,6 +33,16 @@ template <typename T> __device__ __forceinline__ void sts(cplx<T>* p, cplx<T> v)
   }
 }
 
+// the same through global-address-space pointers (see gptr)
+template <typename T> __device__ __forceinline__ cplx<T> lds(gptr<const cplx<T>> p) {
+  using W = typename GWord<T>::t;
+  return __builtin_bit_cast(cplx<T>, __builtin_nontemporal_load(reinterpret_cast<gptr<const W>>(p)));
+}
+template <typename T> __device__ __forceinline__ void sts(gptr<cplx<T>> p, cplx<T> v) {
+  using W = typename GWord<T>::t;
+  __builtin_nontemporal_store(__builtin_bit_cast(W, v), reinterpret_cast<gptr<W>>(p));
+}
+
 template <typename C> __device__ __forceinline__ C conj_(C a) { a.y = -a.y; return a; }
 template <typename T> __device__ __forceinline__ cplx<T> ik_(T k, cplx<T> v) { return mk<T>(-k * v.y, k * v.x); }
 

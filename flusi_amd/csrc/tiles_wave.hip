@@ -14,6 +14,7 @@
 #include "flusi_kernels.h"
 #include "pass_common.h"
 #include "wave_fft.h"
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 
@@ -61,22 +62,25 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
   const int nbs = a.nb_store > a.nb ? a.nb_store : a.nb;  // lines >= nb hold zeros
   const long moff = (long)blockIdx.z * a.bat.mstride;  // batch member's slab
   const T sc = (T)a.scale;
-  const cplx<T>* tw;
+  // twiddles: the block's LDS copy, or global memory where it does not fit
+  std::conditional_t<Sh::TWL, const cplx<T>*, gptr<const cplx<T>>> tw;
   if constexpr (Sh::TWL) {
     cplx<T>* twl = tile + TL * LS;
-    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(mshift(a.tw, moff));
-    for (int i = tid; i < L; i += NT) twl[i] = twg[i];
+    const gptr<const cplx<T>> twg = (gptr<const cplx<T>>)mshift(a.tw, moff);
+    for (int i = tid; i < L; i += NT) twl[i] = gld<T>(twg + i);
     tw = twl;
     __syncthreads();
   } else {
-    tw = reinterpret_cast<const cplx<T>*>(mshift(a.tw, moff));
+    tw = (gptr<const cplx<T>>)mshift(a.tw, moff);
   }
 
   // operands: CONTIG_IN - the lane's 8 positions of its own line;
   // otherwise - the thread's K tile elements (bb = e % TL, a = e / TL)
   cplx<T> r0[8], r1[8];
   auto load = [&](int jb, int which) {
-    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(mshift(which ? a.jobs[jb].in1 : a.jobs[jb].in0, moff));
+    // global-address-space views (see gptr): the prefetched loads then do
+    // not hold up the transform's LDS waits
+    const gptr<const cplx<T>> in = (gptr<const cplx<T>>)mshift(which ? a.jobs[jb].in1 : a.jobs[jb].in0, moff);
     if (CONTIG_IN) {
       const bool ok = b0 + line < a.nb;
       const long base = c * a.sin.c + (long)(b0 + line) * a.sin.b + l;
@@ -102,7 +106,7 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
     if (a.jobs[jb].in1) load(jb, 1);
     for (int o = 0; o < nout; ++o) {
       const int op = o ? a.jobs[jb].op2 : a.jobs[jb].op;
-      cplx<T>* out = reinterpret_cast<cplx<T>*>(mshift(o ? a.jobs[jb].out2 : a.jobs[jb].out, moff));
+      const gptr<cplx<T>> out = (gptr<cplx<T>>)mshift(o ? a.jobs[jb].out2 : a.jobs[jb].out, moff);
       const bool pf = (o == nout - 1) && (jb + 1 < a.njobs);
       cplx<T> v[8];
       if (CONTIG_IN) {

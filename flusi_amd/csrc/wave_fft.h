@@ -34,12 +34,15 @@ template <int L> struct TWPlan {
 template <int S, typename C> __device__ __forceinline__ C twm(C v, C w) {
   return (S < 0) ? cmul(v, w) : conjmul(v, w);
 }
+// twiddle i from an LDS table or from global memory
+template <typename T> __device__ __forceinline__ cplx<T> twv(const cplx<T>* p, int i) { return p[i]; }
+template <typename T> __device__ __forceinline__ cplx<T> twv(gptr<const cplx<T>> p, int i) { return gld<T>(p + i); }
 
 // Stage s: B = 8/R butterflies per lane, j = l + bG; input q of butterfly b
 // is register b + qB (position j + q L/R); outputs go to
 // (j/Ns) Ns R + j%Ns + q Ns, which for the last stage is again register b + qB.
-template <typename T, int L, int S, int s>
-__device__ __forceinline__ void twstage(cplx<T> (&v)[8], const cplx<T>* tw, cplx<T>* ln, int l) {
+template <typename T, int L, int S, int s, typename TWP>
+__device__ __forceinline__ void twstage(cplx<T> (&v)[8], TWP tw, cplx<T>* ln, int l) {
   constexpr int G = TWPlan<L>::G;
   constexpr int R = TWPlan<L>::radix(s), Ns = TWPlan<L>::ns(s), B = 8 / R;
   constexpr bool last = (s == TWPlan<L>::NST - 1);
@@ -52,7 +55,7 @@ __device__ __forceinline__ void twstage(cplx<T> (&v)[8], const cplx<T>* tw, cplx
     if (s > 0) {
       const int step = (j % Ns) * (L / (Ns * R));
 #pragma unroll
-      for (int q = 1; q < R; ++q) x[q] = twm<S>(x[q], tw[q * step]);
+      for (int q = 1; q < R; ++q) x[q] = twm<S>(x[q], twv<T>(tw, q * step));
     }
     Bfly<R, S, T>::run(x);
     if (last) {
@@ -72,8 +75,8 @@ __device__ __forceinline__ void twstage(cplx<T> (&v)[8], const cplx<T>* tw, cplx
   }
 }
 
-template <typename T, int L, int S>
-__device__ __forceinline__ void twfft(cplx<T> (&v)[8], const cplx<T>* tw, cplx<T>* ln, int l_) {
+template <typename T, int L, int S, typename TWP>
+__device__ __forceinline__ void twfft(cplx<T> (&v)[8], TWP tw, cplx<T>* ln, int l_) {
   int l = l_;
   asm volatile("" : "+v"(l));  // recompute addresses per transform (register pressure)
   twstage<T, L, S, 0>(v, tw, ln, l);

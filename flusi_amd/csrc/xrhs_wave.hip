@@ -201,32 +201,33 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
   const bool live = lane < LPW * G && gl < a.nlines;
   cplx<T>* ln = sh[wv] + (lane / G) * LS;
 
-  const cplx<T>* twM = reinterpret_cast<const cplx<T>*>(a.twM);
-  const cplx<T>* twN = reinterpret_cast<const cplx<T>*>(a.twN);
-  const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in);
-  cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out);
-  const T* mask = reinterpret_cast<const T*>(a.mask);
-  const T* us = reinterpret_cast<const T*>(a.us);
+  // global-address-space views of the arguments (see gptr)
+  const gptr<const cplx<T>> twM = (gptr<const cplx<T>>)a.twM;
+  const gptr<const cplx<T>> twN = (gptr<const cplx<T>>)a.twN;
+  const gptr<const cplx<T>> in = (gptr<const cplx<T>>)a.in;
+  const gptr<cplx<T>> out = (gptr<cplx<T>>)a.out;
+  const gptr<const T> mask = (gptr<const T>)a.mask;
+  const gptr<const T> us = (gptr<const T>)a.us;
 
-  for (int i = threadIdx.x; i < M; i += XW_NT) twl[i] = twM[i];
-  for (int i = threadIdx.x; i < NXH; i += XW_NT) twn[i] = twN[i];
+  for (int i = threadIdx.x; i < M; i += XW_NT) twl[i] = gld<T>(twM + i);
+  for (int i = threadIdx.x; i < NXH; i += XW_NT) twn[i] = gld<T>(twN + i);
 
   // ---- loads: six spectra at k = l + qG and k = M
   cplx<T> X[6][4], XM[6];
   const long lb = (long)(live ? gl : 0) * a.prow;
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
-    const cplx<T>* p = in + f * a.fstride_in + lb;
+    const gptr<const cplx<T>> p = in + f * a.fstride_in + lb;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) X[f][q] = p[l + q * G];
-    XM[f] = p[M];
+    for (int q = 0; q < 4; ++q) X[f][q] = gld<T>(p + l + q * G);
+    XM[f] = gld<T>(p + M);
   }
 
   // mask at the physical points 2n, 2n+1 (n = l + qG), read early
   cplx<T> mk2[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
-    mk2[q] = *reinterpret_cast<const cplx<T>*>(mask + (long)(live ? gl : 0) * a.nx + 2 * (l + q * G));
+    mk2[q] = gld<T>(reinterpret_cast<gptr<const cplx<T>>>(mask + (long)(live ? gl : 0) * a.nx + 2 * (l + q * G)));
 
   __syncthreads();  // twiddle tables (the only block-wide wait besides the statistics)
 
@@ -357,18 +358,18 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
     }
     wave_sync();
     if (live) {
-      cplx<T>* o = out + c * a.fstride_out + (long)gl * a.prow;
+      const gptr<cplx<T>> o = out + c * a.fstride_out + (long)gl * a.prow;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const cplx<T> zk = O[c][q];
         const cplx<T> s1 = add(zk, conj_w(zm[q])), d1 = sub(zk, conj_w(zm[q]));
-        o[l + q * G] = scl(sub(s1, ik_w<T>(T(1), cmul(twn[l + q * G], d1))), half);
+        gst<T>(o + l + q * G, scl(sub(s1, ik_w<T>(T(1), cmul(twn[l + q * G], d1))), half));
       }
       if (l == 0) {
         // k = M: partner of k = 0 is itself
         const cplx<T> zk = O[c][0];
         const cplx<T> s2 = add(zk, conj_w(zk)), d2 = sub(zk, conj_w(zk));
-        o[M] = scl(sub(s2, ik_w<T>(T(1), cmul(twn[M], d2))), half);
+        gst<T>(o + M, scl(sub(s2, ik_w<T>(T(1), cmul(twn[M], d2))), half));
       }
     }
   }
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
         r[0] = nanmax(r[0], red[w][0]);
         for (int c = 1; c < 4; ++c) r[c] += red[w][c];
       }
-      for (int c = 0; c < 4; ++c) a.partial[(long)blockIdx.x * 4 + c] = r[c];
+      for (int c = 0; c < 4; ++c) ((gptr<double>)a.partial)[(long)blockIdx.x * 4 + c] = r[c];
     }
   }
 }
