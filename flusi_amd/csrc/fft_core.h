@@ -57,6 +57,9 @@ template <typename T> __device__ __forceinline__ cplx<T> gld(gptr<const cplx<T>>
 template <typename T> __device__ __forceinline__ void gst(gptr<cplx<T>> p, cplx<T> v) {
   *reinterpret_cast<gptr<typename GWord<T>::t>>(p) = __builtin_bit_cast(typename GWord<T>::t, v);
 }
+// twiddle i from an LDS (or generic) table or from global memory
+template <typename T> __device__ __forceinline__ cplx<T> twv(const cplx<T>* __restrict__ p, int i) { return p[i]; }
+template <typename T> __device__ __forceinline__ cplx<T> twv(gptr<const cplx<T>> p, int i) { return gld<T>(p + i); }
 
 // ---------------------------------------------------------------- LDS padding
 template <typename T> struct Pad { static constexpr int PS = (sizeof(T) == 4) ? 4 : 3; };
@@ -136,13 +139,13 @@ template <int S, typename T> struct Bfly<8, S, T> {
 // j + q L/R, multiplies input q by W_{Ns R}^{q k} (k = j mod Ns), applies the
 // radix-R DFT and writes positions (j/Ns) Ns R + k + q Ns.  tw[m] = exp(-2 pi
 // i m / L); the inverse (S = +1) uses its conjugate.
-template <typename T, int L, int R, int S, int Ns>
-__device__ __forceinline__ void twiddle_bfly(cplx<T>* v, int j, const cplx<T>* __restrict__ tw) {
+template <typename T, int L, int R, int S, int Ns, typename TWP>
+__device__ __forceinline__ void twiddle_bfly(cplx<T>* v, int j, TWP tw) {
   if (Ns > 1) {
     const int step = (j % Ns) * (L / (Ns * R));
 #pragma unroll
     for (int q = 1; q < R; ++q) {
-      const cplx<T> w = tw[q * step];
+      const cplx<T> w = twv<T>(tw, q * step);
       v[q] = (S < 0) ? cmul(v[q], w) : conjmul(v[q], w);
     }
   }
@@ -157,8 +160,8 @@ template <int R, int Ns> __device__ __forceinline__ int stage_out(int j, int q) 
 // one kernel can pass a copy the compiler cannot see through, so that the
 // addresses and twiddles derived from it are recomputed per transform rather
 // than kept live (in registers, or spilled) from the first to the last.
-template <typename T, int L, int NL, int NT, int R, int S, int Ns, int LSP>
-__device__ __forceinline__ void lds_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw, int tid) {
+template <typename T, int L, int NL, int NT, int R, int S, int Ns, int LSP, typename TWP>
+__device__ __forceinline__ void lds_stage(cplx<T>* buf, TWP tw, int tid) {
   constexpr int NB = NL * (L / R);
   constexpr int PER = (NB + NT - 1) / NT;
   cplx<T> v[PER][R];
@@ -193,7 +196,8 @@ template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE
 struct Stages;
 template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE, int LAST, int NS>
 struct Stages<T, L, NL, NT, KIND, S, LSP, STAGE, LAST, NS, false> {
-  static __device__ __forceinline__ void run(cplx<T>* buf, const cplx<T>* tw, int tid = threadIdx.x) {
+  template <typename TWP>
+  static __device__ __forceinline__ void run(cplx<T>* buf, TWP tw, int tid = threadIdx.x) {
     constexpr int R = plan_radix_k(KIND, L, STAGE);
     lds_stage<T, L, NL, NT, R, S, NS, LSP>(buf, tw, tid);
     Stages<T, L, NL, NT, KIND, S, LSP, STAGE + 1, LAST, NS * R>::run(buf, tw, tid);
@@ -201,7 +205,7 @@ struct Stages<T, L, NL, NT, KIND, S, LSP, STAGE, LAST, NS, false> {
 };
 template <typename T, int L, int NL, int NT, int KIND, int S, int LSP, int STAGE, int LAST, int NS>
 struct Stages<T, L, NL, NT, KIND, S, LSP, STAGE, LAST, NS, true> {
-  static __device__ __forceinline__ void run(cplx<T>*, const cplx<T>*, int = 0) {}
+  template <typename TWP> static __device__ __forceinline__ void run(cplx<T>*, TWP, int = 0) {}
 };
 
 // product of the radices of stages [0, s) of a plan

@@ -50,13 +50,14 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
   const int b0 = blockIdx.x * TL;
   const long c = blockIdx.y;
   const long moff = (long)blockIdx.z * a.bat.mstride;  // batch member's slab
-  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(mshift(a.tw, moff));
+  // global-address-space views of the arguments (see gptr)
+  const gptr<const cplx<T>> tw = (gptr<const cplx<T>>)mshift(a.tw, moff);
   const T sc = (T)a.scale;
 
   cplx<T> r0[PER0][R0], r1[PER0][R0];
   // operand `which` (0: in0, prefetched; 1: in1, read when its job starts)
   auto load = [&](int jb, int which) {
-    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(mshift(which ? a.jobs[jb].in1 : a.jobs[jb].in0, moff));
+    const gptr<const cplx<T>> in = (gptr<const cplx<T>>)mshift(which ? a.jobs[jb].in1 : a.jobs[jb].in0, moff);
 #pragma unroll
     for (int p = 0; p < PER0; ++p) {
       const int beta = tid + p * NT;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
         const long step = CONTIG_IN ? (long)(L / R0) : (long)(L / R0) * a.sin.a;
 #pragma unroll
         for (int q = 0; q < R0; ++q) {
-          const cplx<T> x = ok ? in[base + q * step] : mk<T>(T(0), T(0));
+          const cplx<T> x = ok ? gld<T>(in + base + q * step) : mk<T>(T(0), T(0));
           if (which) r1[p][q] = x;
           else r0[p][q] = x;
         }
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
     if (a.jobs[jb].in1) load(jb, 1);
     for (int o = 0; o < nout; ++o) {
       const int op = o ? a.jobs[jb].op2 : a.jobs[jb].op;
-      cplx<T>* out = reinterpret_cast<cplx<T>*>(mshift(o ? a.jobs[jb].out2 : a.jobs[jb].out, moff));
+      const gptr<cplx<T>> out = (gptr<cplx<T>>)mshift(o ? a.jobs[jb].out2 : a.jobs[jb].out, moff);
       // FP32 at L >= 384: a copy of the thread index per transform, so that
       // the LDS addresses and twiddles are recomputed in each iteration of
       // the job loop instead of being kept live (spilled) across it
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(NT) void tile_pass_k(TilePassArgs a) {
                                          : c * a.sout.c + (long)(b0 + bb) + (long)j * a.sout.a;
             const long step = !CONTIG_IN ? (long)(L / RL) : (long)(L / RL) * a.sout.a;
 #pragma unroll
-            for (int q = 0; q < RL; ++q) out[base + q * step] = mk<T>(w[q].x * sc, w[q].y * sc);
+            for (int q = 0; q < RL; ++q) gst<T>(out + base + q * step, mk<T>(w[q].x * sc, w[q].y * sc));
           }
         }
       }
@@ -250,20 +251,22 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
   __shared__ double red[NT / 64][4];
   const int tid = threadIdx.x;
   const int l0 = blockIdx.x * TLX;
-  const cplx<T>* twM = reinterpret_cast<const cplx<T>*>(a.twM);
-  const cplx<T>* twN = reinterpret_cast<const cplx<T>*>(a.twN);
+  // global-address-space views of the arguments (see gptr)
+  const gptr<const cplx<T>> twM = (gptr<const cplx<T>>)a.twM;
+  const gptr<const cplx<T>> twN = (gptr<const cplx<T>>)a.twN;
+  const gptr<const double> kxg = (gptr<const double>)a.kx;
   auto L_ = [&](int f, int line) { return buf + (f * TLX + line) * LSP; };
 
   if (MODE != XM_FWD) {
     // ---- A
-    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in);
+    const gptr<const cplx<T>> in = (gptr<const cplx<T>>)a.in;
 #pragma unroll
     for (int f = 0; f < NFI; ++f)
       for (int e = tid; e < TLX * nxh; e += NT) {
         const int line = e / nxh, k = e - line * nxh;
         const int gl = l0 + line;
         cplx<T> v = mk<T>(T(0), T(0));
-        if (gl < a.nlines) v = in[f * a.fstride_in + (long)gl * a.prow + k];
+        if (gl < a.nlines) v = gld<T>(in + f * a.fstride_in + (long)gl * a.prow + k);
         L_(f, line)[lp<T>(k)] = v;
       }
     __syncthreads();
@@ -283,18 +286,18 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
           cplx<T> xa, xb;
           if ((MODE == XM_RHS || MODE == XM_INV_UW) && f == 4) {
             // w_y = dz u_x - dx u_z = A - i kx V_z
-            xa = sub(L_(4, line)[lp<T>(k)], ik_<T>((T)a.kx[k], L_(2, line)[lp<T>(k)]));
-            xb = sub(L_(4, line)[lp<T>(k2)], ik_<T>((T)a.kx[k2], L_(2, line)[lp<T>(k2)]));
+            xa = sub(L_(4, line)[lp<T>(k)], ik_<T>((T)kxg[k], L_(2, line)[lp<T>(k)]));
+            xb = sub(L_(4, line)[lp<T>(k2)], ik_<T>((T)kxg[k2], L_(2, line)[lp<T>(k2)]));
           } else if ((MODE == XM_RHS || MODE == XM_INV_UW) && f == 5) {
             // w_z = dx u_y - dy u_x = i kx V_y - B
-            xa = sub(ik_<T>((T)a.kx[k], L_(1, line)[lp<T>(k)]), L_(5, line)[lp<T>(k)]);
-            xb = sub(ik_<T>((T)a.kx[k2], L_(1, line)[lp<T>(k2)]), L_(5, line)[lp<T>(k2)]);
+            xa = sub(ik_<T>((T)kxg[k], L_(1, line)[lp<T>(k)]), L_(5, line)[lp<T>(k)]);
+            xb = sub(ik_<T>((T)kxg[k2], L_(1, line)[lp<T>(k2)]), L_(5, line)[lp<T>(k2)]);
           } else {
             xa = L_(f, line)[lp<T>(k)];
             xb = L_(f, line)[lp<T>(k2)];
           }
           if (k == 0) { xa.y = T(0); xb.y = T(0); }
-          const cplx<T> ea = conj_(twN[k]);
+          const cplx<T> ea = conj_(twv<T>(twN, k));
           const cplx<T> s1 = add(xa, conj_(xb)), d1 = sub(xa, conj_(xb));
           v[p][q] = add(s1, ik_<T>(T(1), cmul(ea, d1)));
         }
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
 
   if (MODE == XM_INV_UW || MODE == XM_INV) {
     // inverse last stage straight to the physical field
-    cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out);
+    const gptr<cplx<T>> out = (gptr<cplx<T>>)a.out;
     constexpr int NB = NFI * TLX * (M / RM), PER = (NB + NT - 1) / NT;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
@@ -338,7 +341,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
         const int gl = l0 + line;
         if (gl < a.nlines) {
 #pragma unroll
-          for (int q = 0; q < RM; ++q) out[(f * a.fstride_out + (long)gl * a.nx) / 2 + j + q * (M / RM)] = w[q];
+          for (int q = 0; q < RM; ++q) gst<T>(out + (f * a.fstride_out + (long)gl * a.nx) / 2 + j + q * (M / RM), w[q]);
         }
       }
     }
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
 
   if (MODE == XM_FWD) {
     // physical pairs straight into forward stage 0
-    const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in);
+    const gptr<const cplx<T>> in = (gptr<const cplx<T>>)a.in;
     constexpr int NB = NFI * TLX * (M / RM), PER = (NB + NT - 1) / NT;
     cplx<T> v[PER][RM];
 #pragma unroll
@@ -360,7 +363,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
         const int gl = l0 + line;
 #pragma unroll
         for (int q = 0; q < RM; ++q)
-          v[p][q] = (gl < a.nlines) ? in[(f * a.fstride_in + (long)gl * a.nx) / 2 + j + q * (M / RM)] : mk<T>(T(0), T(0));
+          v[p][q] = (gl < a.nlines) ? gld<T>(in + (f * a.fstride_in + (long)gl * a.nx) / 2 + j + q * (M / RM)) : mk<T>(T(0), T(0));
         Bfly<RM, -1, T>::run(v[p]);
 #pragma unroll
         for (int q = 0; q < RM; ++q) L_(f, line)[lp<T>(j * RM + q)] = v[p][q];
@@ -371,8 +374,8 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
 
   if (MODE == XM_RHS) {
     // ---- D
-    const T* mask = reinterpret_cast<const T*>(a.mask);
-    const T* us = reinterpret_cast<const T*>(a.us);
+    const gptr<const T> mask = (gptr<const T>)a.mask;
+    const gptr<const T> us = (gptr<const T>)a.us;
     double umax2 = 0.0, fs[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
     constexpr int NB = TLX * (M / RM), PER = (NB + NT - 1) / NT;
     cplx<T> nout[PER][NFO][RM];
@@ -475,7 +478,7 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
           r[0] = nanmax(r[0], red[w][0]);
           for (int c = 1; c < 4; ++c) r[c] += red[w][c];
         }
-        for (int c = 0; c < 4; ++c) a.partial[(long)blockIdx.x * 4 + c] = r[c];
+        for (int c = 0; c < 4; ++c) ((gptr<double>)a.partial)[(long)blockIdx.x * 4 + c] = r[c];
       }
     } else {
       __syncthreads();
@@ -484,23 +487,23 @@ __global__ __launch_bounds__(NT) void x_pass_k(XPassArgs a) {
 
   // ---- E: remaining forward stages, then split into the nx/2+1 entries
   Stages<T, M, NFO * TLX, NT, PLAN_XF, -1, LSP, 1, NST, RM>::run(buf, twM);
-  cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out);
+  const gptr<cplx<T>> out = (gptr<cplx<T>>)a.out;
   const T half = T(0.5);
   for (int e = tid; e < TLX * NP; e += NT) {
     const int line = e / NP, k = e - line * NP, k2 = M - k;
     const int gl = l0 + line;
     if (gl >= a.nlines) continue;
-    const cplx<T> wa = twN[k], wb = twN[k2];
+    const cplx<T> wa = twv<T>(twN, k), wb = twv<T>(twN, k2);
 #pragma unroll
     for (int f = 0; f < NFO; ++f) {
       const cplx<T> zk = L_(f, line)[lp<T>(k)];
       const cplx<T> zm = L_(f, line)[lp<T>(k2 == M ? 0 : k2)];
       const cplx<T> s1 = add(zk, conj_(zm)), d1 = sub(zk, conj_(zm));
-      cplx<T>* o = out + f * a.fstride_out + (long)gl * a.prow;
-      o[k] = scl(sub(s1, ik_<T>(T(1), cmul(wa, d1))), half);
+      const gptr<cplx<T>> o = out + f * a.fstride_out + (long)gl * a.prow;
+      gst<T>(o + k, scl(sub(s1, ik_<T>(T(1), cmul(wa, d1))), half));
       if (k2 != k) {
         const cplx<T> s2 = add(zm, conj_(zk)), d2 = sub(zm, conj_(zk));
-        o[k2] = scl(sub(s2, ik_<T>(T(1), cmul(wb, d2))), half);
+        gst<T>(o + k2, scl(sub(s2, ik_<T>(T(1), cmul(wb, d2))), half));
       }
     }
   }

@@ -34,9 +34,6 @@ template <int L> struct TWPlan {
 template <int S, typename C> __device__ __forceinline__ C twm(C v, C w) {
   return (S < 0) ? cmul(v, w) : conjmul(v, w);
 }
-// twiddle i from an LDS table or from global memory
-template <typename T> __device__ __forceinline__ cplx<T> twv(const cplx<T>* p, int i) { return p[i]; }
-template <typename T> __device__ __forceinline__ cplx<T> twv(gptr<const cplx<T>> p, int i) { return gld<T>(p + i); }
 
 // Stage s: B = 8/R butterflies per lane, j = l + bG; input q of butterfly b
 // is register b + qB (position j + q L/R); outputs go to
