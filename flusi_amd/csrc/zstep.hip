@@ -308,15 +308,20 @@ template <typename T> __host__ __device__ constexpr int zb_wpe(int L) {
   return (sizeof(T) == 4 && L >= 512) ? 4 : 1;
 }
 
-// LDS of the block kernel: the transform tile, and a second tile where field
-// 0's transformed lines wait for the epilogue when both fit in a CU's 160 KB.
-// Parking them there instead of in registers keeps the kernel within its
-// VGPR budget (without it, the 16-line FP32 kernel at nz = 512 spills 21
-// VGPRs, about 1.3 GB of extra memory traffic per launch at 512^3).
+// LDS of the block kernel: the transform tile, a second tile where field 0's
+// transformed lines wait for the epilogue when both fit in a CU's 160 KB,
+// and the twiddle table.  Parking the lines there instead of in registers
+// keeps the kernel within its VGPR budget (without it, the 16-line FP32
+// kernel at nz = 512 spills 21 VGPRs, about 1.3 GB of extra memory traffic
+// per launch at 512^3).  The twiddles are read from LDS because the vector-
+// memory counter is in order: a stage waiting for a twiddle load from
+// global memory would also wait for the next field's operands, loaded just
+// before it.
 template <typename T, int L, int TL> struct ZBShape {
   static constexpr size_t tile = sizeof(cplx<T>) * TL * line_stride<T>(L);
-  static constexpr bool park = 2 * tile <= 160 * 1024;
-  static constexpr size_t smem = park ? 2 * tile : tile;
+  static constexpr size_t tws = sizeof(cplx<T>) * L;
+  static constexpr bool park = 2 * tile + tws <= 160 * 1024;
+  static constexpr size_t smem = (park ? 2 * tile : tile) + tws;
 };
 
 // EU > 0: the epilogue of at most EU positions is scheduled together (a
@@ -344,9 +349,16 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
   if (TL * sizeof(cplx<T>) < 128 && gridDim.x % 8 == 0) bx = (bx % 8) * (gridDim.x / 8) + bx / 8;
   const int b0 = bx * TL;
   const int kx = blockIdx.y;
-  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
-  const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
   constexpr bool PARK = ZBShape<T, L, TL>::park;
+  // twiddles: copied into LDS before the first field's operands are loaded
+  // (first read after the block barrier that ends stage 0)
+  cplx<T>* twl = buf + (PARK ? 2 : 1) * TL * LSP;
+  {
+    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
+    for (int i = tid; i < L; i += NT) twl[i] = twg[i];
+  }
+  const cplx<T>* tw = twl;
+  const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
   // park: field 0 by (ky line, kz), written and read back by the same thread
   cplx<T>* park = buf + TL * LSP;
   // r[f][p][q]: field f (0..2) at (ky = b0 + bb, kz = j + q L/RL) of
