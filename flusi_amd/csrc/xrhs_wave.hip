@@ -63,6 +63,32 @@ template <int M> struct WPlan {
   static constexpr __host__ __device__ int ns(int s) { return 1 << (2 * s); }
 };
 
+// lp(b + k) = lp(b) + lp(k) for the padded LDS accesses of the length-M
+// transform: every stage's write base b with offsets q Ns, the lane's read
+// base l with offsets q G and, with MIRROR, the mirrored base G - l with
+// offsets q G.  Where it holds, one address per lane and stage serves all
+// four elements through the LDS instruction's immediate offset.
+template <typename T, int M> constexpr bool split_ok(bool mirror) {
+  if constexpr (M == 48) {
+    return false;  // the radix-3 plan addresses its rows itself
+  } else {
+    constexpr int G = M / 4;
+    for (int l = 0; l < G; ++l) {
+      for (int q = 0; q < 4; ++q) {
+        if (lp<T>(l + q * G) != lp<T>(l) + lp<T>(q * G)) return false;
+        if (mirror && lp<T>(G - l + q * G) != lp<T>(G - l) + lp<T>(q * G)) return false;
+      }
+      for (int s = 0; s + 1 < WPlan<M>::NST; ++s) {
+        if (WPlan<M>::radix(s) != 4) return false;
+        const int Ns = WPlan<M>::ns(s), b = (l / Ns) * Ns * 4 + l % Ns;
+        for (int q = 0; q < 4; ++q)
+          if (lp<T>(b + q * Ns) != lp<T>(b) + lp<T>(q * Ns)) return false;
+      }
+    }
+    return true;
+  }
+}
+
 template <int S, typename C> __device__ __forceinline__ C twm(C v, C w) {
   return (S < 0) ? cmul(v, w) : conjmul(v, w);
 }
@@ -86,8 +112,14 @@ __device__ __forceinline__ void wstage(cplx<T> (&v)[4], const cplx<T>* tw, cplx<
     Bfly<4, S, T>::run(v);
     if (!last) {
       const int base = (l / Ns) * Ns * 4 + (l % Ns);
+      if constexpr (split_ok<T, M>(false)) {
+        cplx<T>* wr = ln + lp<T>(base);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) ln[lp<T>(base + q * Ns)] = v[q];
+        for (int q = 0; q < 4; ++q) wr[lp<T>(q * Ns)] = v[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ln[lp<T>(base + q * Ns)] = v[q];
+      }
     }
   } else {
     // butterflies j = l on (v0, v2) and j = l + G on (v1, v3)
@@ -108,8 +140,14 @@ __device__ __forceinline__ void wstage(cplx<T> (&v)[4], const cplx<T>* tw, cplx<
   }
   if (!last) {
     wave_sync();
+    if constexpr (split_ok<T, M>(false)) {
+      const cplx<T>* rd = ln + lp<T>(l);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = ln[lp<T>(l + q * G)];
+      for (int q = 0; q < 4; ++q) v[q] = rd[lp<T>(q * G)];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = ln[lp<T>(l + q * G)];
+    }
     wave_sync();
   }
 }
@@ -250,14 +288,29 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
     // one field at a time: keeps the scheduler from interleaving the
     // independent transforms (their temporaries would exhaust the registers)
     __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) ln[lp<T>(l + q * G)] = X[f][q];
-    if (l == 0) ln[lp<T>(M)] = XM[f];
-    wave_sync();
     cplx<T> xb[4];
+    if constexpr (split_ok<T, M>(true)) {
+      // positions l + qG and their mirrors M - l - qG = (G - l) + (3 - q) G
+      int lo = l;
+      asm volatile("" : "+v"(lo));
+      cplx<T>* rw = ln + lp<T>(lo);
+      const cplx<T>* mr = ln + lp<T>(G - lo);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) xb[q] = ln[lp<T>(M - l - q * G)];
-    wave_sync();
+      for (int q = 0; q < 4; ++q) rw[lp<T>(q * G)] = X[f][q];
+      if (lo == 0) ln[lp<T>(M)] = XM[f];
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xb[q] = mr[lp<T>((3 - q) * G)];
+      wave_sync();
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ln[lp<T>(l + q * G)] = X[f][q];
+      if (l == 0) ln[lp<T>(M)] = XM[f];
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xb[q] = ln[lp<T>(M - l - q * G)];
+      wave_sync();
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       cplx<T> xa = X[f][q], xm = xb[q];
@@ -272,6 +325,24 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
   double umax2 = 0.0, fs[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
   cplx<T> O[NFO][4];
   const int iy = live ? gl % a.ny : 0, iz = live ? gl / a.ny : 0;
+  // A wave whose points all lie outside the mask (most waves of a flow
+  // around a small body) has chi = 0 at every point, so that nl = u x w: it
+  // skips the FP64 penalisation term (which is 0 there for finite u; only
+  // the variant without statistics and sponge, whose other terms need every
+  // point).  The skipped code also frees the registers for a fourth wave
+  // per SIMD in the FP32 kernel.
+  bool masked = false;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) masked = masked || mk2[q].x != T(0) || mk2[q].y != T(0);
+  if (!SPONGE && !STATS && !__any(masked)) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const cplx<T> u0 = X[0][q], u1 = X[1][q], u2 = X[2][q], w0 = X[3][q], w1 = X[4][q], w2 = X[5][q];
+      O[0][q] = mk<T>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+      O[1][q] = mk<T>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+      O[2][q] = mk<T>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+    }
+  } else
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     // one position at a time (FP64 temporaries of several would not fit)
@@ -347,16 +418,31 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
   for (int c = 0; c < NFO; ++c) {
     __builtin_amdgcn_sched_barrier(0);
     wtransform<T, M, -1>(O[c], twl, ln, l);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) ln[lp<T>(l + q * G)] = O[c][q];
-    wave_sync();
     cplx<T> zm[4];
+    if constexpr (split_ok<T, M>(true)) {
+      // mirrors M - n of n = l + qG, through (G - l) + (3 - q) G; n = 0 pairs with 0
+      int lo = l;
+      asm volatile("" : "+v"(lo));
+      cplx<T>* rw = ln + lp<T>(lo);
+      const cplx<T>* mr = ln + lp<T>(G - lo);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int n = l + q * G;
-      zm[q] = ln[lp<T>(n == 0 ? 0 : M - n)];
+      for (int q = 0; q < 4; ++q) rw[lp<T>(q * G)] = O[c][q];
+      wave_sync();
+      zm[0] = (lo == 0 ? ln : mr + lp<T>(3 * G))[0];
+#pragma unroll
+      for (int q = 1; q < 4; ++q) zm[q] = mr[lp<T>((3 - q) * G)];
+      wave_sync();
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ln[lp<T>(l + q * G)] = O[c][q];
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int n = l + q * G;
+        zm[q] = ln[lp<T>(n == 0 ? 0 : M - n)];
+      }
+      wave_sync();
     }
-    wave_sync();
     if (live) {
       const gptr<cplx<T>> o = out + c * a.fstride_out + (long)gl * a.prow;
 #pragma unroll
