@@ -147,9 +147,10 @@ def accuracy_series(ini, steps, every):
 
 def hbm_probe(gib=4, reps=5):
     """Copy (bytes read + written), read-only and write-only bandwidth of
-    16-byte non-temporal streams over two `gib` GiB buffers on this device
-    (flusi_amd/libflusi_probe.so), in bytes/s; None when the probe library
-    is missing."""
+    16-byte non-temporal streams over two `gib` GiB buffers on this device,
+    and of a copy that reads 128-byte segments from a strided side (the tile
+    passes' pattern), from flusi_amd/libflusi_probe.so, in bytes/s; None when
+    the probe library is missing."""
     import ctypes
     path = os.path.join(ROOT, "flusi_amd", "libflusi_probe.so")
     if not os.path.exists(path):
@@ -157,11 +158,11 @@ def hbm_probe(gib=4, reps=5):
     lib = ctypes.CDLL(path)
     lib.flusi_probe_hbm.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     lib.flusi_probe_hbm.restype = ctypes.c_int
-    out = (ctypes.c_double * 3)()
+    out = (ctypes.c_double * 4)()
     rc = lib.flusi_probe_hbm(int(gib) << 30, int(reps), out)
     if rc != 0:
         return {"error": rc}
-    return {"copy": out[0], "read": out[1], "write": out[2], "buffer_gib": gib}
+    return {"copy": out[0], "read": out[1], "write": out[2], "seg_copy": out[3], "buffer_gib": gib}
 
 
 def host_info():
@@ -404,8 +405,8 @@ def main():
                     help="only run the HBM probe on 1 GiB buffers (counter calibration for HBM bytes)")
     args = ap.parse_args()
     if args.copy_calib:
-        # counter calibration: probe_read_k and probe_write_k stream exactly 1
-        # GiB per launch (2 launches each), probe_copy_k 1 GiB each way
+        # counter calibration: every launch of probe_read_k and probe_write_k
+        # streams exactly 1 GiB, of probe_copy_k 1 GiB each way
         print(json.dumps({"probe": hbm_probe(gib=1, reps=1), "bytes_per_launch": 1 << 30}), flush=True)
         return
     if args.gpus != 1:
@@ -448,7 +449,7 @@ def main():
         # the practical ceiling beside the nominal peak: a plain device copy
         out["roofline"]["copy_bw"] = probe["copy"] / 1e9
         out["roofline"]["frac_of_copy"] = ach / probe["copy"]
-        out["hbm_probe_GBps"] = {k: round(probe[k] / 1e9, 1) for k in ("copy", "read", "write")}
+        out["hbm_probe_GBps"] = {k: round(probe[k] / 1e9, 1) for k in ("copy", "read", "write", "seg_copy")}
     out["pass_ms_per_step"] = {k: v / args.steps for k, v in ms32.items() if v > 0}
     out["pass_achieved_GBps"] = {k: round(pb[k] / (ms32[k] / launches / 1e3) / 1e9, 1)
                                  for k in pb if ms32.get(k, 0.0) > 0}
