@@ -31,6 +31,27 @@ __host__ __device__ constexpr int mode_group(int mode) {
        : (mode == SM_EULER || mode == SM_AB2) ? MG_AB2
        : (mode == SM_RK2_1 || mode == SM_RK2_2) ? MG_RK2 : MG_RK4;
 }
+// MG = MG_FIXED + mode fixes the stepper mode itself at compile time, for the
+// operands the steppers always pass with it (fixed_mode_ok): the epilogue
+// then has no branch on the mode, the diffusion switch or the aliasing of its
+// operands, and a position's state loads are issued together.  With the mode
+// known only at run time, each of those loads sat under its own branch and
+// was waited for before the next one was issued.
+constexpr int MG_FIXED = 16;
+__host__ __device__ constexpr int mg_group(int mg) { return mg >= MG_FIXED ? mode_group(mg - MG_FIXED) : mg; }
+// the stepper mode of a launch of kernel group MG
+template <int MG> __device__ __forceinline__ int mg_mode(const ZStepArgs& a) {
+  if constexpr (MG >= MG_FIXED) return MG - MG_FIXED;
+  else return mg_group(MG) == MG_RHS ? SM_RHS_ONLY : (MG == MG_RK4 && a.mode < SM_RK4_1 ? SM_RK4_1 : a.mode);
+}
+template <int MG> __device__ __forceinline__ bool mg_final_stage(const ZStepArgs& a) {
+  if constexpr (mg_group(MG) == MG_RHS) {
+    return false;
+  } else {
+    const int m = mg_mode<MG>(a);
+    return m == SM_EULER || m == SM_AB2 || m == SM_RK2_2 || m == SM_RK4_4;
+  }
+}
 
 // A batch member's slab is reached through index offsets rather than shifted
 // pointers (the kernel argument pointers then stay rematerialisable, which
@@ -76,6 +97,11 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
   const double k2d = (kxv_d * kxv_d + kyv_d * kyv_d) + kzv_d * kzv_d;  // FP64: integrating factors
   const R kxv = (R)kxv_d, kyv = (R)kyv_d, kzv = (R)kzv_d, k2 = (R)k2d;
   const bool alias = ((kxt2 + a.kyt2[ky + mo.md]) + a.kzt2[kzi + mo.md]) >= 1.0;
+  constexpr int G = mg_group(MG);
+  constexpr bool FIX = MG >= MG_FIXED;
+  const int mode = mg_mode<MG>(a);
+  // explicit diffusion: RK4 only, where the fixed-mode kernels assume it
+  const bool ediff = FIX ? G == MG_RK4 : a.explicit_diffusion != 0;
   const R scale = (R)a.scale, nu = (R)a.nu, dt = (R)a.dt;
   R nr[3], ni[3];
 #pragma unroll
@@ -103,7 +129,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       if (a.meanflow_dynamic[c]) { nr[c] = (R)(-a.pen_sum[c + mo.md] * a.dynamic_scale); ni[c] = R(0); }
   }
   cplx<T> udv[3];  // the diffusion operand, reused when it is u itself (RK4 stage 1)
-  if (a.explicit_diffusion) {
+  if (ediff) {
     // rhs.f90:441-468
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -112,7 +138,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       ni[c] -= nu * k2 * (R)udv[c].y;
     }
   }
-  if constexpr (MG == MG_RHS) {
+  if constexpr (G == MG_RHS) {
     if (a.mode == SM_PRESSURE) {
       // rhs.f90:342-379 pressure: -i (k . nlk) / k^2, zero at k = 0
       R pr = R(0), pi = R(0);
@@ -135,15 +161,14 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
     nr[2] -= kzv * qr; ni[2] -= kzv * qi;
   }
   R unr[3] = {0, 0, 0}, uni[3] = {0, 0, 0};
-  const int mode = MG == MG_RHS ? SM_RHS_ONLY : (MG == MG_RK4 && a.mode < SM_RK4_1 ? SM_RK4_1 : a.mode);
   switch (mode) {
     case SM_RHS_ONLY:
-      if constexpr (MG == MG_RHS) {
+      if constexpr (G == MG_RHS) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) sts<T>(RO + c * a.fs + si, mk<T>((T)nr[c], (T)ni[c]));
       }
       break;
-    case SM_EULER: if constexpr (MG == MG_AB2) {
+    case SM_EULER: if constexpr (G == MG_AB2) {
       // fluid_time_step.f90:718-727
       const R E = (R)vis_factor(a.vis_dt_new, a.nu, k2d, a.dealias_on && alias);
 #pragma unroll
@@ -156,7 +181,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       }
       break;
     }
-    case SM_AB2: if constexpr (MG == MG_AB2) {
+    case SM_AB2: if constexpr (G == MG_AB2) {
       // fluid_time_step.f90:802-811
       const R E = (R)vis_factor(a.vis_dt_new, a.nu, k2d, a.dealias_on && alias);
       const R b10 = (R)a.b10, b11 = (R)a.b11;
@@ -171,7 +196,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       }
       break;
     }
-    case SM_RK2_1: if constexpr (MG == MG_RK2) {
+    case SM_RK2_1: if constexpr (G == MG_RK2) {
       // fluid_time_step.f90:554-575 (N0 weighted with the IF of the previous dt)
       const R Eo = (R)vis_factor(a.vis_dt_old, a.nu, k2d, a.dealias_on && alias);
       const R En = (R)vis_factor(a.vis_dt_new, a.nu, k2d, a.dealias_on && alias);
@@ -184,7 +209,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       }
       break;
     }
-    case SM_RK2_2: if constexpr (MG == MG_RK2) {
+    case SM_RK2_2: if constexpr (G == MG_RK2) {
       // fluid_time_step.f90:594-597
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -195,21 +220,21 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       }
       break;
     }
-    default: if constexpr (MG == MG_RK4) {
+    default: if constexpr (G == MG_RK4) {
       // RK4 (fluid_time_step.f90:636-669) with an accumulator register
       const R kill = alias ? R(0) : R(1);
       const R dt6 = (R)(a.dt / 6.0), dt3 = (R)(a.dt / 3.0);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const R kr = nr[c] * kill, ki = ni[c] * kill;
-        if (a.mode == SM_RK4_1) {
-          const cplx<T> u0 = (a.explicit_diffusion && a.udiff == a.u) ? udv[c] : lds<T>(U + c * a.fs + si);
+        if (mode == SM_RK4_1) {
+          const cplx<T> u0 = (FIX || (a.explicit_diffusion && a.udiff == a.u)) ? udv[c] : lds<T>(U + c * a.fs + si);
           sts<T>(ACC + c * a.fs + si, mk<T>((T)((R)u0.x + dt6 * kr), (T)((R)u0.y + dt6 * ki)));
           sts<T>(UST + c * a.fs + si, mk<T>((T)((R)u0.x + R(0.5) * dt * kr), (T)((R)u0.y + R(0.5) * dt * ki)));
-        } else if (a.mode == SM_RK4_2 || a.mode == SM_RK4_3) {
+        } else if (mode == SM_RK4_2 || mode == SM_RK4_3) {
           const cplx<T> ac = lds<T>(ACC + c * a.fs + si);
           R u0r, u0i;
-          if (a.mode == SM_RK4_2 && a.explicit_diffusion && a.udiff == a.ustage) {
+          if (mode == SM_RK4_2 && (FIX || (a.explicit_diffusion && a.udiff == a.ustage))) {
             // stage 1 left acc = u_n + dt/6 k1 and ustage = u_n + dt/2 k1 (the
             // diffusion operand held in udv), so u_n = acc + (acc - ustage)/2
             // without a read of u_n
@@ -219,7 +244,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
             const cplx<T> u0 = lds<T>(U + c * a.fs + si);
             u0r = (R)u0.x; u0i = (R)u0.y;
           }
-          const R cst = (a.mode == SM_RK4_2) ? R(0.5) * dt : dt;
+          const R cst = (mode == SM_RK4_2) ? R(0.5) * dt : dt;
           sts<T>(ACC + c * a.fs + si, mk<T>((T)((R)ac.x + dt3 * kr), (T)((R)ac.y + dt3 * ki)));
           sts<T>(UST + c * a.fs + si, mk<T>((T)(u0r + cst * kr), (T)(u0i + cst * ki)));
         } else {
@@ -238,7 +263,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       for (int c = 0; c < 3; ++c)
         if (a.set_mean[c]) { unr[c] = (R)a.mean_value[c]; uni[c] = R(0); }
     }
-    if (a.mode == SM_RK2_2 || a.mode == SM_RK4_4 || (kx == 0 && ky == 0 && kzi == 0)) {
+    if (mode == SM_RK2_2 || mode == SM_RK4_4 || (kx == 0 && ky == 0 && kzi == 0)) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) sts<T>(U + c * a.fs + si, mk<T>((T)unr[c], (T)uni[c]));
     }
@@ -497,7 +522,7 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
   const double kxt2 = a.kxt2[kx + mo.md];
   const double wE = (kx == 0 || kx == a.nx / 2) ? 0.5 : 1.0;
   double esum = 0.0, ecomp = 0.0;
-  const bool final_stage = MG != MG_RHS && (a.mode == SM_EULER || a.mode == SM_AB2 || a.mode == SM_RK2_2 || a.mode == SM_RK4_4);
+  const bool final_stage = mg_final_stage<MG>(a);
 #pragma unroll
   for (int p = 0; p < PERL; ++p)
 #pragma unroll
@@ -652,7 +677,7 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
   const double kxt2 = a.kxt2[kx + mo.md];
   const double wE = (kx == 0 || kx == a.nx / 2) ? 0.5 : 1.0;
   double esum = 0.0, ecomp = 0.0;
-  const bool final_stage = MG != MG_RHS && (a.mode == SM_EULER || a.mode == SM_AB2 || a.mode == SM_RK2_2 || a.mode == SM_RK4_4);
+  const bool final_stage = mg_final_stage<MG>(a);
   const int ky = b0 + line;
   if (ky < a.ny) {
 #pragma unroll
@@ -709,7 +734,7 @@ static void run_z(const ZStepArgs& a, hipStream_t s) {
       return;
     }
   }
-  if constexpr (L >= 256) {
+  if constexpr (L >= 256 && MG < MG_FIXED) {
     switch (a.eu) {
       case 1: run_zb<T, L, SP, MG, 1>(a, s); return;
       case 2: run_zb<T, L, SP, MG, 2>(a, s); return;
@@ -719,8 +744,36 @@ static void run_z(const ZStepArgs& a, hipStream_t s) {
   }
   run_zb<T, L, SP, MG, 0>(a, s);
 }
+// The operands Solver::fluid_timestep passes with each stepper mode
+// (explicit diffusion with RK4 only; RK4's diffusion operand is u_n in stage
+// 1 and the stage state afterwards), which the fixed-mode kernels assume.
+static bool fixed_mode_ok(const ZStepArgs& a) {
+  switch (a.mode) {
+    case SM_RK4_1: return a.explicit_diffusion && a.udiff == a.u;
+    case SM_RK4_2: case SM_RK4_3: case SM_RK4_4: return a.explicit_diffusion && a.udiff == a.ustage;
+    case SM_EULER: case SM_AB2: case SM_RK2_1: case SM_RK2_2: return !a.explicit_diffusion;
+    default: return false;
+  }
+}
 template <typename T, int L, bool SP>
 static void run_z_mg(const ZStepArgs& a, hipStream_t s) {
+  // fixed-mode kernels: without the sponge, nz >= 64, the default epilogue
+  // grouping (the other variants keep the run-time mode)
+  if constexpr (!SP && L >= 64) {
+    if (fixed_mode_ok(a) && (L < 256 || a.eu == 0 || (L == 256 && a.wave))) {
+      switch (a.mode) {
+        case SM_EULER: run_z<T, L, SP, MG_FIXED + SM_EULER>(a, s); return;
+        case SM_AB2: run_z<T, L, SP, MG_FIXED + SM_AB2>(a, s); return;
+        case SM_RK2_1: run_z<T, L, SP, MG_FIXED + SM_RK2_1>(a, s); return;
+        case SM_RK2_2: run_z<T, L, SP, MG_FIXED + SM_RK2_2>(a, s); return;
+        case SM_RK4_1: run_z<T, L, SP, MG_FIXED + SM_RK4_1>(a, s); return;
+        case SM_RK4_2: run_z<T, L, SP, MG_FIXED + SM_RK4_2>(a, s); return;
+        case SM_RK4_3: run_z<T, L, SP, MG_FIXED + SM_RK4_3>(a, s); return;
+        case SM_RK4_4: run_z<T, L, SP, MG_FIXED + SM_RK4_4>(a, s); return;
+        default: break;
+      }
+    }
+  }
   switch (mode_group(a.mode)) {
     case MG_RHS: run_z<T, L, SP, MG_RHS>(a, s); break;
     case MG_AB2: run_z<T, L, SP, MG_AB2>(a, s); break;
