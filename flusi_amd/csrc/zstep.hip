@@ -63,6 +63,36 @@ template <typename T> __device__ __forceinline__ MOff member_off(const ZStepArgs
   return MOff{b / (long)sizeof(cplx<T>), b / (long)sizeof(double)};
 }
 
+// Operands of the epilogue at one position for a fixed-mode kernel, loaded
+// one position ahead (zpreload), so that a position's loads are in flight
+// while the previous position computes and before its stores are issued
+// (the vector-memory counter is in order: a wait for loads issued after a
+// store also waits for the store).  st[0] = the diffusion operand (RK4) or
+// u_n; st[1] = acc (RK4 stages 2-4), n_{-1} (AB2) or n_0 (RK2 stage 2);
+// st[2] = u_n (RK4 stage 3).
+template <typename T> struct ZPre {
+  cplx<T> st[3][3];
+  double ky, kz, kyt2, kzt2;
+};
+template <typename T, int MG>
+__device__ __forceinline__ void zpreload(const ZStepArgs& a, const MOff& mo, int kx, int ky, int kzi, ZPre<T>& pr) {
+  constexpr int m = MG - MG_FIXED;
+  const long si = ((long)kx * a.ny + ky) * a.nz + kzi + mo.mc;
+  pr.ky = a.ky[ky + mo.md];
+  pr.kz = a.kz[kzi + mo.md];
+  pr.kyt2 = a.kyt2[ky + mo.md];
+  pr.kzt2 = a.kzt2[kzi + mo.md];
+  const cplx<T>* s0 = reinterpret_cast<const cplx<T>*>(m >= SM_RK4_1 ? a.udiff : a.u);
+  const cplx<T>* s1 = reinterpret_cast<const cplx<T>*>(m >= SM_RK4_2 ? a.acc : (m == SM_AB2 ? a.r1 : a.r0));
+  constexpr bool has1 = m >= SM_RK4_2 || m == SM_AB2 || m == SM_RK2_2;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    pr.st[0][c] = lds<T>(s0 + c * a.fs + si);
+    if constexpr (has1) pr.st[1][c] = lds<T>(s1 + c * a.fs + si);
+    if constexpr (m == SM_RK4_3) pr.st[2][c] = lds<T>(reinterpret_cast<const cplx<T>*>(a.u) + c * a.fs + si);
+  }
+}
+
 // Epilogue at one spectral point (kx, ky, kz): rv[f] is field f of the
 // z-transformed right-hand side (unscaled).  Sponge curl (sponge.f90:22-83),
 // dynamic mean flow, explicit diffusion (rhs.f90:441-468), projection
@@ -83,7 +113,8 @@ template <typename T> __device__ __forceinline__ MOff member_off(const ZStepArgs
 template <typename T, bool SPONGE, int MG, bool FOLDED = false>
 __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, int kx, int ky, int kzi, const cplx<T>* rv,
                                           double kxv_d, double kxt2, double wE,
-                                          bool final_stage, double& esum, double& ecomp) {
+                                          bool final_stage, double& esum, double& ecomp,
+                                          const ZPre<T>* pre = nullptr) {
   using R = T;
   cplx<T>* U = reinterpret_cast<cplx<T>*>(a.u);
   cplx<T>* UST = reinterpret_cast<cplx<T>*>(a.ustage);
@@ -93,12 +124,17 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
   const cplx<T>* RG1 = reinterpret_cast<const cplx<T>*>(a.r1);
   cplx<T>* RO = reinterpret_cast<cplx<T>*>(a.rhs_out);
   const long si = ((long)kx * a.ny + ky) * a.nz + kzi + mo.mc;
-  const double kyv_d = a.ky[ky + mo.md], kzv_d = a.kz[kzi + mo.md];
-  const double k2d = (kxv_d * kxv_d + kyv_d * kyv_d) + kzv_d * kzv_d;  // FP64: integrating factors
-  const R kxv = (R)kxv_d, kyv = (R)kyv_d, kzv = (R)kzv_d, k2 = (R)k2d;
-  const bool alias = ((kxt2 + a.kyt2[ky + mo.md]) + a.kzt2[kzi + mo.md]) >= 1.0;
   constexpr int G = mg_group(MG);
   constexpr bool FIX = MG >= MG_FIXED;
+  // operands: preloaded (fixed-mode kernels) or read here; slot as in ZPre
+  auto opnd = [&](const void* base, int slot, int c) -> cplx<T> {
+    if constexpr (FIX) return pre->st[slot][c];
+    else return lds<T>(reinterpret_cast<const cplx<T>*>(base) + c * a.fs + si);
+  };
+  const double kyv_d = FIX ? pre->ky : a.ky[ky + mo.md], kzv_d = FIX ? pre->kz : a.kz[kzi + mo.md];
+  const double k2d = (kxv_d * kxv_d + kyv_d * kyv_d) + kzv_d * kzv_d;  // FP64: integrating factors
+  const R kxv = (R)kxv_d, kyv = (R)kyv_d, kzv = (R)kzv_d, k2 = (R)k2d;
+  const bool alias = ((kxt2 + (FIX ? pre->kyt2 : a.kyt2[ky + mo.md])) + (FIX ? pre->kzt2 : a.kzt2[kzi + mo.md])) >= 1.0;
   const int mode = mg_mode<MG>(a);
   // explicit diffusion: RK4 only, where the fixed-mode kernels assume it
   const bool ediff = FIX ? G == MG_RK4 : a.explicit_diffusion != 0;
@@ -133,7 +169,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
     // rhs.f90:441-468
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      udv[c] = lds<T>(UD + c * a.fs + si);
+      udv[c] = opnd(UD, 0, c);
       nr[c] -= nu * k2 * (R)udv[c].x;
       ni[c] -= nu * k2 * (R)udv[c].y;
     }
@@ -173,7 +209,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       const R E = (R)vis_factor(a.vis_dt_new, a.nu, k2d, a.dealias_on && alias);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const cplx<T> u0 = lds<T>(U + c * a.fs + si);
+        const cplx<T> u0 = opnd(U, 0, c);
         unr[c] = ((R)u0.x + dt * nr[c]) * E;
         uni[c] = ((R)u0.y + dt * ni[c]) * E;
         sts<T>(U + c * a.fs + si, mk<T>((T)unr[c], (T)uni[c]));
@@ -187,8 +223,8 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       const R b10 = (R)a.b10, b11 = (R)a.b11;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const cplx<T> u0 = lds<T>(U + c * a.fs + si);
-        const cplx<T> n1 = lds<T>(RG1 + c * a.fs + si);
+        const cplx<T> u0 = opnd(U, 0, c);
+        const cplx<T> n1 = opnd(RG1, 1, c);
         unr[c] = ((R)u0.x + b10 * nr[c] + b11 * (R)n1.x) * E;
         uni[c] = ((R)u0.y + b10 * ni[c] + b11 * (R)n1.y) * E;
         sts<T>(U + c * a.fs + si, mk<T>((T)unr[c], (T)uni[c]));
@@ -203,7 +239,7 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const R n0r = nr[c] * Eo, n0i = ni[c] * Eo;
-        const cplx<T> u0 = lds<T>(U + c * a.fs + si);
+        const cplx<T> u0 = opnd(U, 0, c);
         sts<T>(RG0 + c * a.fs + si, mk<T>((T)n0r, (T)n0i));
         sts<T>(U + c * a.fs + si, mk<T>((T)((R)u0.x * En + dt * n0r), (T)((R)u0.y * En + dt * n0i)));
       }
@@ -213,8 +249,8 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       // fluid_time_step.f90:594-597
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const cplx<T> u0 = lds<T>(U + c * a.fs + si);
-        const cplx<T> n0 = lds<T>(RG0 + c * a.fs + si);
+        const cplx<T> u0 = opnd(U, 0, c);
+        const cplx<T> n0 = opnd(RG0, 1, c);
         unr[c] = (R)u0.x + R(0.5) * dt * (-(R)n0.x + nr[c]);
         uni[c] = (R)u0.y + R(0.5) * dt * (-(R)n0.y + ni[c]);
       }
@@ -228,11 +264,11 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
       for (int c = 0; c < 3; ++c) {
         const R kr = nr[c] * kill, ki = ni[c] * kill;
         if (mode == SM_RK4_1) {
-          const cplx<T> u0 = (FIX || (a.explicit_diffusion && a.udiff == a.u)) ? udv[c] : lds<T>(U + c * a.fs + si);
+          const cplx<T> u0 = (FIX || (a.explicit_diffusion && a.udiff == a.u)) ? udv[c] : opnd(U, 0, c);
           sts<T>(ACC + c * a.fs + si, mk<T>((T)((R)u0.x + dt6 * kr), (T)((R)u0.y + dt6 * ki)));
           sts<T>(UST + c * a.fs + si, mk<T>((T)((R)u0.x + R(0.5) * dt * kr), (T)((R)u0.y + R(0.5) * dt * ki)));
         } else if (mode == SM_RK4_2 || mode == SM_RK4_3) {
-          const cplx<T> ac = lds<T>(ACC + c * a.fs + si);
+          const cplx<T> ac = opnd(ACC, 1, c);
           R u0r, u0i;
           if (mode == SM_RK4_2 && (FIX || (a.explicit_diffusion && a.udiff == a.ustage))) {
             // stage 1 left acc = u_n + dt/6 k1 and ustage = u_n + dt/2 k1 (the
@@ -241,14 +277,14 @@ __device__ __forceinline__ void zepilogue(const ZStepArgs& a, const MOff& mo, in
             u0r = (R)ac.x + R(0.5) * ((R)ac.x - (R)udv[c].x);
             u0i = (R)ac.y + R(0.5) * ((R)ac.y - (R)udv[c].y);
           } else {
-            const cplx<T> u0 = lds<T>(U + c * a.fs + si);
+            const cplx<T> u0 = opnd(U, 2, c);
             u0r = (R)u0.x; u0i = (R)u0.y;
           }
           const R cst = (mode == SM_RK4_2) ? R(0.5) * dt : dt;
           sts<T>(ACC + c * a.fs + si, mk<T>((T)((R)ac.x + dt3 * kr), (T)((R)ac.y + dt3 * ki)));
           sts<T>(UST + c * a.fs + si, mk<T>((T)(u0r + cst * kr), (T)(u0i + cst * ki)));
         } else {
-          const cplx<T> ac = lds<T>(ACC + c * a.fs + si);
+          const cplx<T> ac = opnd(ACC, 1, c);
           unr[c] = ((R)ac.x + dt6 * kr) * kill;
           uni[c] = ((R)ac.y + dt6 * ki) * kill;
         }
@@ -375,6 +411,10 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
   const int b0 = bx * TL;
   const int kx = blockIdx.y;
   constexpr bool PARK = ZBShape<T, L, TL>::park;
+  // fixed-mode kernels load each epilogue position's operands one position
+  // ahead (ZPre); without the sponge, field 2 then waits in the transform tile
+  constexpr bool AHEAD = MG >= MG_FIXED;
+  constexpr bool TILE2 = AHEAD && !SPONGE;
   // twiddles: copied into LDS before the first field's operands are loaded
   // (first read after the block barrier that ends stage 0)
   cplx<T>* twl = buf + (PARK ? 2 : 1) * TL * LSP;
@@ -443,6 +483,20 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
         }
       }
       __syncthreads();
+    }
+    if constexpr (TILE2) {
+      // field 2 waits in the transform tile, free after the last barrier;
+      // the registers go to the next position's operands
+#pragma unroll
+      for (int p = 0; p < PERL; ++p) {
+        const int beta = tid + p * NT;
+        if (NBL % NT == 0 || beta < NBL) {
+          const int bb = beta / (L / RL), j = beta - bb * (L / RL);
+#pragma unroll
+          for (int q = 0; q < RL; ++q) buf[bb * LSP + lp<T>(j + q * (L / RL))] = r[2][p][q];
+        }
+      }
+      asm volatile("" ::: "memory");  // read back below, not forwarded from registers
     }
   } else {
     // Transform number i as straight-line code per i (with a loop left to
@@ -523,22 +577,39 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
   const double wE = (kx == 0 || kx == a.nx / 2) ? 0.5 : 1.0;
   double esum = 0.0, ecomp = 0.0;
   const bool final_stage = mg_final_stage<MG>(a);
+  // position k = p RL + q of this thread: ky line bb, kz index kzi; false
+  // where it lies beyond the tile or the grid
+  constexpr int NPOS = PERL * RL;
+  auto at = [&](int k, int& bb, int& kzi) -> bool {
+    const int beta = tid + (k / RL) * NT;
+    bb = beta / (L / RL);
+    kzi = beta - bb * (L / RL) + (k % RL) * (L / RL);
+    return (NBL % NT == 0 || beta < NBL) && b0 + bb < a.ny;
+  };
+  // operands of position k, read at a valid line for every thread (a
+  // position that does not exist reads line b0 and is not used), so that the
+  // loads need no branch
+  ZPre<T> pre[2];
+  auto preload = [&](int k, ZPre<T>& pr) {
+    int bb, kzi;
+    const bool ok = at(k, bb, kzi);
+    zpreload<T, MG>(a, mo, kx, ok ? b0 + bb : b0, ok ? kzi : 0, pr);
+  };
+  if constexpr (AHEAD) preload(0, pre[0]);
 #pragma unroll
-  for (int p = 0; p < PERL; ++p)
-#pragma unroll
-  for (int q = 0; q < RL; ++q) {
-    const int beta = tid + p * NT;
-    if (!(NBL % NT == 0 || beta < NBL)) continue;
-    const int bb = beta / (L / RL), j = beta - bb * (L / RL);
-    const int kzi = j + q * (L / RL);
-    const int ky = b0 + bb;
-    if (ky >= a.ny) continue;
+  for (int k = 0; k < NPOS; ++k) {
+    if constexpr (AHEAD)
+      if (k + 1 < NPOS) preload(k + 1, pre[(k + 1) & 1]);
+    int bb, kzi;
+    if (!at(k, bb, kzi)) continue;
+    const int p = k / RL, q = k % RL;
     cplx<T> rv[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) rv[f] = (PARK && f == 0) ? park[bb * LSP + lp<T>(kzi)] : r[f][p][q];
-    zepilogue<T, false, MG, SPONGE>(a, mo, kx, ky, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
+    for (int f = 0; f < 3; ++f)
+      rv[f] = (PARK && f == 0) ? park[bb * LSP + lp<T>(kzi)] : ((TILE2 && f == 2) ? buf[bb * LSP + lp<T>(kzi)] : r[f][p][q]);
+    zepilogue<T, false, MG, SPONGE>(a, mo, kx, b0 + bb, kzi, rv, kxv, kxt2, wE, final_stage, esum, ecomp, &pre[k & 1]);
     if constexpr (EU > 0)
-      if ((p * RL + q + 1) % EU == 0) __builtin_amdgcn_sched_barrier(0);
+      if ((k + 1) % EU == 0) __builtin_amdgcn_sched_barrier(0);
   }
   if (final_stage && a.energy_partial) {
     double v = wave_sum(esum + ecomp);
@@ -680,12 +751,18 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
   const bool final_stage = mg_final_stage<MG>(a);
   const int ky = b0 + line;
   if (ky < a.ny) {
+    // fixed-mode kernels: operands loaded one position ahead (ZPre)
+    constexpr bool AHEAD = MG >= MG_FIXED;
+    ZPre<T> pre[2];
+    if constexpr (AHEAD) zpreload<T, MG>(a, mo, kx, ky, l, pre[0]);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
+      if constexpr (AHEAD)
+        if (q + 1 < 8) zpreload<T, MG>(a, mo, kx, ky, l + (q + 1) * G, pre[(q + 1) & 1]);
       cplx<T> rv[3];
 #pragma unroll
       for (int f = 0; f < 3; ++f) rv[f] = r[f][q];
-      zepilogue<T, false, MG, SPONGE>(a, mo, kx, ky, l + q * G, rv, kxv, kxt2, wE, final_stage, esum, ecomp);
+      zepilogue<T, false, MG, SPONGE>(a, mo, kx, ky, l + q * G, rv, kxv, kxt2, wE, final_stage, esum, ecomp, &pre[q & 1]);
     }
   }
   if (final_stage && a.energy_partial) {
