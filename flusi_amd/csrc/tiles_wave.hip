@@ -47,7 +47,12 @@ __device__ __forceinline__ cplx<T> apply_op(int op, double kscale, int aa, cplx<
   return x;
 }
 
-template <typename T, int L, int S, bool CONTIG_IN>
+// SF: every store of the grid lands on a row below the store bound (the host
+// checks it), so that the stores need no guard.  Loads and stores then carry
+// no branch, and the wait for the next job's prefetched operands does not
+// have to cover the current job's stores (with a guard, the compiler cannot
+// count the stores in flight and waits for all of them).
+template <typename T, int L, int S, bool CONTIG_IN, bool SF>
 __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs a) {
   using Sh = TWShape<T, L>;
   constexpr int TL = Sh::TL, NT = Sh::NT, LS = Sh::LS, K = Sh::K;
@@ -76,6 +81,9 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
 
   // operands: CONTIG_IN - the lane's 8 positions of its own line;
   // otherwise - the thread's K tile elements (bb = e % TL, a = e / TL)
+  // (lines beyond nb read line nb - 1, unused: zeros are taken where the
+  // operands are consumed, so that neither the loads nor a select right
+  // after them needs a branch or a wait)
   cplx<T> r0[8], r1[8];
   auto load = [&](int jb, int which) {
     // global-address-space views (see gptr): the prefetched loads then do
@@ -83,10 +91,10 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
     const gptr<const cplx<T>> in = (gptr<const cplx<T>>)mshift(which ? a.jobs[jb].in1 : a.jobs[jb].in0, moff);
     if (CONTIG_IN) {
       const bool ok = b0 + line < a.nb;
-      const long base = c * a.sin.c + (long)(b0 + line) * a.sin.b + l;
+      const long base = c * a.sin.c + (long)(ok ? b0 + line : a.nb - 1) * a.sin.b + l;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const cplx<T> x = ok ? lds<T>(in + base + q * G) : mk<T>(T(0), T(0));
+        const cplx<T> x = lds<T>(in + base + q * G);
         if (which) r1[q] = x; else r0[q] = x;
       }
     } else {
@@ -94,16 +102,22 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
       for (int k = 0; k < K; ++k) {
         const int e = tid + k * NT, bb = e % TL, aa = e / TL;
         const bool ok = b0 + bb < a.nb;
-        const cplx<T> x = ok ? lds<T>(in + c * a.sin.c + (long)(b0 + bb) + (long)aa * a.sin.a) : mk<T>(T(0), T(0));
+        const cplx<T> x = lds<T>(in + c * a.sin.c + (long)(ok ? b0 + bb : a.nb - 1) + (long)aa * a.sin.a);
         if (which) r1[k] = x; else r0[k] = x;
       }
     }
   };
 
-  load(0, 0);
+  // a job's operands (both inputs) are loaded while the previous job is
+  // transformed and before its stores: a wait for them then never covers
+  // stores in flight
+  auto load_job = [&](int jb) {
+    load(jb, 0);
+    if (a.jobs[jb].in1) load(jb, 1);
+  };
+  load_job(0);
   for (int jb = 0; jb < a.njobs; ++jb) {
     const int nout = a.jobs[jb].out2 ? 2 : 1;
-    if (a.jobs[jb].in1) load(jb, 1);
     for (int o = 0; o < nout; ++o) {
       const int op = o ? a.jobs[jb].op2 : a.jobs[jb].op;
       const gptr<cplx<T>> out = (gptr<cplx<T>>)mshift(o ? a.jobs[jb].out2 : a.jobs[jb].out, moff);
@@ -112,7 +126,11 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
       if (CONTIG_IN) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = apply_op<T, L>(op, a.kscale, l + q * G, r0[q], r1[q]);
-        if (pf) load(jb + 1, 0);
+        if (b0 + line >= a.nb) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = mk<T>(T(0), T(0));
+        }
+        if (pf) load_job(jb + 1);
         twfft<T, L, S>(v, tw, ln, l);
         // turn the tile: natural order into the line's row, then 128-byte
         // segments of TL lines out
@@ -123,22 +141,23 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
         for (int k = 0; k < K; ++k) {
           const int e = tid + k * NT, bb = e % TL, aa = e / TL;
           const cplx<T> x = tile[bb * LS + lp<T>(aa)];
-          if (b0 + bb < nbs) sts<T>(out + c * a.sout.c + (long)(b0 + bb) + (long)aa * a.sout.a, mk<T>(x.x * sc, x.y * sc));
+          if (SF || b0 + bb < nbs) sts<T>(out + c * a.sout.c + (long)(b0 + bb) + (long)aa * a.sout.a, mk<T>(x.x * sc, x.y * sc));
         }
         __syncthreads();
       } else {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const int e = tid + k * NT, bb = e % TL, aa = e / TL;
-          tile[bb * LS + lp<T>(aa)] = apply_op<T, L>(op, a.kscale, aa, r0[k], r1[k]);
+          const cplx<T> x = apply_op<T, L>(op, a.kscale, aa, r0[k], r1[k]);
+          tile[bb * LS + lp<T>(aa)] = b0 + bb < a.nb ? x : mk<T>(T(0), T(0));
         }
         __syncthreads();
-        if (pf) load(jb + 1, 0);
+        if (pf) load_job(jb + 1);
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = ln[lp<T>(l + q * G)];
         wave_sync();
         twfft<T, L, S>(v, tw, ln, l);
-        if (b0 + line < a.nb) {
+        if (SF || b0 + line < a.nb) {
           const long base = c * a.sout.c + (long)(b0 + line) * a.sout.b + l;
 #pragma unroll
           for (int q = 0; q < 8; ++q) sts<T>(out + base + q * G, mk<T>(v[q].x * sc, v[q].y * sc));
@@ -149,13 +168,23 @@ __global__ __launch_bounds__((TWShape<T, L>::NT)) void tile_wave_k(TilePassArgs 
   }
 }
 
-template <typename T, int L, int S, bool CI>
-void run_tw(const TilePassArgs& a, hipStream_t s) {
+template <typename T, int L, int S, bool CI, bool SF>
+void run_tw_sf(const TilePassArgs& a, hipStream_t s) {
   using Sh = TWShape<T, L>;
   static bool init = false;
-  if (!init) { set_smem(tile_wave_k<T, L, S, CI>, Sh::smem); init = true; }
+  if (!init) { set_smem(tile_wave_k<T, L, S, CI, SF>, Sh::smem); init = true; }
   dim3 grid((a.nb + Sh::TL - 1) / Sh::TL, a.nc, batch_n(a.bat));
-  hipLaunchKernelGGL((tile_wave_k<T, L, S, CI>), grid, dim3(Sh::NT), Sh::smem, s, a);
+  hipLaunchKernelGGL((tile_wave_k<T, L, S, CI, SF>), grid, dim3(Sh::NT), Sh::smem, s, a);
+}
+template <typename T, int L, int S, bool CI>
+void run_tw(const TilePassArgs& a, hipStream_t s) {
+  // stores unguarded when the grid's last tile ends at or below the store
+  // bound (nb_store rows for the segment-side stores, nb for the line stores)
+  const int tl = TWShape<T, L>::TL;
+  const long rows = (long)((a.nb + tl - 1) / tl) * tl;
+  const long bound = CI ? (a.nb_store > a.nb ? a.nb_store : a.nb) : a.nb;
+  if (rows <= bound) run_tw_sf<T, L, S, CI, true>(a, s);
+  else run_tw_sf<T, L, S, CI, false>(a, s);
 }
 
 template <typename T, int L>
