@@ -415,14 +415,18 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
   // ahead (ZPre); without the sponge, field 2 then waits in the transform tile
   constexpr bool AHEAD = MG >= MG_FIXED;
   constexpr bool TILE2 = AHEAD && !SPONGE;
-  // twiddles: copied into LDS before the first field's operands are loaded
-  // (first read after the block barrier that ends stage 0)
+  // twiddles in LDS (first read after the block barrier that ends stage 0):
+  // loaded into registers after the first field's operands are requested
+  // and written before that barrier, so that the two memory latencies overlap
   cplx<T>* twl = buf + (PARK ? 2 : 1) * TL * LSP;
-  {
-    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
-    for (int i = tid; i < L; i += NT) twl[i] = twg[i];
-  }
   const cplx<T>* tw = twl;
+  constexpr int NTW = (L + NT - 1) / NT;
+  cplx<T> twr[NTW];
+  auto put_tw = [&] {
+#pragma unroll
+    for (int k = 0; k < NTW; ++k)
+      if (tid + k * NT < L) twl[tid + k * NT] = twr[k];
+  };
   const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
   // park: field 0 by (ky line, kz), written and read back by the same thread
   cplx<T>* park = buf + TL * LSP;
@@ -452,6 +456,12 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
     }
   };
   loadq(fld(0));
+  {
+    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
+#pragma unroll
+    for (int k = 0; k < NTW; ++k)
+      if (tid + k * NT < L) twr[k] = twg[tid + k * NT];
+  }
   if constexpr (!SPONGE) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -465,6 +475,7 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
           for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
         }
       }
+      if (f == 0) put_tw();
       __syncthreads();
       if (f + 1 < NF) loadq(f + 1);
       Stages<T, L, TL, NT, PLAN_TILE, -1, LSP, 1, NSTG - 1, R0>::run(buf, tw);
@@ -518,6 +529,7 @@ __global__ __launch_bounds__(NT, zb_wpe<T>(L)) void zstep_k(ZStepArgs a) {
           for (int q = 0; q < R0; ++q) buf[bb * LSP + lp<T>(j * R0 + q)] = v[p][q];
         }
       }
+      if constexpr (i == 0) put_tw();
       __syncthreads();
       if constexpr (i + 1 < NF) loadq(fld(i + 1));
       Stages<T, L, TL, NT, PLAN_TILE, -1, LSP, 1, NSTG - 1, R0>::run(buf, tw, tl);
@@ -655,15 +667,17 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
   cplx<T>* ln = tile + line * LS;
   const int b0 = blockIdx.x * TL;
   const int kx = blockIdx.y;
-  const cplx<T>* tw;
-  if constexpr (Sh::TWL) {
-    cplx<T>* twl = tile + TL * LS;
-    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
-    for (int i = tid; i < L; i += NT) twl[i] = twg[i];
-    tw = twl;
-  } else {
-    tw = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
-  }
+  // twiddles: the block's LDS copy (made after the first field's loads are
+  // issued, so that the two memory latencies overlap; first read after the
+  // first block barrier), or global memory where it does not fit
+  const cplx<T>* tw = Sh::TWL ? tile + TL * LS : reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
+  auto copy_tw = [&] {
+    if constexpr (Sh::TWL) {
+      cplx<T>* twl = tile + TL * LS;
+      const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw) + mo.mc;
+      for (int i = tid; i < L; i += NT) twl[i] = twg[i];
+    }
+  };
   const cplx<T>* nq = reinterpret_cast<const cplx<T>*>(a.nq);
   cplx<T> x[K];
   auto load = [&](int f) {
@@ -677,6 +691,7 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
   cplx<T> r[3][8];
   if constexpr (!SPONGE) {
     load(0);
+    copy_tw();
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
 #pragma unroll
@@ -705,6 +720,7 @@ __global__ __launch_bounds__((ZWShape<T, L>::NT)) void zstep_wave_k(ZStepArgs a)
       }
     };
     loadt(fld(0), tid);
+    copy_tw();
     auto xform = [&](auto ic) {
       constexpr int i = decltype(ic)::value;
       constexpr int f = fld(i);
