@@ -75,6 +75,8 @@ struct XPassArgs {
   // RHS extras
   const void* mask;    // N reals (element type of the pass)
   const void* us;      // 3N reals, component stride N
+  const uint8_t* mflag;  // per x line (ny*nz): 0 where the line's mask is all zero; null: read every line
+  const void* mzero;   // nx zeros of the element type, read instead of a line whose flag is 0
   long npts;           // N
   double eps_inv;      // 1/eta
   int sponge;          // 0 none, 1 cavity, 2 outlet_x
@@ -159,6 +161,10 @@ int x_pass_blocks(int nx, int nlines, bool fp64);
 template <typename T> void launch_x_rhs_wave(const XPassArgs& a, hipStream_t s);
 bool x_rhs_wave_supported(int nx);
 int x_rhs_wave_blocks(int nx, int nlines);
+// per x line (iy, iz) of [y0, y1] x [z0, z1]: flag[iz ny + iy] = 1 where any
+// mask value of the line is nonzero, else 0 (xrhs_wave.hip)
+template <typename T>
+void launch_mask_flags(const void* mask, uint8_t* flag, int nx, int ny, int y0, int y1, int z0, int z1, hipStream_t s);
 // energy partial sums one z-step launch writes per member (the block kernel
 // or, wave = true, the in-wave kernel)
 int zstep_blocks(int nz, int ny, int nxh, bool fp64, bool wave);

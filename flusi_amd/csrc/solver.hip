@@ -254,6 +254,12 @@ class Solver : public SolverBase {
   void* mask = nullptr;
   void* us = nullptr;
   uint8_t* color = nullptr;
+  // per x line: may the mask be nonzero (the x pass reads only those lines'
+  // mask); valid after an unbatched, uncaptured insect mask update, which
+  // recomputes the lines it redrew
+  uint8_t* mflag = nullptr;
+  void* mzero = nullptr;
+  bool mflag_valid = false;
   void* phys = nullptr;     // 7 real fields for diagnostics (u, w, div)
   double* tab = nullptr;    // kx ky kz kxt2 kyt2 kzt2
   double *kx_d, *ky_d, *kz_d, *kxt2_d, *kyt2_d, *kzt2_d;
@@ -573,6 +579,7 @@ class Solver : public SolverBase {
     XPassArgs a = xargs();
     a.in = P; a.out = P; a.fstride_in = pc; a.fstride_out = pc; a.nf = 6;
     a.mask = mask; a.us = us; a.eps_inv = 1.0 / p.eps;
+    a.mflag = mflag_valid ? mflag : nullptr; a.mzero = mzero;
     a.sponge = sponge_code();
     a.sponge_thickness = p.sponge_thickness;
     a.sponge_eps_inv = a.sponge ? 1.0 / p.eps_sponge : 0.0;
@@ -1279,6 +1286,10 @@ class Solver : public SolverBase {
       throw FlusiError(4453, "insects.f90::DrawInsect: the parameters iMoving or iPenalization are wrong.");
     const int M = batch_n(bat);
     auto member = [&](int m) -> Solver<T>& { return M > 1 ? *batch[m] : *this; };
+    // the mask changes below; the line flags are brought up to date only by
+    // the unbatched, uncaptured insect update
+    const bool flags_were_valid = mflag_valid;
+    mflag_valid = false;
     timed(TM_MASK, [&] {
       if (p.iPenalization == 1) {
         if (p.iMask == "sphere" || p.iMask == "Sphere") {
@@ -1327,12 +1338,14 @@ class Solver : public SolverBase {
             dev = d;
           }
           launch_insect<T>(geom, recs[0], dev, mask, us, color, stream, bat);
+          if (M == 1 && !capture) update_mask_flags(recs[0], flags_were_valid);
         } else if (p.iMask == "none" || p.iMask == "empty" || p.iMask == "no") {
         } else {
           throw FlusiError(3333, "create_mask_fsi(): unkown mask function iMask");
         }
       }
       if (p.iCavity != "no" && p.iPenalization == 1) {
+        mflag_valid = false;
         CavityArgs c{};
         c.size = p.cavity_size;
         if (p.iCavity == "yes") { c.ux = c.uy = c.uz = 0.0; }
@@ -1345,6 +1358,22 @@ class Solver : public SolverBase {
       }
     });
     kcheck("create_mask");
+  }
+
+  // Line flags after an insect update: insect_k writes only inside its region
+  // box, so lines outside it keep their flags; the lines crossing the box
+  // (or every line, when the flags were not valid before) are recomputed
+  // from the mask.
+  void update_mask_flags(const InsectArgs& r, bool were_valid) {
+    if (!mflag) {
+      mflag = (uint8_t*)dalloc((size_t)ny * nz);
+      mzero = dalloc(sizeof(T) * nx);
+      launch_zero(mzero, sizeof(T) * nx, stream);
+    }
+    const bool empty = r.reg_lo[0] > r.reg_hi[0] || r.reg_lo[1] > r.reg_hi[1] || r.reg_lo[2] > r.reg_hi[2];
+    if (!were_valid) launch_mask_flags<T>(mask, mflag, nx, ny, 0, ny - 1, 0, nz - 1, stream);
+    else if (!empty) launch_mask_flags<T>(mask, mflag, nx, ny, r.reg_lo[1], r.reg_hi[1], r.reg_lo[2], r.reg_hi[2], stream);
+    mflag_valid = true;
   }
 
   // full: visit the whole grid (a batched or captured launch has one fixed

@@ -261,11 +261,17 @@ __global__ __launch_bounds__(XW_NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ?
     XM[f] = gld<T>(p + M);
   }
 
-  // mask at the physical points 2n, 2n+1 (n = l + qG), read early
+  // mask at the physical points 2n, 2n+1 (n = l + qG), read early; a line
+  // whose flag says its mask is all zero (most lines of a flow around a
+  // small body) reads the shared zero row instead, which stays in cache
   cplx<T> mk2[4];
+  {
+    const long ml = live ? gl : 0;
+    const bool zero_line = a.mflag && ((gptr<const uint8_t>)a.mflag)[ml] == 0;
+    const gptr<const T> mrow = zero_line ? (gptr<const T>)a.mzero : mask + ml * a.nx;
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-    mk2[q] = gld<T>(reinterpret_cast<gptr<const cplx<T>>>(mask + (long)(live ? gl : 0) * a.nx + 2 * (l + q * G)));
+    for (int q = 0; q < 4; ++q) mk2[q] = gld<T>(reinterpret_cast<gptr<const cplx<T>>>(mrow + 2 * (l + q * G)));
+  }
 
   __syncthreads();  // twiddle tables (the only block-wide wait besides the statistics)
 
@@ -494,7 +500,35 @@ void run_wave(const XPassArgs& a, hipStream_t s) {
   }
 }
 
+// One wavefront per x line of the region: the line's flag is 1 where any of
+// its nx mask values is nonzero (NaN included).  The wavefront returns as a
+// whole (w depends on the wave only).
+template <typename T>
+__global__ __launch_bounds__(256) void mask_flags_k(const T* __restrict__ mask, uint8_t* __restrict__ flag, int nx, int ny,
+                                                    int y0, int ey, int z0, long nl) {
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nl) return;
+  const int lane = threadIdx.x & 63;
+  const long line = (long)(z0 + (int)(w / ey)) * ny + (y0 + (int)(w % ey));
+  const T* row = mask + line * nx;
+  bool nzv = false;
+  for (int i = lane; i < nx; i += 64) nzv = nzv || !(row[i] == T(0));
+  const bool any = __any(nzv);
+  if (lane == 0) flag[line] = any ? 1 : 0;
+}
+
 }  // namespace
+
+template <typename T>
+void launch_mask_flags(const void* mask, uint8_t* flag, int nx, int ny, int y0, int y1, int z0, int z1, hipStream_t s) {
+  if (y1 < y0 || z1 < z0) return;
+  const int ey = y1 - y0 + 1;
+  const long nl = (long)ey * (z1 - z0 + 1);
+  hipLaunchKernelGGL(mask_flags_k<T>, dim3((unsigned)((nl + 3) / 4)), dim3(256), 0, s, (const T*)mask, flag, nx, ny, y0,
+                     ey, z0, nl);
+}
+template void launch_mask_flags<float>(const void*, uint8_t*, int, int, int, int, int, int, hipStream_t);
+template void launch_mask_flags<double>(const void*, uint8_t*, int, int, int, int, int, int, hipStream_t);
 
 bool x_rhs_wave_supported(int nx) { return nx == 64 || nx == 96 || nx == 128 || nx == 256 || nx == 512; }
 
