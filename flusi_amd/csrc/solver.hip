@@ -260,6 +260,7 @@ class Solver : public SolverBase {
   uint8_t* mflag = nullptr;
   void* mzero = nullptr;
   bool mflag_valid = false;
+  bool mflag_on = true;     // FLUSI_MASK_FLAGS=0: every line's mask is read
   void* phys = nullptr;     // 7 real fields for diagnostics (u, w, div)
   double* tab = nullptr;    // kx ky kz kxt2 kyt2 kzt2
   double *kx_d, *ky_d, *kz_d, *kxt2_d, *kyt2_d, *kzt2_d;
@@ -411,6 +412,7 @@ class Solver : public SolverBase {
       if (const char* e = getenv("FLUSI_TILE_WAVE")) twave = atoi(e) != 0;
       if (const char* e = getenv("FLUSI_ZWAVE")) zwave = atoi(e) != 0;
       if (const char* e = getenv("FLUSI_ZSTEP_EU")) zeu = atoi(e);
+      if (const char* e = getenv("FLUSI_MASK_FLAGS")) mflag_on = atoi(e) != 0;
       if (const char* e = getenv("FLUSI_TILE_PREFETCH")) tpf = atoi(e) != 0;
       if (const char* e = getenv("FLUSI_XWAVE_WPE")) xwpe = atoi(e);
       xwave = x_rhs_wave_supported(nx);
@@ -1338,7 +1340,7 @@ class Solver : public SolverBase {
             dev = d;
           }
           launch_insect<T>(geom, recs[0], dev, mask, us, color, stream, bat);
-          if (M == 1 && !capture) update_mask_flags(recs[0], flags_were_valid);
+          if (M == 1 && !capture && mflag_on) update_mask_flags(recs[0], flags_were_valid);
         } else if (p.iMask == "none" || p.iMask == "empty" || p.iMask == "no") {
         } else {
           throw FlusiError(3333, "create_mask_fsi(): unkown mask function iMask");

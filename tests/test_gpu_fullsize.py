@@ -133,6 +133,45 @@ def test_insect_rk4_steps_match_oracle(tmp_path, shape, prec, tol):
     assert ncol == 0 if prec == flusi_amd.FP64 else ncol <= 1e-6 * col.size + 2, ncol
 
 
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("shape", [(64, 64, 512), (512, 64, 64)])
+@pytest.mark.parametrize("prec,tol", [(flusi_amd.FP64, 1e-16), (flusi_amd.FP32_MIXED, 2e-8)])
+def test_insect_rk4_fixed_mode_and_line_flags_match_general_path(tmp_path, monkeypatch, shape, prec, tol):
+    """The same three RK4 insect steps through the z step's fixed-mode kernels
+    (one per RK4 stage, operands loaded one position ahead) and the x pass's
+    mask line flags (the default), and through the run-time-mode z-step
+    kernel (FLUSI_ZSTEP_EU=1 selects it at nz >= 256) with every mask line
+    read (FLUSI_MASK_FLAGS=0): the same arithmetic.  With nz < 256 both runs
+    take the same z-step kernels and only the line flags differ, which must
+    not change a bit; at nz = 512 the two z-step kernels agree to round-off
+    (measured 9.8e-18 FP64 and 4.1e-9 FP32, the bounds about five times
+    that)."""
+    ini = _insect_ini(tmp_path, *shape)
+    print(f"[fixed vs general path] grid {shape} precision {prec}", flush=True)
+    sim = fo.Simulation(ini, workers=WORKERS)
+    sim.init_fields()
+    u0 = _lowmode_state(sim, seed=23, amp=0.3)
+
+    def run():
+        with flusi_amd.Flusi(ini, prec) as s:
+            s.init_fields()
+            s.set_uk(u0)
+            for _ in range(3):
+                s.step()
+            return s.get_uk(), s.get_mask()[0]
+
+    uk_fast, m_fast = run()
+    monkeypatch.setenv("FLUSI_ZSTEP_EU", "1")
+    monkeypatch.setenv("FLUSI_MASK_FLAGS", "0")
+    uk_gen, m_gen = run()
+    err = rel_l2(uk_fast, uk_gen)
+    print(f"fixed-mode + line flags vs general path, grid {shape} precision {prec}: rel L2 {err:.3e}", flush=True)
+    assert np.array_equal(m_fast, m_gen)
+    if shape[2] < 256:
+        assert err == 0.0
+    assert err <= tol
+
+
 # ------------------------------------------------------------- adjoint at 128^3
 
 
