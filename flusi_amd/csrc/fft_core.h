@@ -142,7 +142,8 @@ template <int S, typename T> struct Bfly<8, S, T> {
 template <typename T, int L, int R, int S, int Ns, typename TWP>
 __device__ __forceinline__ void twiddle_bfly(cplx<T>* v, int j, TWP tw) {
   if (Ns > 1) {
-    const int step = (j % Ns) * (L / (Ns * R));
+    // (unsigned: j mod Ns without the sign handling of int %)
+    const int step = (int)((unsigned)j % Ns) * (L / (Ns * R));
 #pragma unroll
     for (int q = 1; q < R; ++q) {
       const cplx<T> w = twv<T>(tw, q * step);
@@ -152,7 +153,8 @@ __device__ __forceinline__ void twiddle_bfly(cplx<T>* v, int j, TWP tw) {
   Bfly<R, S, T>::run(v);
 }
 template <int R, int Ns> __device__ __forceinline__ int stage_out(int j, int q) {
-  return (j / Ns) * Ns * R + (j % Ns) + q * Ns;
+  const unsigned u = (unsigned)j;
+  return (int)(u / Ns) * Ns * R + (int)(u % Ns) + q * Ns;
 }
 
 // One Stockham stage over NL lines (line i at i*LSP) held in LDS.  tid is
@@ -169,7 +171,7 @@ __device__ __forceinline__ void lds_stage(cplx<T>* buf, TWP tw, int tid) {
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
     if (NB % NT == 0 || b < NB) {
-      const int line = b / (L / R), j = b - line * (L / R);
+      const int line = (int)((unsigned)b / (L / R)), j = b - line * (L / R);
       const cplx<T>* src = buf + line * LSP;
 #pragma unroll
       for (int q = 0; q < R; ++q) v[p][q] = src[lp<T>(j + q * (L / R))];
@@ -181,7 +183,7 @@ __device__ __forceinline__ void lds_stage(cplx<T>* buf, TWP tw, int tid) {
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
     if (NB % NT == 0 || b < NB) {
-      const int line = b / (L / R), j = b - line * (L / R);
+      const int line = (int)((unsigned)b / (L / R)), j = b - line * (L / R);
       cplx<T>* dst = buf + line * LSP;
 #pragma unroll
       for (int q = 0; q < R; ++q) dst[lp<T>(stage_out<R, Ns>(j, q))] = v[p][q];

@@ -45,12 +45,14 @@ __device__ __forceinline__ void twstage(cplx<T> (&v)[8], TWP tw, cplx<T>* ln, in
   constexpr bool last = (s == TWPlan<L>::NST - 1);
 #pragma unroll
   for (int b = 0; b < B; ++b) {
-    const int j = l + b * G;
+    // unsigned: j % Ns and j / Ns become a mask and a shift (l is an opaque
+    // copy whose sign the compiler cannot see)
+    const unsigned j = (unsigned)l + b * G;
     cplx<T> x[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) x[q] = v[b + q * B];
     if (s > 0) {
-      const int step = (j % Ns) * (L / (Ns * R));
+      const int step = (int)(j % Ns) * (L / (Ns * R));
 #pragma unroll
       for (int q = 1; q < R; ++q) x[q] = twm<S>(x[q], twv<T>(tw, q * step));
     }
@@ -59,7 +61,7 @@ __device__ __forceinline__ void twstage(cplx<T> (&v)[8], TWP tw, cplx<T>* ln, in
 #pragma unroll
       for (int q = 0; q < R; ++q) v[b + q * B] = x[q];
     } else {
-      const int base = (j / Ns) * Ns * R + (j % Ns);
+      const int base = (int)(j / Ns) * Ns * R + (int)(j % Ns);
 #pragma unroll
       for (int q = 0; q < R; ++q) ln[lp<T>(base + q * Ns)] = x[q];
     }

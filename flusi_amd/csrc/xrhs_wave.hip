@@ -100,18 +100,22 @@ template <int S, typename C> __device__ __forceinline__ C twm(C v, C w) {
 // tw: the block's LDS copy of exp(-2 pi i m / M), m < M.
 template <typename T, int M, int S, int s>
 __device__ __forceinline__ void wstage(cplx<T> (&v)[4], const cplx<T>* tw, cplx<T>* ln, int l) {
+  // index arithmetic on the unsigned lane index: l % Ns and l / Ns are then
+  // a mask and a shift (on a signed int the compiler has to allow l < 0,
+  // which costs four instructions each)
+  const unsigned ul = (unsigned)l;
   constexpr int G = WPlan<M>::G;
   constexpr int R = WPlan<M>::radix(s), Ns = WPlan<M>::ns(s);
   constexpr bool last = (s == WPlan<M>::NST - 1);
   if (R == 4) {
     if (s > 0) {
-      const int step = (l % Ns) * (M / (Ns * 4));
+      const int step = (int)(ul % Ns) * (M / (Ns * 4));
 #pragma unroll
       for (int q = 1; q < 4; ++q) v[q] = twm<S>(v[q], tw[q * step]);
     }
     Bfly<4, S, T>::run(v);
     if (!last) {
-      const int base = (l / Ns) * Ns * 4 + (l % Ns);
+      const int base = (int)(ul / Ns) * Ns * 4 + (int)(ul % Ns);
       if constexpr (split_ok<T, M>(false)) {
         cplx<T>* wr = ln + lp<T>(base);
 #pragma unroll
@@ -124,14 +128,14 @@ __device__ __forceinline__ void wstage(cplx<T> (&v)[4], const cplx<T>* tw, cplx<
   } else {
     // butterflies j = l on (v0, v2) and j = l + G on (v1, v3)
     if (s > 0) {
-      v[2] = twm<S>(v[2], tw[(l % Ns) * (M / (Ns * 2))]);
-      v[3] = twm<S>(v[3], tw[((l + G) % Ns) * (M / (Ns * 2))]);
+      v[2] = twm<S>(v[2], tw[(int)(ul % Ns) * (M / (Ns * 2))]);
+      v[3] = twm<S>(v[3], tw[(int)((ul + G) % Ns) * (M / (Ns * 2))]);
     }
     cplx<T> a0[2] = {v[0], v[2]}, a1[2] = {v[1], v[3]};
     Bfly<2, S, T>::run(a0);
     Bfly<2, S, T>::run(a1);
     if (!last) {
-      const int b0 = (l / Ns) * Ns * 2 + (l % Ns), b1 = ((l + G) / Ns) * Ns * 2 + ((l + G) % Ns);
+      const int b0 = (int)(ul / Ns) * Ns * 2 + (int)(ul % Ns), b1 = (int)((ul + G) / Ns) * Ns * 2 + (int)((ul + G) % Ns);
       ln[lp<T>(b0)] = a0[0]; ln[lp<T>(b0 + Ns)] = a0[1];
       ln[lp<T>(b1)] = a1[0]; ln[lp<T>(b1 + Ns)] = a1[1];
     } else {
